@@ -1,0 +1,188 @@
+/*
+ * tbfgpu.h -- C ABI of the MI355X pressure-projection engine for TBFsolver.
+ *
+ * A drop-in for the reference's two pressure-solver plug-ins, selected in the
+ * reference at compile time by POIS_OPT (src/Makefile:3,17-24):
+ *   FAST_MODE: type(fastPoissonSolver) / fastPoissonSolverCTOR / solveFPS
+ *   MG_MODE:   type(pcg) / pcgCTOR / solvePCG (multigrid-preconditioned CG)
+ * plus the device-resident pieces around them (RHS, coefficient, projection,
+ * continuity error) and the input mirrors used by the benchmarks.
+ *
+ * Conventions
+ *  - Fields are FP64.  Host-array entry points take Fortran arrays with their
+ *    lower/upper bounds (lb/ub per axis, as lbound/ubound return them); only
+ *    the interior 1..n is read, and outputs get their halo filled the way the
+ *    reference's updateBoundaries does for the channel (periodic x and z,
+ *    homogeneous Neumann in y).
+ *  - Device entry points (*_dev) take device pointers to interior-only arrays,
+ *    Fortran order (i fastest), and a HIP stream handle passed as void*
+ *    (NULL = default stream).  They are asynchronous unless stated otherwise.
+ *  - Every entry point returns a status code; tbfg_last_error() gives the
+ *    message of the calling thread's last failure.  Nothing aborts.
+ *  - A handle may be used by one host thread at a time; distinct handles are
+ *    independent.
+ */
+#ifndef TBFGPU_H
+#define TBFGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference aborts through mpiABORT, errorHandler.f90:30-50) */
+#define TBFG_OK 0
+#define TBFG_EINVAL 1        /* bad arguments, grid or hierarchy              */
+#define TBFG_ENOMEM 2        /* device or host allocation failed             */
+#define TBFG_ENOTCONVERGED 3 /* max iterations reached; result still written */
+#define TBFG_ENUMERIC 4      /* non-finite values in input or iteration      */
+#define TBFG_EDEVICE 5       /* HIP runtime error                            */
+
+#define TBFG_PREC_FP64 0  /* all arithmetic and storage in FP64                  */
+#define TBFG_PREC_MIXED 1 /* FP32 inner solver inside an FP64 outer iteration    */
+
+/* Grid: x and z periodic, y walls.  stretch[a] = 1 selects the hyperbolic
+ * tangent distribution of grid.f90:293-330 with factor gamma (the reference
+ * fixes gamma = 1.4 at grid.f90:27). */
+typedef struct tbfg_grid_desc {
+    int32_t n[3];
+    double L[3];
+    int32_t stretch[3];
+    double gamma;
+} tbfg_grid_desc;
+
+/* ---------------------------------------------------------------- library */
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* tbfg_last_error(void);
+/* Library version string. */
+const char* tbfg_version(void);
+/* Number of visible devices (0 on a host without a GPU; no device is initialised
+ * by the host-only entry points below). */
+int tbfg_device_count(void);
+
+/* ---------------------------------------------------------- FFT solver (FAST) */
+
+typedef struct tbfg_fps tbfg_fps;
+
+typedef struct tbfg_fps_opts {
+    int32_t precision;     /* TBFG_PREC_FP64 or TBFG_PREC_MIXED                       */
+    int32_t refine_sweeps; /* MIXED only: FP64 iterative-refinement sweeps (default 2) */
+} tbfg_fps_opts;
+
+/* Replaces fastPoissonSolverCTOR (src/fastPoissonSolver/fastPoissonSolver.f90:123-156):
+ * eigenvalues, y coefficients and device work buffers.  x and z must be uniform,
+ * nx and nz in {2^a 3^b}, nx even, n <= 1024 along x and z, ny <= 1024. */
+int tbfg_fps_create(const tbfg_grid_desc* grid, const tbfg_fps_opts* opts, tbfg_fps** out);
+
+/* Replaces solveFPS (fastPoissonSolver.f90:337-384): solves lap(p) = s with the
+ * FFT-x/z + tridiagonal-y operator, removes the volume-weighted mean of p
+ * (:388-423) and fills p's halo (updateBoundaries).  s and p are host arrays
+ * with Fortran bounds s_lb..s_ub, p_lb..p_ub (interior 1..n inside). Blocking. */
+int tbfg_fps_solve(tbfg_fps* h, const double* s, const int64_t s_lb[3], const int64_t s_ub[3],
+                   double* p, const int64_t p_lb[3], const int64_t p_ub[3]);
+
+/* Device-resident variant: s_dev, p_dev interior-only (nx*ny*nz doubles). */
+int tbfg_fps_solve_dev(tbfg_fps* h, const double* s_dev, double* p_dev, void* stream);
+
+/* The constant-coefficient operator the FFT solver inverts (modified wavenumbers
+ * in x and z, the 3-point y operator of fastPoissonSolver.f90:255-276): q = L p. */
+int tbfg_fps_apply_operator_dev(tbfg_fps* h, const double* p_dev, double* q_dev, void* stream);
+
+/* Device bytes held by the handle. */
+int64_t tbfg_fps_footprint(const tbfg_fps* h);
+
+int tbfg_fps_destroy(tbfg_fps* h);
+
+/* -------------------------------------------------------- PCG + multigrid (MG) */
+
+typedef struct tbfg_pcg tbfg_pcg;
+
+typedef struct tbfg_pcg_params {
+    int32_t n_levels;      /* nLevels     (channel_18/specs/pcg_solver:21)           */
+    int32_t n_pre;         /* nPreSweep                                              */
+    int32_t n_post;        /* nPostSweep                                             */
+    int32_t max_iter;      /* maxIterPCG                                             */
+    double tol;            /* tolPCG on ||r||_2 / ||q||_2                             */
+    int32_t mg_cycles;     /* V-cycles per preconditioner application (default 1)    */
+    int32_t precision;     /* TBFG_PREC_FP64 (reference algorithm) or TBFG_PREC_MIXED */
+    int32_t coarse_cycles; /* inner cycles approximating the exact coarsest solve    */
+    int32_t flexible;      /* 0 Fletcher-Reeves, 1 Polak-Ribiere, -1 by precision    */
+} tbfg_pcg_params;
+
+/* Fills *p with the reference's defaults for the channel case. */
+void tbfg_pcg_default_params(tbfg_pcg_params* p);
+
+/* Replaces pcgCTOR + multiGridCTOR (src/pcg/pcg.f90:62-99, src/multiGrid/multiGrid.f90:72-112):
+ * builds the level hierarchy and allocates the device work space.  Every axis
+ * must be divisible by 2^(n_levels-1) (multiGrid.f90:116-145). */
+int tbfg_pcg_create(const tbfg_grid_desc* grid, const tbfg_pcg_params* prm, tbfg_pcg** out);
+
+/* Replaces solvePCG (pcg.f90:103-140): psi (in: warm start, out: solution with halo
+ * filled), beta = alpha*dt/rho and q = div(u*) are host Fortran arrays.  *iters
+ * receives the number of direction updates (the reference's iter_ is iters+1 on
+ * convergence, pcg.f90:401), *rel_res the final ||r||/||q||.  Returns
+ * TBFG_ENOTCONVERGED (with psi written) when max_iter is reached.  Blocking. */
+int tbfg_pcg_solve(tbfg_pcg* h, double* psi, const int64_t psi_lb[3], const int64_t psi_ub[3],
+                   const double* beta, const int64_t beta_lb[3], const int64_t beta_ub[3],
+                   const double* q, const int64_t q_lb[3], const int64_t q_ub[3], int32_t* iters,
+                   double* rel_res);
+
+/* Device-resident variant (interior-only arrays).  Synchronises the stream to
+ * test convergence. */
+int tbfg_pcg_solve_dev(tbfg_pcg* h, double* psi_dev, const double* beta_dev, const double* q_dev,
+                       int32_t* iters, double* rel_res, void* stream);
+
+/* Applies the handle's multigrid preconditioner once: z = M r (device arrays). */
+int tbfg_pcg_precondition_dev(tbfg_pcg* h, const double* beta_dev, const double* r_dev, double* z_dev,
+                              void* stream);
+
+/* Device bytes held by the handle (for admission by HBM footprint). */
+int64_t tbfg_pcg_footprint(const tbfg_pcg* h);
+
+int tbfg_pcg_destroy(tbfg_pcg* h);
+
+/* ------------------------------------------------------- stencils and fields */
+
+/* computeQv (pcg.f90:144-183): q = A p with A = div(beta grad .), channel ghosts. */
+int tbfg_apply_operator_dev(const tbfg_grid_desc* grid, const double* beta_dev, const double* p_dev,
+                            double* q_dev, void* stream);
+
+/* MG computeSource (poissonEqn.f90:155-190) / auxiliaryRoutines.f90:66-79:
+ * div = du/dx + dv/dy + dw/dz.  ux,uz: nx*ny*nz faces (face i between cells i
+ * and i+1, periodic); uy: nx*(ny+1)*nz faces, j = 0..ny (walls at 0 and ny). */
+int tbfg_divergence_dev(const tbfg_grid_desc* grid, const double* ux, const double* uy, const double* uz,
+                        double* div, void* stream);
+
+/* computeBeta (poissonEqn.f90:311-342): beta = alpha_dt / rho. */
+int tbfg_beta_dev(const tbfg_grid_desc* grid, double alpha_dt, const double* rho, double* beta,
+                  void* stream);
+
+/* MG makeVelocityDivFree -> addPressureGrad (momentumEqn.f90:795-869,980-993). */
+int tbfg_project_mg_dev(const tbfg_grid_desc* grid, double alpha_dt, const double* rho, const double* psi,
+                        double* ux, double* uy, double* uz, void* stream);
+
+/* computeContinuityError (auxiliaryRoutines.f90:42-91): returns max|div u| (not
+ * multiplied by dt) in *out.  Blocking. */
+int tbfg_max_divergence_dev(const tbfg_grid_desc* grid, const double* ux, const double* uy, const double* uz,
+                            double* out, void* stream);
+
+/* --------------------------------------------------------- input mirrors (host) */
+
+/* Seeded staggered velocity, u,v,w ~ N(0,1) on unique faces, v = 0 on the walls
+ * (counter-based generator: splitmix64 -> uniform -> Box-Muller). Host arrays. */
+int tbfg_make_velocity(const tbfg_grid_desc* grid, uint64_t seed, double* ux, double* uy, double* uz);
+
+/* Non-overlapping spheres (periodic x/z distance), centres uniform in
+ * [0,Lx) x [R, Ly-R] x [0,Lz) by rejection; VOF per init_Bubble_vf
+ * (initialConditions.f90:192-398) with nref^3 sub-sampling of interface cells,
+ * two cell->vertex->cell smoothing passes (VOF.f90:235-262) and
+ * rho = rho_g*cs + rho_l*(1-cs) (VOF.f90:222-223).  centres (3*nb doubles) may be
+ * NULL.  Host arrays. */
+int tbfg_make_bubble_density(const tbfg_grid_desc* grid, int32_t nb, double diameter, uint64_t seed,
+                             int32_t nref, double rho_l, double rho_g, double* rho, double* centres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

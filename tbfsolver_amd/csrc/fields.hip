@@ -1,0 +1,223 @@
+// fields.hip -- device kernels around the pressure solve: divergence of the
+// staggered velocity (RHS of MG_MODE and the continuity error), the PCG
+// coefficient beta = alpha dt / rho, and the MG_MODE projection.
+//
+// Staggered layout: ux, uz hold nx*ny*nz faces (face i between cells i and i+1,
+// face 0 == face nx by periodicity); uy holds nx*(ny+1)*nz faces, j = 0..ny,
+// with the walls at j = 0 and j = ny.
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "fields.hpp"
+
+namespace tbfg {
+
+// per-axis metrics: df[i-1] = df(i), dcn[i-1] = dc(i+1), i = 1..n
+struct FieldGeo {
+    int nx, ny, nz;
+    const double* df[3];
+    const double* dcn[3];
+};
+
+struct FieldGeoOwner {
+    DevBuf buf;
+    FieldGeo view{};
+};
+
+static std::mutex g_geo_mu;
+static std::map<std::string, std::unique_ptr<FieldGeoOwner>> g_geo_cache;
+
+static int get_geo(const tbfg_grid_desc* g, FieldGeo& out)
+{
+    TBFG_TRY(validate_grid(g));
+    int dev = 0;
+    TBFG_HIP(hipGetDevice(&dev));
+    std::string key(reinterpret_cast<const char*>(g), sizeof(*g));
+    key += std::to_string(dev);
+    std::lock_guard<std::mutex> lk(g_geo_mu);
+    auto it = g_geo_cache.find(key);
+    if (it == g_geo_cache.end()) {
+        HostLevel L = make_level(g, 0);
+        std::vector<double> h;
+        size_t off[3][2];
+        for (int a = 0; a < 3; ++a) {
+            off[a][0] = h.size();
+            for (int i = 1; i <= L.n[a]; ++i) h.push_back(L.ax[a].df(i));
+            off[a][1] = h.size();
+            for (int i = 1; i <= L.n[a]; ++i) h.push_back(L.ax[a].dc(i + 1));
+        }
+        std::unique_ptr<FieldGeoOwner> o(new FieldGeoOwner());
+        TBFG_TRY(upload(o->buf, h.data(), h.size() * 8));
+        o->view.nx = L.n[0];
+        o->view.ny = L.n[1];
+        o->view.nz = L.n[2];
+        for (int a = 0; a < 3; ++a) {
+            o->view.df[a] = o->buf.as<double>() + off[a][0];
+            o->view.dcn[a] = o->buf.as<double>() + off[a][1];
+        }
+        it = g_geo_cache.emplace(key, std::move(o)).first;
+    }
+    out = it->second->view;
+    return TBFG_OK;
+}
+
+__device__ __forceinline__ double div_cell(const FieldGeo& G, const double* __restrict__ ux,
+                                           const double* __restrict__ uy, const double* __restrict__ uz, int i, int j,
+                                           int k)
+{
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const size_t c = i + (size_t)nx * (j + (size_t)ny * k);
+    const size_t w = (i > 0 ? i - 1 : nx - 1) + (size_t)nx * (j + (size_t)ny * k);
+    const size_t b = i + (size_t)nx * (j + (size_t)ny * (k > 0 ? k - 1 : nz - 1));
+    const size_t yt = i + (size_t)nx * ((j + 1) + (size_t)(ny + 1) * k);
+    const size_t yb = i + (size_t)nx * (j + (size_t)(ny + 1) * k);
+    const double dx = (ux[c] - ux[w]) / G.df[0][i];
+    const double dy = (uy[yt] - uy[yb]) / G.df[1][j];
+    const double dz = (uz[c] - uz[b]) / G.df[2][k];
+    return dx + dy + dz;
+}
+
+// poissonEqn.f90:176-188 / auxiliaryRoutines.f90:66-79
+__global__ void k_divergence(FieldGeo G, const double* __restrict__ ux, const double* __restrict__ uy,
+                             const double* __restrict__ uz, double* __restrict__ div)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    if (i >= G.nx) return;
+    div[i + (size_t)G.nx * (j + (size_t)G.ny * k)] = div_cell(G, ux, uy, uz, i, j, k);
+}
+
+// auxiliaryRoutines.f90:60-82: max |div u| (the reference prints it times dt)
+__global__ void k_maxdiv(FieldGeo G, const double* __restrict__ ux, const double* __restrict__ uy,
+                         const double* __restrict__ uz, unsigned long long* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    double v = 0.0;
+    if (i < G.nx) v = fabs(div_cell(G, ux, uy, uz, i, j, k));
+    if (!(v == v)) v = __longlong_as_double(0x7ff0000000000000LL);  // NaN -> +inf
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    __shared__ double wmax[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wmax[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) m = fmax(m, wmax[w]);
+        // non-negative doubles order like their bit patterns
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// poissonEqn.f90:331-337
+__global__ void k_beta(size_t n, double alpha_dt, const double* __restrict__ rho, double* __restrict__ beta)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) beta[t] = alpha_dt / rho[t];
+}
+
+// momentumEqn.f90:806-866 (addPressureGrad) on all periodic x/z faces and the
+// interior y faces (setEqnBounds, momentumEqn.f90:1134-1190).
+__global__ void k_project_mg(FieldGeo G, double alpha_dt, const double* __restrict__ rho,
+                             const double* __restrict__ psi, double* __restrict__ ux, double* __restrict__ uy,
+                             double* __restrict__ uz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    if (i >= nx) return;
+    const size_t c = i + (size_t)nx * (j + (size_t)ny * k);
+    const size_t e = (i + 1 < nx ? i + 1 : 0) + (size_t)nx * (j + (size_t)ny * k);
+    const size_t f = i + (size_t)nx * (j + (size_t)ny * (k + 1 < nz ? k + 1 : 0));
+    const double ic = 1.0 / rho[c];
+    double inv = 0.5 * (1.0 / rho[e] + ic);
+    ux[c] = ux[c] - alpha_dt * inv * (psi[e] - psi[c]) / G.dcn[0][i];
+    inv = 0.5 * (1.0 / rho[f] + ic);
+    uz[c] = uz[c] - alpha_dt * inv * (psi[f] - psi[c]) / G.dcn[2][k];
+    if (j + 1 < ny) {
+        const size_t n = i + (size_t)nx * ((j + 1) + (size_t)ny * k);
+        const size_t yf = i + (size_t)nx * ((j + 1) + (size_t)(ny + 1) * k);
+        inv = 0.5 * (1.0 / rho[n] + ic);
+        uy[yf] = uy[yf] - alpha_dt * inv * (psi[n] - psi[c]) / G.dcn[1][j];
+    }
+}
+
+static dim3 row_block(int nx) { return dim3(nx >= 256 ? 256 : (unsigned)next_pow2(nx < 64 ? 64 : nx)); }
+
+int divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, const double* uz, double* div,
+                   void* stream)
+{
+    if (!ux || !uy || !uz || !div) {
+        set_error("tbfg_divergence_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    FieldGeo G;
+    TBFG_TRY(get_geo(g, G));
+    dim3 blk = row_block(G.nx);
+    hipLaunchKernelGGL(k_divergence, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G, ux, uy,
+                       uz, div);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+int max_divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, const double* uz, double* out,
+                       void* stream)
+{
+    if (!ux || !uy || !uz || !out) {
+        set_error("tbfg_max_divergence_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    FieldGeo G;
+    TBFG_TRY(get_geo(g, G));
+    DevBuf res;
+    TBFG_TRY(res.alloc(sizeof(unsigned long long)));
+    hipStream_t s = (hipStream_t)stream;
+    TBFG_HIP(hipMemsetAsync(res.p, 0, sizeof(unsigned long long), s));
+    dim3 blk = row_block(G.nx);
+    hipLaunchKernelGGL(k_maxdiv, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, s, G, ux, uy, uz,
+                       res.as<unsigned long long>());
+    TBFG_HIP(hipGetLastError());
+    unsigned long long bits = 0;
+    TBFG_HIP(hipMemcpyAsync(&bits, res.p, sizeof bits, hipMemcpyDeviceToHost, s));
+    TBFG_HIP(hipStreamSynchronize(s));
+    double v;
+    static_assert(sizeof v == sizeof bits, "64-bit double");
+    std::memcpy(&v, &bits, sizeof v);
+    *out = v;
+    return TBFG_OK;
+}
+
+int beta_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, double* beta, void* stream)
+{
+    TBFG_TRY(validate_grid(g));
+    if (!rho || !beta) {
+        set_error("tbfg_beta_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    const size_t n = (size_t)g->n[0] * g->n[1] * g->n[2];
+    hipLaunchKernelGGL(k_beta, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, n, alpha_dt, rho, beta);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+int project_mg_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, const double* psi, double* ux,
+                   double* uy, double* uz, void* stream)
+{
+    if (!rho || !psi || !ux || !uy || !uz) {
+        set_error("tbfg_project_mg_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    FieldGeo G;
+    TBFG_TRY(get_geo(g, G));
+    dim3 blk = row_block(G.nx);
+    hipLaunchKernelGGL(k_project_mg, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G,
+                       alpha_dt, rho, psi, ux, uy, uz);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+}  // namespace tbfg

@@ -1,0 +1,15 @@
+// fields.hpp -- RHS, coefficient, projection and continuity-error kernels.
+#pragma once
+#include <memory>
+
+#include "common.hpp"
+
+namespace tbfg {
+int divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, const double* uz, double* div,
+                   void* stream);
+int max_divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, const double* uz, double* out,
+                       void* stream);
+int beta_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, double* beta, void* stream);
+int project_mg_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, const double* psi, double* ux,
+                   double* uy, double* uz, void* stream);
+}  // namespace tbfg

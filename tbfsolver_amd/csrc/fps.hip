@@ -1,0 +1,796 @@
+// fps.hip -- FAST_MODE pressure solve on one GPU (fastPoissonSolver.f90:337-423).
+//
+// Five HBM passes, each an LDS-staged kernel:
+//   K1 x-forward : real lines -> half spectrum (two real lines packed into one
+//                  complex FFT), written as A[kx][j][k]            (r2c, :350-353)
+//   K2 z-forward : A[kx][j][:] -> B[kx][kz][j]                     (:356-357)
+//   K3 y-solve   : one wavefront per (kx,kz) column of B, partitioned Thomas
+//                  + parallel cyclic reduction across the 64 lanes (:280-333)
+//   K4 z-inverse : B -> A                                           (:366-367)
+//   K5 x-inverse : A -> real lines, scaled by 1/(nx nz)             (:370-374)
+// The pencil transposes of pencilDec.f90 become the layout changes folded into
+// the loads and stores of K1/K2/K4/K5.  The volume-weighted mean removal
+// (:388-423) is done exactly in spectral space on the (0,0) column, where
+// sum_ijk V p = dx dz sum_j dyf_j P00(j).
+//
+// Mixed precision: K1-K5 in FP32, the (0,0) column in FP64, followed by FP64
+// iterative-refinement sweeps p += S32(s - L p) with the exact FFT operator L.
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "fps.hpp"
+#include "reduce.hpp"
+
+namespace tbfg {
+
+template <class T>
+struct C2 {
+    T x, y;
+};
+
+template <class T>
+__device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <class T>
+__device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <class T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <class T>
+__device__ __forceinline__ C2<T> cscale(C2<T> a, T s) { return {a.x * s, a.y * s}; }
+
+constexpr int kFftThreads = 256;
+constexpr int kMaxBfly = 8;        // butterflies per thread per pass
+constexpr int kLdsElems = 4096;    // complex elements (lines + twiddles) per block: 64 KB in FP64
+
+struct FftPlan {
+    int n;
+    int nrad;
+    int rad[16];
+};
+
+static int make_plan(int n, FftPlan& p)
+{
+    p.n = n;
+    p.nrad = 0;
+    int m = n;
+    while (m % 4 == 0) {
+        p.rad[p.nrad++] = 4;
+        m /= 4;
+    }
+    while (m % 2 == 0) {
+        p.rad[p.nrad++] = 2;
+        m /= 2;
+    }
+    while (m % 3 == 0) {
+        p.rad[p.nrad++] = 3;
+        m /= 3;
+    }
+    return m == 1 ? TBFG_OK : TBFG_EINVAL;
+}
+
+// R-point DFT in registers, SIGN = -1 forward, +1 backward
+template <class T, int SIGN>
+__device__ __forceinline__ void dft2(C2<T>* v)
+{
+    const C2<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <class T, int SIGN>
+__device__ __forceinline__ void dft4(C2<T>* v)
+{
+    const C2<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const C2<T> t2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+    // SIGN*i*d: forward multiplies by -i, backward by +i
+    const C2<T> t3 = (SIGN < 0) ? C2<T>{d.y, -d.x} : C2<T>{-d.y, d.x};
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+}
+
+template <class T, int SIGN>
+__device__ __forceinline__ void dft3(C2<T>* v)
+{
+    const T h = (T)0.8660254037844386;  // sqrt(3)/2
+    const C2<T> t1 = cadd(v[1], v[2]);
+    const C2<T> t2 = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
+    const C2<T> dd = csub(v[1], v[2]);
+    // i * SIGN * h * dd
+    const C2<T> t3 = (SIGN < 0) ? C2<T>{h * dd.y, -h * dd.x} : C2<T>{-h * dd.y, h * dd.x};
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, t3);
+    v[2] = csub(t2, t3);
+}
+
+// Stockham autosort FFT of `nlines` lines of length N held in LDS (line l at
+// buf + l*N); tw[t] = exp(-2 pi i t / N).  Each pass reads all its butterfly
+// inputs into registers, synchronises, then writes.
+template <class T, int SIGN>
+__device__ void fft_lines(C2<T>* buf, int N, int nlines, const C2<T>* tw, const FftPlan& plan)
+{
+    int Ns = 1;
+    for (int s = 0; s < plan.nrad; ++s) {
+        const int R = plan.rad[s];
+        const int nb = N / R;
+        const int total = nlines * nb;
+        const int stride = N / (Ns * R);
+        C2<T> v[kMaxBfly][4];
+#pragma unroll
+        for (int u = 0; u < kMaxBfly; ++u) {
+            const int b = threadIdx.x + u * kFftThreads;
+            if (b < total) {
+                const int line = b / nb, j = b - line * nb;
+                const int k = j % Ns;
+                const C2<T>* src = buf + line * N + j;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (r < R) {
+                        C2<T> x = src[r * nb];
+                        if (r > 0 && k > 0) {
+                            C2<T> w = tw[r * k * stride];
+                            if (SIGN > 0) w.y = -w.y;
+                            x = cmul(x, w);
+                        }
+                        v[u][r] = x;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kMaxBfly; ++u) {
+            const int b = threadIdx.x + u * kFftThreads;
+            if (b < total) {
+                const int line = b / nb, j = b - line * nb;
+                const int k = j % Ns;
+                if (R == 4)
+                    dft4<T, SIGN>(v[u]);
+                else if (R == 2)
+                    dft2<T, SIGN>(v[u]);
+                else
+                    dft3<T, SIGN>(v[u]);
+                C2<T>* dst = buf + line * N + (j / Ns) * Ns * R + k;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (r < R) dst[r * Ns] = v[u][r];
+            }
+        }
+        __syncthreads();
+        Ns *= R;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void load_twiddles(C2<T>* dst, const C2<T>* __restrict__ src, int n)
+{
+    for (int t = threadIdx.x; t < n; t += kFftThreads) dst[t] = src[t];
+}
+
+// K1: s(:, j, k0..k0+KT-1) -> A[kx][j][k], kx = 0..nx/2.
+template <class T, class TIN>
+__global__ void __launch_bounds__(kFftThreads)
+    k_xfwd(int nx, int ny, int nz, int KT, FftPlan plan, const C2<T>* __restrict__ tw, const TIN* __restrict__ s,
+           C2<T>* __restrict__ A)
+{
+    extern __shared__ unsigned char smem[];
+    C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
+    const int L = KT >> 1;
+    C2<T>* twl = buf + L * nx;
+    const int j = blockIdx.y;
+    const int k0 = blockIdx.x * KT;
+    load_twiddles(twl, tw, nx);
+    for (int t = threadIdx.x; t < L * nx; t += kFftThreads) {
+        const int l = t / nx, i = t - l * nx;
+        const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
+        const size_t o1 = o0 + (size_t)nx * ny;
+        buf[t] = C2<T>{(T)s[o0], (T)s[o1]};
+    }
+    __syncthreads();
+    fft_lines<T, -1>(buf, nx, L, twl, plan);
+    const int nh = nx / 2 + 1;
+    for (int t = threadIdx.x; t < nh * KT; t += kFftThreads) {
+        const int m = t / KT, kk = t - m * KT;
+        const int l = kk >> 1;
+        const C2<T> Z = buf[l * nx + m];
+        const C2<T> Zn = buf[l * nx + (m == 0 ? 0 : nx - m)];
+        C2<T> val;
+        if ((kk & 1) == 0)
+            val = C2<T>{(T)0.5 * (Z.x + Zn.x), (T)0.5 * (Z.y - Zn.y)};  // (Z + conj Zn)/2
+        else
+            val = C2<T>{(T)0.5 * (Z.y + Zn.y), (T)0.5 * (Zn.x - Z.x)};  // (Z - conj Zn)/(2i)
+        A[((size_t)m * ny + j) * nz + k0 + kk] = val;
+    }
+}
+
+// K2: A[kx][j0..j0+JT-1][:] -> B[kx][kz][j]
+template <class T>
+__global__ void __launch_bounds__(kFftThreads)
+    k_zfwd(int ny, int nz, int JT, FftPlan plan, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ A,
+           C2<T>* __restrict__ B)
+{
+    extern __shared__ unsigned char smem[];
+    C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
+    C2<T>* twl = buf + JT * nz;
+    const int m = blockIdx.y;
+    const int j0 = blockIdx.x * JT;
+    load_twiddles(twl, tw, nz);
+    const C2<T>* src = A + ((size_t)m * ny + j0) * nz;
+    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) buf[t] = src[t];
+    __syncthreads();
+    fft_lines<T, -1>(buf, nz, JT, twl, plan);
+    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) {
+        const int kz = t / JT, l = t - kz * JT;
+        B[((size_t)m * nz + kz) * ny + j0 + l] = buf[l * nz + kz];
+    }
+}
+
+// K4: B[kx][kz][j0..] -> A[kx][j][k]
+template <class T>
+__global__ void __launch_bounds__(kFftThreads)
+    k_zinv(int ny, int nz, int JT, FftPlan plan, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ B,
+           C2<T>* __restrict__ A)
+{
+    extern __shared__ unsigned char smem[];
+    C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
+    C2<T>* twl = buf + JT * nz;
+    const int m = blockIdx.y;
+    const int j0 = blockIdx.x * JT;
+    load_twiddles(twl, tw, nz);
+    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) {
+        const int kz = t / JT, l = t - kz * JT;
+        buf[l * nz + kz] = B[((size_t)m * nz + kz) * ny + j0 + l];
+    }
+    __syncthreads();
+    fft_lines<T, +1>(buf, nz, JT, twl, plan);
+    C2<T>* dst = A + ((size_t)m * ny + j0) * nz;
+    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) dst[t] = buf[t];
+}
+
+// K5: A[kx][j][k0..k0+KT-1] -> p(:, j, k) scaled by 1/(nx nz).
+// mode 0: p = val; mode 1: p = p + val - shift[0] (refinement correction).
+template <class T>
+__global__ void __launch_bounds__(kFftThreads)
+    k_xinv(int nx, int ny, int nz, int KT, FftPlan plan, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ A,
+           double* __restrict__ p, int mode, const double* __restrict__ shift)
+{
+    extern __shared__ unsigned char smem[];
+    C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
+    const int L = KT >> 1;
+    C2<T>* twl = buf + L * nx;
+    const int j = blockIdx.y;
+    const int k0 = blockIdx.x * KT;
+    const int nh = nx / 2 + 1;
+    load_twiddles(twl, tw, nx);
+    // Z_l[m] = X1[m] + i X2[m] with Hermitian extension X[nx-m] = conj X[m];
+    // DC and Nyquist keep their real parts only, as a c2r transform does.
+    for (int t = threadIdx.x; t < nh * L; t += kFftThreads) {
+        const int m = t / L, l = t - m * L;
+        const size_t o = ((size_t)m * ny + j) * nz + k0 + 2 * l;
+        C2<T> x1 = A[o], x2 = A[o + 1];
+        if (m == 0 || 2 * m == nx) {
+            buf[l * nx + m] = C2<T>{x1.x, x2.x};
+        } else {
+            buf[l * nx + m] = C2<T>{x1.x - x2.y, x1.y + x2.x};
+            buf[l * nx + nx - m] = C2<T>{x1.x + x2.y, x2.x - x1.y};
+        }
+    }
+    __syncthreads();
+    fft_lines<T, +1>(buf, nx, L, twl, plan);
+    const double scale = 1.0 / ((double)nx * (double)nz);
+    const double sh = (mode == 1) ? shift[0] : 0.0;
+    for (int t = threadIdx.x; t < L * nx; t += kFftThreads) {
+        const int l = t / nx, i = t - l * nx;
+        const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
+        const size_t o1 = o0 + (size_t)nx * ny;
+        const double v0 = (double)buf[t].x * scale, v1 = (double)buf[t].y * scale;
+        if (mode == 0) {
+            p[o0] = v0;
+            p[o1] = v1;
+        } else {
+            p[o0] = p[o0] + v0 - sh;
+            p[o1] = p[o1] + v1 - sh;
+        }
+    }
+}
+
+// ------------------------------------------------------------ K3 y-solve
+
+template <class T>
+__device__ __forceinline__ T shfl_t(T v, int src) { return __shfl(v, src, 64); }
+
+// Solves am_j x_{j-1} + (lam - am_j - ap_j) x_j + ap_j x_{j+1} = d_j for one
+// column with one wavefront.  Lane p owns rows p*m..p*m+m-1 (m <= MR).  Each
+// lane eliminates its block to an affine map of the last unknowns of its own
+// and the previous block; the resulting 64-unknown tridiagonal system is solved
+// by parallel cyclic reduction with shuffles, then the block interiors are
+// recovered.  Systems with lam != 0 are strictly diagonally dominant, so no
+// pivoting is needed (LAPACK zgttrf at fastPoissonSolver.f90:316 never pivots
+// them).  pinned: the singular (0,0) column, solved with x_{ny-1} = 0 and then
+// made volume-weighted mean free (the reference patches U(n,n)=eps instead,
+// :318-322, and removes the mean afterwards, :388-423).
+template <class TC, int MR, class TS>
+__device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict__ am, const TC* __restrict__ ap,
+                         TC lam, bool pinned, const TC* __restrict__ dyf, TC inv_ly)
+{
+    const int lane = threadIdx.x & 63;
+    const int m = (ny + 63) / 64;
+    TC a[MR], b[MR], c[MR];
+    C2<TC> d[MR];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        const int row = lane * m + r;
+        if (r < m && row < ny && !(pinned && row == ny - 1)) {
+            a[r] = am[row];
+            c[r] = ap[row];
+            b[r] = lam - a[r] - c[r];
+            const C2<TS> v = col[row];
+            d[r] = C2<TC>{(TC)v.x, (TC)v.y};
+        } else {
+            a[r] = 0;
+            c[r] = 0;
+            b[r] = 1;
+            d[r] = C2<TC>{0, 0};
+        }
+    }
+    // local elimination of rows 0..m-2 (T' with external couplings moved right)
+    C2<TC> al[MR];
+    TC be[MR], ga[MR], cp[MR];
+    C2<TC> fa = {0, 0}, la = {0, 0};
+    TC fb = 0, fg = 1, lb = 1, lg = 0;  // first-row and previous-row maps for m == 1
+    if (m >= 2) {
+        const int mm = m - 1;
+#pragma unroll
+        for (int r = 0; r < MR - 1; ++r) {
+            if (r < mm) {
+                const TC cint = (r < mm - 1) ? c[r] : (TC)0;
+                const TC rg = (r == mm - 1) ? -c[r] : (TC)0;
+                const TC rb = (r == 0) ? -a[0] : (TC)0;
+                TC den;
+                if (r == 0) {
+                    den = b[0];
+                    al[0] = C2<TC>{d[0].x / den, d[0].y / den};
+                    be[0] = rb / den;
+                    ga[0] = rg / den;
+                } else {
+                    den = b[r] - a[r] * cp[r - 1];
+                    al[r] = C2<TC>{(d[r].x - a[r] * al[r - 1].x) / den, (d[r].y - a[r] * al[r - 1].y) / den};
+                    be[r] = (rb - a[r] * be[r - 1]) / den;
+                    ga[r] = (rg - a[r] * ga[r - 1]) / den;
+                }
+                cp[r] = cint / den;
+            }
+        }
+#pragma unroll
+        for (int r = MR - 3; r >= 0; --r) {
+            if (r < mm - 1) {
+                al[r].x -= cp[r] * al[r + 1].x;
+                al[r].y -= cp[r] * al[r + 1].y;
+                be[r] -= cp[r] * be[r + 1];
+                ga[r] -= cp[r] * ga[r + 1];
+            }
+        }
+        fa = al[0];
+        fb = be[0];
+        fg = ga[0];
+#pragma unroll
+        for (int r = 0; r < MR - 1; ++r)
+            if (r == mm - 1) {
+                la = al[r];
+                lb = be[r];
+                lg = ga[r];
+            }
+    }
+    // last row of the block
+    TC ae = 0, bee = 1, ce = 0;
+    C2<TC> de = {0, 0};
+#pragma unroll
+    for (int r = 0; r < MR; ++r)
+        if (r == m - 1) {
+            ae = a[r];
+            bee = b[r];
+            ce = c[r];
+            de = d[r];
+        }
+    // first-row map of the next block
+    C2<TC> na = {shfl_t(fa.x, lane + 1), shfl_t(fa.y, lane + 1)};
+    TC nb_ = shfl_t(fb, lane + 1), ng = shfl_t(fg, lane + 1);
+    if (lane == 63) {
+        na = C2<TC>{0, 0};
+        nb_ = 0;
+        ng = 0;
+    }
+    TC A = ae * lb;
+    TC B = ae * lg + bee + ce * nb_;
+    TC C = ce * ng;
+    C2<TC> D = {de.x - ae * la.x - ce * na.x, de.y - ae * la.y - ce * na.y};
+    // parallel cyclic reduction over the 64 lanes
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        TC Am = shfl_t(A, lane - s), Bm = shfl_t(B, lane - s), Cm = shfl_t(C, lane - s);
+        C2<TC> Dm = {shfl_t(D.x, lane - s), shfl_t(D.y, lane - s)};
+        TC Ap = shfl_t(A, lane + s), Bp = shfl_t(B, lane + s), Cp = shfl_t(C, lane + s);
+        C2<TC> Dp = {shfl_t(D.x, lane + s), shfl_t(D.y, lane + s)};
+        if (lane - s < 0) {
+            Am = 0;
+            Bm = 1;
+            Cm = 0;
+            Dm = C2<TC>{0, 0};
+        }
+        if (lane + s > 63) {
+            Ap = 0;
+            Bp = 1;
+            Cp = 0;
+            Dp = C2<TC>{0, 0};
+        }
+        const TC k1 = A / Bm, k2 = C / Bp;
+        const TC A2 = -Am * k1, C2v = -Cp * k2;
+        const TC B2 = B - Cm * k1 - Ap * k2;
+        D = C2<TC>{D.x - Dm.x * k1 - Dp.x * k2, D.y - Dm.y * k1 - Dp.y * k2};
+        A = A2;
+        B = B2;
+        C = C2v;
+    }
+    const C2<TC> y = {D.x / B, D.y / B};
+    C2<TC> ym = {shfl_t(y.x, lane - 1), shfl_t(y.y, lane - 1)};
+    if (lane == 0) ym = C2<TC>{0, 0};
+    C2<TC> x[MR];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        if (r < m - 1)
+            x[r] = C2<TC>{al[r].x + be[r] * ym.x + ga[r] * y.x, al[r].y + be[r] * ym.y + ga[r] * y.y};
+        else
+            x[r] = y;
+    }
+    if (pinned) {
+        // spectral volume-weighted mean: P00 -= sum_j dyf_j P00(j) / Ly
+        TC sx = 0, sy = 0;
+#pragma unroll
+        for (int r = 0; r < MR; ++r) {
+            const int row = lane * m + r;
+            if (r < m && row < ny) {
+                sx += dyf[row] * x[r].x;
+                sy += dyf[row] * x[r].y;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            sx += __shfl_xor(sx, off, 64);
+            sy += __shfl_xor(sy, off, 64);
+        }
+        sx *= inv_ly;
+        sy *= inv_ly;
+#pragma unroll
+        for (int r = 0; r < MR; ++r) {
+            x[r].x -= sx;
+            x[r].y -= sy;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        const int row = lane * m + r;
+        if (r < m && row < ny) col[row] = C2<TS>{(TS)x[r].x, (TS)x[r].y};
+    }
+}
+
+struct TriTables {
+    const double* lx;   // nh
+    const double* lz;   // nz
+    const double* am;   // ny
+    const double* ap;
+    const double* dyf;
+    const float* amf;
+    const float* apf;
+    const float* dyff;
+    double inv_ly;
+};
+
+template <class T, int MR>
+__global__ void __launch_bounds__(256) k_ysolve(int nh, int ny, int nz, TriTables tt, C2<T>* __restrict__ B)
+{
+    const int sys = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (sys >= nh * nz) return;
+    const int m = sys / nz, kz = sys - m * nz;  // B is [kx][kz][j]
+    C2<T>* col = B + (size_t)sys * ny;
+    const double lam = tt.lx[m] + tt.lz[kz];
+    const bool pinned = (m == 0 && kz == 0);
+    if (sizeof(T) == 8 || pinned) {
+        tri_wave<double, MR, T>(col, ny, tt.am, tt.ap, lam, pinned, tt.dyf, tt.inv_ly);
+    } else {
+        tri_wave<float, MR, T>(col, ny, tt.amf, tt.apf, (float)lam, false, tt.dyff, (float)tt.inv_ly);
+    }
+}
+
+// ----------------------------------------------------- refinement residual
+
+struct OpTables {
+    int nx, ny, nz;
+    double idx2, idz2;
+    const double* am;
+    const double* ap;
+    const double* vol_y;  // dyf_j * dx * dz
+};
+
+// r = s - L p with L the operator the FFT solver inverts, and sum V p.
+__global__ void k_fft_residual(OpTables o, const double* __restrict__ s, const double* __restrict__ p,
+                               double* __restrict__ r, double* __restrict__ out_mean, double inv_vg, Reducer red)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    double v[1] = {0.0};
+    if (i < o.nx) {
+        const int nx = o.nx, ny = o.ny, nz = o.nz;
+        const size_t c = i + (size_t)nx * (j + (size_t)ny * k);
+        const double pc = p[c];
+        const double pe = p[c - i + (i + 1 < nx ? i + 1 : 0)];
+        const double pw = p[c - i + (i > 0 ? i - 1 : nx - 1)];
+        const double pf = p[i + (size_t)nx * (j + (size_t)ny * (k + 1 < nz ? k + 1 : 0))];
+        const double pb = p[i + (size_t)nx * (j + (size_t)ny * (k > 0 ? k - 1 : nz - 1))];
+        const double pn = (j + 1 < ny) ? p[c + nx] : pc;
+        const double ps = (j > 0) ? p[c - nx] : pc;
+        const double lp = (pe - 2.0 * pc + pw) * o.idx2 + (pf - 2.0 * pc + pb) * o.idz2 + o.am[j] * (ps - pc) +
+                          o.ap[j] * (pn - pc);
+        r[c] = s[c] - lp;
+        v[0] = pc * o.vol_y[j];
+    }
+    double tot[1];
+    if (grid_sum<1>(v, red, tot) && threadIdx.x == 0) out_mean[0] = tot[0] * inv_vg;
+}
+
+__global__ void k_fft_operator(OpTables o, const double* __restrict__ p, double* __restrict__ q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    if (i >= o.nx) return;
+    const int nx = o.nx, ny = o.ny, nz = o.nz;
+    const size_t c = i + (size_t)nx * (j + (size_t)ny * k);
+    const double pc = p[c];
+    const double pe = p[c - i + (i + 1 < nx ? i + 1 : 0)];
+    const double pw = p[c - i + (i > 0 ? i - 1 : nx - 1)];
+    const double pf = p[i + (size_t)nx * (j + (size_t)ny * (k + 1 < nz ? k + 1 : 0))];
+    const double pb = p[i + (size_t)nx * (j + (size_t)ny * (k > 0 ? k - 1 : nz - 1))];
+    const double pn = (j + 1 < ny) ? p[c + nx] : pc;
+    const double ps = (j > 0) ? p[c - nx] : pc;
+    q[c] = (pe - 2.0 * pc + pw) * o.idx2 + (pf - 2.0 * pc + pb) * o.idz2 + o.am[j] * (ps - pc) + o.ap[j] * (pn - pc);
+}
+
+}  // namespace tbfg
+
+// ------------------------------------------------------------------ handle
+
+using namespace tbfg;
+
+struct tbfg_fps {
+    tbfg_grid_desc g{};
+    int precision = 0, refine = 0;
+    int nx = 0, ny = 0, nz = 0, nh = 0;
+    int KT = 0, JT = 0, MR = 1;
+    FftPlan px{}, pz{};
+    DevBuf A, B;               // spectral work arrays (complex, T)
+    DevBuf tab;                // lx, lz, am, ap, dyf, vol_y (double)
+    DevBuf tabf;               // am, ap, dyf (float)
+    DevBuf twx, twz, twxf, twzf;
+    DevBuf r, partials, counter, mean;
+    TriTables tt{};
+    OpTables ot{};
+    double inv_vg = 0;
+    int64_t footprint = 0;
+};
+
+namespace tbfg {
+
+// launch shape of the per-row FP64 kernels (refinement residual, operator)
+static unsigned row_threads(int nx) { return nx >= 256 ? 256u : 64u; }
+static unsigned row_blocks(int nx) { return ceil_div(nx, row_threads(nx)); }
+
+static int choose_lines(int n, int count, int pairs)
+{
+    // largest L with (L+1)*n <= kLdsElems (lines plus twiddles), (pairs ? 2L : L)
+    // dividing count, L <= 64
+    int best = 0;
+    for (int L = 1; L <= 64; ++L) {
+        if ((L + 1) * n > kLdsElems) break;
+        const int span = pairs ? 2 * L : L;
+        if (count % span == 0) best = L;
+    }
+    return best;
+}
+
+int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
+{
+    if (!out) {
+        set_error("tbfg_fps_create: NULL output");
+        return TBFG_EINVAL;
+    }
+    *out = nullptr;
+    TBFG_TRY(validate_grid(g));
+    std::unique_ptr<tbfg_fps> h(new tbfg_fps());
+    h->g = *g;
+    h->precision = o ? o->precision : TBFG_PREC_FP64;
+    h->refine = o ? o->refine_sweeps : 2;
+    if (h->precision != TBFG_PREC_FP64 && h->precision != TBFG_PREC_MIXED) {
+        set_error("precision = %d", h->precision);
+        return TBFG_EINVAL;
+    }
+    if (h->precision == TBFG_PREC_MIXED && (h->refine < 0 || h->refine > 8)) {
+        set_error("refine_sweeps = %d (expected 0..8)", h->refine);
+        return TBFG_EINVAL;
+    }
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    if (nx % 2 || nz % 2 || nx > 1024 || nz > 1024 || ny > 1024 || nx < 2 || nz < 2) {
+        set_error("FFT solver needs even nx, nz <= 1024 and ny <= 1024 (got %d x %d x %d)", nx, ny, nz);
+        return TBFG_EINVAL;
+    }
+    if (make_plan(nx, h->px) || make_plan(nz, h->pz)) {
+        set_error("FFT lengths must be 2^a 3^b (got nx=%d nz=%d)", nx, nz);
+        return TBFG_EINVAL;
+    }
+    h->nx = nx;
+    h->ny = ny;
+    h->nz = nz;
+    h->nh = nx / 2 + 1;
+    h->KT = 2 * choose_lines(nx, nz, 1);
+    h->JT = choose_lines(nz, ny, 0);
+    if (h->KT < 2 || h->JT < 1) {
+        set_error("no line tiling for %d x %d x %d", nx, ny, nz);
+        return TBFG_EINVAL;
+    }
+    // butterflies per thread: L*N/2 <= kMaxBfly * threads
+    if ((h->KT / 2) * nx / 2 > kMaxBfly * kFftThreads || h->JT * nz / 2 > kMaxBfly * kFftThreads) {
+        set_error("line tiling exceeds the per-thread butterfly budget");
+        return TBFG_EINVAL;
+    }
+    const int m = (ny + 63) / 64;
+    h->MR = m <= 1 ? 1 : (m <= 2 ? 2 : (m <= 4 ? 4 : (m <= 8 ? 8 : 16)));
+    HostLevel L = make_level(g, 0);
+    // fastPoissonSolver.f90:215-251 (eigenvalues), :255-276 (y coefficients)
+    std::vector<double> t;
+    const double dx = L.ax[0].c(2) - L.ax[0].c(1), dz = L.ax[2].c(2) - L.ax[2].c(1);
+    const double pi = 3.141592653589793;
+    size_t o_lx = t.size();
+    for (int i = 1; i <= h->nh; ++i) t.push_back(2.0 * (std::cos((i - 1) * 2.0 * pi / nx) - 1.0) / (dx * dx));
+    size_t o_lz = t.size();
+    for (int i = 1; i <= nz; ++i) t.push_back(2.0 * (std::cos((i - 1) * 2.0 * pi / nz) - 1.0) / (dz * dz));
+    size_t o_am = t.size();
+    for (int j = 1; j <= ny; ++j) t.push_back(j == 1 ? 0.0 : 1.0 / (L.ax[1].dc(j) * L.ax[1].df(j)));
+    size_t o_ap = t.size();
+    for (int j = 1; j <= ny; ++j) t.push_back(j == ny ? 0.0 : 1.0 / (L.ax[1].dc(j + 1) * L.ax[1].df(j)));
+    size_t o_dyf = t.size();
+    double ly = 0.0;
+    for (int j = 1; j <= ny; ++j) {
+        t.push_back(L.ax[1].df(j));
+        ly += L.ax[1].df(j);
+    }
+    size_t o_vy = t.size();
+    for (int j = 1; j <= ny; ++j) t.push_back(L.ax[1].df(j) * L.ax[0].df(1) * L.ax[2].df(1));
+    TBFG_TRY(upload(h->tab, t.data(), t.size() * 8));
+    std::vector<float> tf;
+    for (size_t q = o_am; q < o_vy; ++q) tf.push_back((float)t[q]);
+    TBFG_TRY(upload(h->tabf, tf.data(), tf.size() * 4));
+    const double* tb = h->tab.as<double>();
+    const float* tbf = h->tabf.as<float>();
+    h->tt.lx = tb + o_lx;
+    h->tt.lz = tb + o_lz;
+    h->tt.am = tb + o_am;
+    h->tt.ap = tb + o_ap;
+    h->tt.dyf = tb + o_dyf;
+    h->tt.amf = tbf;
+    h->tt.apf = tbf + ny;
+    h->tt.dyff = tbf + 2 * ny;
+    h->tt.inv_ly = 1.0 / ly;
+    h->ot.nx = nx;
+    h->ot.ny = ny;
+    h->ot.nz = nz;
+    h->ot.idx2 = 1.0 / (dx * dx);
+    h->ot.idz2 = 1.0 / (dz * dz);
+    h->ot.am = h->tt.am;
+    h->ot.ap = h->tt.ap;
+    h->ot.vol_y = tb + o_vy;
+    h->inv_vg = 1.0 / (ly * L.ax[0].df(1) * nx * L.ax[2].df(1) * nz);
+    // twiddles exp(-2 pi i t / n) in FP64 and rounded to FP32
+    auto tw = [&](int n, DevBuf& d64, DevBuf& d32) -> int {
+        std::vector<double> w(2 * (size_t)n);
+        std::vector<float> wf(2 * (size_t)n);
+        for (int q = 0; q < n; ++q) {
+            w[2 * q] = std::cos(2.0 * pi * q / n);
+            w[2 * q + 1] = -std::sin(2.0 * pi * q / n);
+            wf[2 * q] = (float)w[2 * q];
+            wf[2 * q + 1] = (float)w[2 * q + 1];
+        }
+        TBFG_TRY(upload(d64, w.data(), w.size() * 8));
+        TBFG_TRY(upload(d32, wf.data(), wf.size() * 4));
+        return TBFG_OK;
+    };
+    TBFG_TRY(tw(nx, h->twx, h->twxf));
+    TBFG_TRY(tw(nz, h->twz, h->twzf));
+    const size_t es = h->precision == TBFG_PREC_MIXED ? 8 : 16;  // complex element size
+    const size_t nspec = (size_t)h->nh * ny * nz;
+    TBFG_TRY(h->A.alloc(nspec * es));
+    TBFG_TRY(h->B.alloc(nspec * es));
+    if (h->precision == TBFG_PREC_MIXED) {
+        TBFG_TRY(h->r.alloc((size_t)nx * ny * nz * 8));
+        TBFG_TRY(h->partials.alloc((size_t)row_blocks(nx) * ny * nz * 8 + 64));
+        TBFG_TRY(h->counter.alloc(64));
+        TBFG_HIP(hipMemset(h->counter.p, 0, 64));
+        TBFG_TRY(h->mean.alloc(64));
+    }
+    h->footprint = (int64_t)(h->A.bytes + h->B.bytes + h->tab.bytes + h->tabf.bytes + h->twx.bytes + h->twz.bytes +
+                             h->twxf.bytes + h->twzf.bytes + h->r.bytes + h->partials.bytes + h->counter.bytes +
+                             h->mean.bytes);
+    *out = h.release();
+    return TBFG_OK;
+}
+
+template <class T, int MR>
+static void launch_ysolve(tbfg_fps* h, C2<T>* B, hipStream_t s)
+{
+    const int nsys = h->nh * h->nz;
+    hipLaunchKernelGGL((k_ysolve<T, MR>), dim3(ceil_div(nsys, 4)), dim3(256), 0, s, h->nh, h->ny, h->nz, h->tt, B);
+}
+
+// one spectral solve: in (double) -> p (double), mode 0 write / 1 accumulate
+template <class T, class TIN>
+static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipStream_t s)
+{
+    const int nx = h->nx, ny = h->ny, nz = h->nz, nh = h->nh;
+    C2<T>* A = h->A.as<C2<T>>();
+    C2<T>* B = h->B.as<C2<T>>();
+    const bool f32 = sizeof(T) == 4;
+    const C2<T>* twx = f32 ? h->twxf.as<C2<T>>() : h->twx.as<C2<T>>();
+    const C2<T>* twz = f32 ? h->twzf.as<C2<T>>() : h->twz.as<C2<T>>();
+    const size_t ldx = ((size_t)(h->KT / 2) * nx + nx) * sizeof(C2<T>);
+    const size_t ldz = ((size_t)h->JT * nz + nz) * sizeof(C2<T>);
+    hipLaunchKernelGGL((k_xfwd<T, TIN>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px,
+                       twx, in, A);
+    hipLaunchKernelGGL((k_zfwd<T>), dim3(ny / h->JT, nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz, A, B);
+    switch (h->MR) {
+        case 1: launch_ysolve<T, 1>(h, B, s); break;
+        case 2: launch_ysolve<T, 2>(h, B, s); break;
+        case 4: launch_ysolve<T, 4>(h, B, s); break;
+        case 8: launch_ysolve<T, 8>(h, B, s); break;
+        default: launch_ysolve<T, 16>(h, B, s); break;
+    }
+    hipLaunchKernelGGL((k_zinv<T>), dim3(ny / h->JT, nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz, B, A);
+    hipLaunchKernelGGL((k_xinv<T>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px, twx, A,
+                       p, mode, h->mean.as<double>());
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+int fps_solve_dev(tbfg_fps* h, const double* s, double* p, void* stream)
+{
+    if (!h || !s || !p) {
+        set_error("tbfg_fps_solve_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (h->precision == TBFG_PREC_FP64) return spectral_solve<double, double>(h, s, p, 0, st);
+    TBFG_TRY((spectral_solve<float, double>(h, s, p, 0, st)));
+    Reducer red{h->partials.as<double>(), h->counter.as<unsigned>()};
+    for (int it = 0; it < h->refine; ++it) {
+        const dim3 blk(row_threads(h->nx));
+        hipLaunchKernelGGL(k_fft_residual, dim3(row_blocks(h->nx), h->ny, h->nz), blk, 0, st, h->ot, s, p,
+                           h->r.as<double>(), h->mean.as<double>(), h->inv_vg, red);
+        TBFG_TRY((spectral_solve<float, double>(h, h->r.as<double>(), p, 1, st)));
+    }
+    return TBFG_OK;
+}
+
+int fps_apply_operator_dev(tbfg_fps* h, const double* p, double* q, void* stream)
+{
+    const dim3 blk(row_threads(h->nx));
+    hipLaunchKernelGGL(k_fft_operator, dim3(row_blocks(h->nx), h->ny, h->nz), blk, 0, (hipStream_t)stream, h->ot,
+                       p, q);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+void fps_destroy(tbfg_fps* h) { delete h; }
+
+const tbfg_grid_desc* fps_grid(const tbfg_fps* h) { return &h->g; }
+
+int64_t fps_footprint(const tbfg_fps* h) { return h ? h->footprint : 0; }
+
+}  // namespace tbfg

@@ -1,0 +1,780 @@
+// mg_kernels.hip -- CDNA4 kernels of the multigrid-preconditioned CG path:
+// 7-point variable-coefficient operator, red/black Gauss-Seidel half-sweeps,
+// fused residual+restriction, prolongation+correction, PCG vector updates with
+// fused deterministic reductions, and the single-workgroup tail kernel that
+// runs every small level (and the coarsest "exact" solve) in one launch.
+#include "mg_kernels.hpp"
+#include "mg_launch.hpp"
+#include "reduce.hpp"
+
+namespace tbfg {
+
+// ------------------------------------------------------------ device helpers
+
+__device__ __forceinline__ int wrap_up(int i, int n) { return i + 1 < n ? i + 1 : 0; }
+__device__ __forceinline__ int wrap_dn(int i, int n) { return i > 0 ? i - 1 : n - 1; }
+__device__ __forceinline__ int clamp_up(int j, int n) { return j + 1 < n ? j + 1 : j; }
+__device__ __forceinline__ int clamp_dn(int j) { return j > 0 ? j - 1 : j; }
+
+__device__ __forceinline__ void neighbours(int i, int j, int k, int nx, int ny, int nz, size_t& c, size_t& e,
+                                           size_t& w, size_t& n, size_t& s, size_t& f, size_t& b)
+{
+    const size_t row = (size_t)nx * ((size_t)j + (size_t)ny * k);
+    c = row + i;
+    e = row + wrap_up(i, nx);
+    w = row + wrap_dn(i, nx);
+    n = (size_t)nx * ((size_t)clamp_up(j, ny) + (size_t)ny * k) + i;
+    s = (size_t)nx * ((size_t)clamp_dn(j) + (size_t)ny * k) + i;
+    f = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * wrap_up(k, nz));
+    b = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * wrap_dn(k, nz));
+}
+
+// face coefficients a_f = 0.5*(beta_c + beta_nb) / (dc*df) (rbgs.f90:220-226)
+template <class T>
+struct Coef {
+    T aR, aL, aT, aB, aF, aK;
+};
+
+template <class T, class TB>
+__device__ __forceinline__ Coef<T> coefs(const LevelDev<T>& L, const TB* __restrict__ beta, int i, int j, int k,
+                                         size_t c, size_t e, size_t w, size_t n, size_t s, size_t f, size_t b)
+{
+    const T bc = (T)beta[c];
+    Coef<T> a;
+    a.aR = (T)0.5 * ((T)beta[e] + bc) * L.a[0].ip[i];
+    a.aL = (T)0.5 * (bc + (T)beta[w]) * L.a[0].im[i];
+    a.aT = (T)0.5 * ((T)beta[n] + bc) * L.a[1].ip[j];
+    a.aB = (T)0.5 * (bc + (T)beta[s]) * L.a[1].im[j];
+    a.aF = (T)0.5 * ((T)beta[f] + bc) * L.a[2].ip[k];
+    a.aK = (T)0.5 * (bc + (T)beta[b]) * L.a[2].im[k];
+    return a;
+}
+
+// (A p)_c, pcg.f90:171-176
+template <class T, class TP>
+__device__ __forceinline__ T apply_cell(const Coef<T>& a, const TP* __restrict__ p, size_t c, size_t e, size_t w,
+                                        size_t n, size_t s, size_t f, size_t b)
+{
+    const T pc = (T)p[c];
+    return a.aR * ((T)p[e] - pc) - a.aL * (pc - (T)p[w]) + a.aT * ((T)p[n] - pc) - a.aB * (pc - (T)p[s]) +
+           a.aF * ((T)p[f] - pc) - a.aK * (pc - (T)p[b]);
+}
+
+// ---------------------------------------------------------- CG-path kernels
+
+// q = A p (computeQv, pcg.f90:144-183), FP64, no reduction.
+__global__ void k_apply_op(LevelDev<double> L, const double* __restrict__ beta, const double* __restrict__ p,
+                           double* __restrict__ q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    size_t c, e, w, n, s, f, b;
+    neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
+    Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
+    q[c] = apply_cell<double>(a, p, c, e, w, n, s, f, b);
+}
+
+// qv = A d and alpha = delta / (d . qv) (pcg.f90:144-224).
+__global__ void k_op_dot(LevelDev<double> L, const double* __restrict__ beta, const double* __restrict__ d,
+                         double* __restrict__ qv, PcgState* st, Reducer red)
+{
+    if (st->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    double v[1] = {0.0};
+    if (i < L.nx && j < L.ny) {
+        size_t c, e, w, n, s, f, b;
+        neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
+        Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
+        const double out = apply_cell<double>(a, d, c, e, w, n, s, f, b);
+        qv[c] = out;
+        v[0] = d[c] * out;
+    }
+    double tot[1];
+    if (grid_sum<1>(v, red, tot) && threadIdx.x == 0 && threadIdx.y == 0) {
+        const double den = tot[0];
+        if (!(den != 0.0) || !isfinite(den) || !isfinite(st->delta)) {
+            st->done = 1;
+            st->status = TBFG_ENUMERIC;
+            st->alpha = 0.0;
+        } else {
+            st->alpha = st->delta / den;
+        }
+    }
+}
+
+// r = q - A psi with ||r||^2, ||q||^2 (computeResiduals, rbgs.f90:276-333, used at pcg.f90:116).
+__global__ void k_residual_init(LevelDev<double> L, const double* __restrict__ beta, const double* __restrict__ q,
+                                const double* __restrict__ psi, double* __restrict__ r, PcgState* st, Reducer red)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    double v[2] = {0.0, 0.0};
+    if (i < L.nx && j < L.ny) {
+        size_t c, e, w, n, s, f, b;
+        neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
+        Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
+        const double qc = q[c];
+        const double rc = qc - apply_cell<double>(a, psi, c, e, w, n, s, f, b);
+        r[c] = rc;
+        v[0] = rc * rc;
+        v[1] = qc * qc;
+    }
+    double tot[2];
+    if (grid_sum<2>(v, red, tot) && threadIdx.x == 0 && threadIdx.y == 0) {
+        st->rr = tot[0];
+        st->qq = tot[1];
+        st->iter = 0;
+        st->status = TBFG_OK;
+        st->done = 0;
+        const double res = sqrt(tot[0]) / sqrt(tot[1] + 2.2250738585072014e-308);
+        st->res = res;
+        if (!isfinite(res)) {
+            st->done = 1;
+            st->status = TBFG_ENUMERIC;
+        } else if (res <= st->tol) {
+            st->done = 1;
+        } else if (st->max_iter <= 0) {
+            st->done = 1;
+            st->status = TBFG_ENOTCONVERGED;
+        }
+    }
+}
+
+// psi += alpha d; r -= alpha qv; ||r|| and the stop test (pcg.f90:228-262,331-355,359-429).
+__global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __restrict__ r,
+                             const double* __restrict__ d, const double* __restrict__ qv, PcgState* st, Reducer red)
+{
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double v[1] = {0.0};
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ncell) {
+        psi[t] = psi[t] + alpha * d[t];
+        const double rn = r[t] - alpha * qv[t];
+        r[t] = rn;
+        v[0] = rn * rn;
+    }
+    double tot[1];
+    if (grid_sum<1>(v, red, tot) && threadIdx.x == 0) {
+        st->rr = tot[0];
+        st->iter += 1;
+        const double res = sqrt(tot[0]) / sqrt(st->qq + 2.2250738585072014e-308);
+        st->res = res;
+        if (!isfinite(res)) {
+            st->done = 1;
+            st->status = TBFG_ENUMERIC;
+        } else if (res <= st->tol) {
+            st->done = 1;
+            st->status = TBFG_OK;
+        } else if (st->iter >= st->max_iter) {
+            st->done = 1;
+            st->status = TBFG_ENOTCONVERGED;
+        }
+    }
+}
+
+// r.z, qv.z, sum z, sum r, sum qv in one pass; the preconditioned residual is
+// used with its arithmetic mean removed (rbgs.f90:403-446 leaves z mean-free):
+// z' = z - m, r.z' = r.z - m sum r, qv.z' = qv.z - m sum qv.
+template <class TZ>
+__global__ void k_pcg_dots(size_t ncell, const double* __restrict__ r, const TZ* __restrict__ z,
+                           const double* __restrict__ qv, int init, PcgState* st, Reducer red)
+{
+    if (st->done) return;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ncell) {
+        const double zc = (double)z[t];
+        const double rc = r[t];
+        v[0] = rc * zc;
+        v[2] = zc;
+        v[3] = rc;
+        if (!init) {
+            const double qc = qv[t];
+            v[1] = qc * zc;
+            v[4] = qc;
+        }
+    }
+    double tot[5];
+    if (grid_sum<5>(v, red, tot) && threadIdx.x == 0) {
+        const double m = tot[2] / (double)ncell;
+        const double rz = tot[0] - m * tot[3];
+        st->zmean = m;
+        if (init) {
+            st->delta = rz;
+            st->gamma = 0.0;
+        } else {
+            const double qz = tot[1] - m * tot[4];
+            const double g = st->flexible ? (-st->alpha * qz / st->delta) : (rz / st->delta);
+            st->gamma = g;
+            st->delta = rz;
+            if (!isfinite(g) || !isfinite(rz)) {
+                st->done = 1;
+                st->status = TBFG_ENUMERIC;
+            }
+        }
+    }
+}
+
+// d = (z - m) + gamma d (pcg.f90:266-290 with d0 <- d, pcg.f90:127,133-134)
+template <class TZ>
+__global__ void k_pcg_dir(size_t ncell, const TZ* __restrict__ z, double* __restrict__ d, int init,
+                          const PcgState* st)
+{
+    if (st->done) return;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    const double zc = (double)z[t] - st->zmean;
+    d[t] = init ? zc : zc + st->gamma * d[t];
+}
+
+// ------------------------------------------------------------- MG kernels
+
+// one red or black half-sweep of iterateGS (rbgs.f90:205-259) on a level.
+// FIRST: the level starts from p = 0 (rbgs.f90:196-198 / pcg.f90:118), so the
+// red pass reads no p and the black pass uses 0 as its own wall ghost.
+template <class T, class TQ, int COLOUR, bool FIRST>
+__device__ __forceinline__ void rbgs_cell(const LevelDev<T>& L, const T* __restrict__ beta,
+                                          const TQ* __restrict__ q, T* __restrict__ p, int h, int j, int k)
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int i = ((COLOUR + j + k + 1) & 1) + 2 * h;
+    size_t c, e, w, n, s, f, b;
+    neighbours(i, j, k, nx, ny, nz, c, e, w, n, s, f, b);
+    Coef<T> a = coefs<T>(L, beta, i, j, k, c, e, w, n, s, f, b);
+    const T den = a.aR + a.aL + a.aT + a.aB + a.aF + a.aK;
+    const T qc = (T)q[c];
+    T num;
+    if (FIRST && COLOUR == RED) {
+        num = -qc;
+    } else {
+        const T self = FIRST ? (T)0 : p[c];
+        const T pn = (n == c) ? self : p[n];
+        const T ps = (s == c) ? self : p[s];
+        num = -qc + (a.aR * p[e] + a.aL * p[w]) + (a.aT * pn + a.aB * ps) + (a.aF * p[f] + a.aK * p[b]);
+    }
+    p[c] = num / den;
+}
+
+template <class T, class TQ, int COLOUR, bool FIRST>
+__global__ void k_rbgs(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q, T* __restrict__ p,
+                       const int* done)
+{
+    if (done && *done) return;
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (h >= (L.nx >> 1) || j >= L.ny) return;
+    rbgs_cell<T, TQ, COLOUR, FIRST>(L, beta, q, p, h, j, k);
+}
+
+// fused residual (rbgs.f90:300-315) and restriction (interpolation.f90:97-118):
+// one thread per coarse cell, qc = sum_{children} wx wy wz (q - A p).
+template <class T, class TQ>
+__device__ __forceinline__ T resid_restrict_cell(const LevelDev<T>& L, const T* __restrict__ beta,
+                                                 const TQ* __restrict__ q, const T* __restrict__ p, int I, int J,
+                                                 int K)
+{
+    T acc = 0;
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk) {
+        const int k = 2 * K + dk;
+        const T wz = dk ? L.a[2].rhi[K] : L.a[2].rlo[K];
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const int j = 2 * J + dj;
+            const T wy = dj ? L.a[1].rhi[J] : L.a[1].rlo[J];
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const int i = 2 * I + di;
+                const T wx = di ? L.a[0].rhi[I] : L.a[0].rlo[I];
+                size_t c, e, w, n, s, f, b;
+                neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
+                Coef<T> a = coefs<T>(L, beta, i, j, k, c, e, w, n, s, f, b);
+                const T res = (T)q[c] - apply_cell<T>(a, p, c, e, w, n, s, f, b);
+                acc += (wx * wy * wz) * res;
+            }
+        }
+    }
+    return acc;
+}
+
+template <class T, class TQ>
+__global__ void k_resid_restrict(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q,
+                                 const T* __restrict__ p, T* __restrict__ qc, const int* done)
+{
+    if (done && *done) return;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    const int ncx = L.nx >> 1, ncy = L.ny >> 1;
+    if (I >= ncx || J >= ncy) return;
+    qc[I + (size_t)ncx * (J + (size_t)ncy * K)] = resid_restrict_cell<T, TQ>(L, beta, q, p, I, J, K);
+}
+
+// plain restriction of a field (beta, restrictField at multiGrid.f90:210)
+template <class TF, class T>
+__global__ void k_restrict(LevelDev<T> L, const TF* __restrict__ fine, T* __restrict__ coarse)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    const int nx = L.nx, ny = L.ny;
+    const int ncx = nx >> 1, ncy = ny >> 1;
+    if (I >= ncx || J >= ncy) return;
+    T acc = 0;
+    for (int dk = 0; dk < 2; ++dk)
+        for (int dj = 0; dj < 2; ++dj)
+            for (int di = 0; di < 2; ++di) {
+                const T w = (di ? L.a[0].rhi[I] : L.a[0].rlo[I]) * (dj ? L.a[1].rhi[J] : L.a[1].rlo[J]) *
+                            (dk ? L.a[2].rhi[K] : L.a[2].rlo[K]);
+                acc += w * (T)fine[(2 * I + di) + (size_t)nx * ((2 * J + dj) + (size_t)ny * (2 * K + dk))];
+            }
+    coarse[I + (size_t)ncx * (J + (size_t)ncy * K)] = acc;
+}
+
+// prolongation with coarse ghosts and correction p += P pc (multiGrid.f90:222-223).
+template <class T>
+__device__ __forceinline__ void prolong_cell(const LevelDev<T>& L, int ncx, int ncy, int ncz, const T* __restrict__ pc,
+                                             T* __restrict__ p, int i, int j, int k)
+{
+    int il = ((i + 1) >> 1) - 1, jl = ((j + 1) >> 1) - 1, kl = ((k + 1) >> 1) - 1;
+    int ih = il + 1, jh = jl + 1, kh = kl + 1;
+    if (il < 0) il = ncx - 1;
+    if (ih >= ncx) ih = 0;
+    if (jl < 0) jl = 0;
+    if (jh >= ncy) jh = ncy - 1;
+    if (kl < 0) kl = ncz - 1;
+    if (kh >= ncz) kh = 0;
+    const T wxl = L.a[0].plo[i], wxh = L.a[0].phi[i];
+    const T wyl = L.a[1].plo[j], wyh = L.a[1].phi[j];
+    const T wzl = L.a[2].plo[k], wzh = L.a[2].phi[k];
+    const size_t sxy = (size_t)ncx * ncy;
+    const size_t r_ll = (size_t)ncx * jl + sxy * kl, r_hl = (size_t)ncx * jh + sxy * kl;
+    const size_t r_lh = (size_t)ncx * jl + sxy * kh, r_hh = (size_t)ncx * jh + sxy * kh;
+    const T v = wzl * (wyl * (wxl * pc[r_ll + il] + wxh * pc[r_ll + ih]) + wyh * (wxl * pc[r_hl + il] + wxh * pc[r_hl + ih])) +
+                wzh * (wyl * (wxl * pc[r_lh + il] + wxh * pc[r_lh + ih]) + wyh * (wxl * pc[r_hh + il] + wxh * pc[r_hh + ih]));
+    const size_t c = i + (size_t)L.nx * (j + (size_t)L.ny * k);
+    p[c] = p[c] + v;
+}
+
+template <class T>
+__global__ void k_prolong_add(LevelDev<T> L, const T* __restrict__ pc, T* __restrict__ p, const int* done)
+{
+    if (done && *done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    prolong_cell<T>(L, L.nx >> 1, L.ny >> 1, L.nz >> 1, pc, p, i, j, k);
+}
+
+template <class TS, class TD>
+__global__ void k_convert(size_t n, const TS* __restrict__ src, TD* __restrict__ dst)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = (TD)src[t];
+}
+
+// ------------------------------------------------------------ tail kernel
+
+// The levels below ~32K cells are latency bound: every phase is a few
+// microseconds of work, less than a kernel launch.  One workgroup runs all of
+// them back to back with workgroup barriers; their arrays stay in L2.
+template <class T>
+__device__ void tail_smooth(const TailLevel<T>& lv, int sweeps, bool first)
+{
+    const LevelDev<T>& L = lv.geo;
+    const int half = L.nx >> 1;
+    const int rows = L.ny * L.nz;
+    const int work = half * rows;
+    if (first && sweeps == 0) {
+        const int cells = L.nx * rows;
+        for (int t = threadIdx.x; t < cells; t += blockDim.x) lv.p[t] = (T)0;
+        __syncthreads();
+        return;
+    }
+    for (int sw = 0; sw < sweeps; ++sw) {
+        const bool f0 = first && sw == 0;
+        for (int t = threadIdx.x; t < work; t += blockDim.x) {
+            const int h = t % half, row = t / half;
+            const int j = row % L.ny, k = row / L.ny;
+            if (f0)
+                rbgs_cell<T, T, RED, true>(L, lv.beta, lv.q, lv.p, h, j, k);
+            else
+                rbgs_cell<T, T, RED, false>(L, lv.beta, lv.q, lv.p, h, j, k);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < work; t += blockDim.x) {
+            const int h = t % half, row = t / half;
+            const int j = row % L.ny, k = row / L.ny;
+            if (f0)
+                rbgs_cell<T, T, BLACK, true>(L, lv.beta, lv.q, lv.p, h, j, k);
+            else
+                rbgs_cell<T, T, BLACK, false>(L, lv.beta, lv.q, lv.p, h, j, k);
+        }
+        __syncthreads();
+    }
+}
+
+template <class T>
+__device__ void tail_resid_restrict(const TailLevel<T>& lv, const TailLevel<T>& lc)
+{
+    const LevelDev<T>& L = lv.geo;
+    const int ncx = L.nx >> 1, ncy = L.ny >> 1, ncz = L.nz >> 1;
+    const int cells = ncx * ncy * ncz;
+    for (int t = threadIdx.x; t < cells; t += blockDim.x) {
+        const int I = t % ncx, r = t / ncx;
+        const int J = r % ncy, K = r / ncy;
+        lc.q[t] = resid_restrict_cell<T, T>(L, lv.beta, lv.q, lv.p, I, J, K);
+    }
+    __syncthreads();
+}
+
+template <class T>
+__device__ void tail_prolong(const TailLevel<T>& lv, const TailLevel<T>& lc)
+{
+    const LevelDev<T>& L = lv.geo;
+    const int cells = L.nx * L.ny * L.nz;
+    for (int t = threadIdx.x; t < cells; t += blockDim.x) {
+        const int i = t % L.nx, r = t / L.nx;
+        const int j = r % L.ny, k = r / L.ny;
+        prolong_cell<T>(L, lc.geo.nx, lc.geo.ny, lc.geo.nz, lc.p, lv.p, i, j, k);
+    }
+    __syncthreads();
+}
+
+template <class T>
+__device__ void tail_dense(const TailPlan<T>& P, const TailLevel<T>& lv)
+{
+    const int nb = lv.geo.nx * lv.geo.ny * lv.geo.nz;
+    for (int r = threadIdx.x; r < nb; r += blockDim.x) {
+        double acc = 0.0;
+        const double* g = P.G + (size_t)r * nb;
+        for (int c = 0; c < nb; ++c) acc += g[c] * (double)lv.q[c];
+        lv.p[r] = (T)acc;
+    }
+    __syncthreads();
+}
+
+// V-cycle over tail levels top..bot (multiGrid.f90:186-232); at `bot` either the
+// dense solve or, at the reference's coarsest level, coarse_cycles inner cycles
+// over the sub-hierarchy below it.
+template <class T>
+__device__ void tail_vcycle(const TailPlan<T>& P, int top, int bot, bool reset_top);
+
+template <class T>
+__device__ void tail_bottom(const TailPlan<T>& P, int bot)
+{
+    if (bot == P.user_last && P.bottom > P.user_last) {
+        for (int c = 0; c < P.coarse_cycles; ++c) {
+            // first inner cycle starts from p = 0 at the coarsest user level
+            const TailPlan<T>& Q = P;
+            const int top = P.user_last;
+            const bool reset = (c == 0);
+            for (int l = top; l < Q.bottom; ++l) {
+                tail_smooth<T>(Q.lv[l], Q.n_pre, (l > top) || reset);
+                tail_resid_restrict<T>(Q.lv[l], Q.lv[l + 1]);
+            }
+            tail_dense<T>(Q, Q.lv[Q.bottom]);
+            for (int l = Q.bottom - 1; l >= top; --l) {
+                tail_prolong<T>(Q.lv[l], Q.lv[l + 1]);
+                tail_smooth<T>(Q.lv[l], Q.n_post, false);
+            }
+        }
+        if (P.coarse_cycles <= 0) {
+            const int cells = P.lv[bot].geo.nx * P.lv[bot].geo.ny * P.lv[bot].geo.nz;
+            for (int t = threadIdx.x; t < cells; t += blockDim.x) P.lv[bot].p[t] = (T)0;
+            __syncthreads();
+        }
+    } else {
+        tail_dense<T>(P, P.lv[bot]);
+    }
+}
+
+template <class T>
+__device__ void tail_vcycle(const TailPlan<T>& P, int top, int bot, bool reset_top)
+{
+    for (int l = top; l < bot; ++l) {
+        tail_smooth<T>(P.lv[l], P.n_pre, (l > top) || reset_top);
+        tail_resid_restrict<T>(P.lv[l], P.lv[l + 1]);
+    }
+    tail_bottom<T>(P, bot);
+    for (int l = bot - 1; l >= top; --l) {
+        tail_prolong<T>(P.lv[l], P.lv[l + 1]);
+        tail_smooth<T>(P.lv[l], P.n_post, false);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_tail(const TailPlan<T>* __restrict__ plan, const int* done)
+{
+    if (done && *done) return;
+    const TailPlan<T>& P = *plan;
+    tail_vcycle<T>(P, 0, P.user_last, true);
+}
+
+// Dense bottom operator: G = (M + c 1 1^T)^-1 (I - 1 1^T / n) (-V), with
+// M = -V A the V-scaled symmetric matrix of poissMat.f90:746-763.  x = G q is the
+// Lagrange-constrained exact solve (poissMat.f90:246-259,467-503, sum x = 0).
+// Gauss-Jordan on the symmetric positive definite M + c 1 1^T, one workgroup.
+template <class T>
+__global__ void __launch_bounds__(1024) k_dense_setup(LevelDev<double> L, const T* __restrict__ beta_t,
+                                                     const double* __restrict__ vol, double* __restrict__ W,
+                                                     double* __restrict__ G)
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int nb = nx * ny * nz;
+    double* A = W;                        // n x n: M + c 1 1^T, reduced to the identity
+    double* B = W + (size_t)nb * nb;      // n x n: identity, becomes the inverse
+    double* bet = W + 2 * (size_t)nb * nb;
+    __shared__ double s_trace;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) bet[t] = (double)beta_t[t];
+    for (size_t t = threadIdx.x; t < (size_t)nb * nb; t += blockDim.x) {
+        A[t] = 0.0;
+        B[t] = ((t / nb) == (t % nb)) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // rows of M: for each face with neighbour nb != c: M[c][nb] -= V a_f, M[c][c] += V a_f
+    for (int c = threadIdx.x; c < nb; c += blockDim.x) {
+        const int i = c % nx, r = c / nx, j = r % ny, k = r / ny;
+        size_t cc, e, w, n, s, f, b;
+        neighbours(i, j, k, nx, ny, nz, cc, e, w, n, s, f, b);
+        Coef<double> a = coefs<double>(L, bet, i, j, k, cc, e, w, n, s, f, b);
+        const double v = vol[c];
+        const size_t nbr[6] = {e, w, n, s, f, b};
+        const double af[6] = {a.aR, a.aL, a.aT, a.aB, a.aF, a.aK};
+        for (int q = 0; q < 6; ++q) {
+            if (nbr[q] == cc) continue;
+            A[(size_t)c * nb + nbr[q]] -= v * af[q];
+            A[(size_t)c * nb + c] += v * af[q];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tr = 0.0;
+        for (int c = 0; c < nb; ++c) tr += A[(size_t)c * nb + c];
+        s_trace = (tr > 0.0 ? tr : 1.0) / ((double)nb * nb);
+    }
+    __syncthreads();
+    const double cshift = s_trace;
+    for (size_t t = threadIdx.x; t < (size_t)nb * nb; t += blockDim.x) A[t] += cshift;
+    __syncthreads();
+    // Gauss-Jordan without pivoting (SPD)
+    for (int piv = 0; piv < nb; ++piv) {
+        const double inv = 1.0 / A[(size_t)piv * nb + piv];
+        __syncthreads();
+        for (int c = threadIdx.x; c < nb; c += blockDim.x) {
+            A[(size_t)piv * nb + c] *= inv;
+            B[(size_t)piv * nb + c] *= inv;
+        }
+        __syncthreads();
+        for (size_t t = threadIdx.x; t < (size_t)nb * nb; t += blockDim.x) {
+            const int r = (int)(t / nb), c = (int)(t % nb);
+            if (r == piv) continue;
+            const double fct = A[(size_t)r * nb + piv];
+            if (c != piv) A[t] -= fct * A[(size_t)piv * nb + c];
+            B[t] -= fct * B[(size_t)piv * nb + c];
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < nb; r += blockDim.x)
+            if (r != piv) A[(size_t)r * nb + piv] = 0.0;
+        __syncthreads();
+    }
+    // G[i][j] = -V_j (B[i][j] - rowmean_i(B))
+    for (int r = threadIdx.x; r < nb; r += blockDim.x) {
+        double s = 0.0;
+        for (int c = 0; c < nb; ++c) s += B[(size_t)r * nb + c];
+        s /= (double)nb;
+        for (int c = 0; c < nb; ++c) G[(size_t)r * nb + c] = -vol[c] * (B[(size_t)r * nb + c] - s);
+    }
+}
+
+// ------------------------------------------------------------ host launches
+
+namespace {
+dim3 block2d(int nx_threads)
+{
+    int bx = next_pow2(nx_threads < 1 ? 1 : nx_threads);
+    if (bx > 64) bx = 64;
+    return dim3(bx, 256 / bx, 1);
+}
+inline hipStream_t S(void* s) { return (hipStream_t)s; }
+}  // namespace
+
+int launch_apply_op(const LevelDev<double>& L, const double* beta, const double* p, double* q, void* stream)
+{
+    dim3 blk = block2d(L.nx);
+    dim3 grd(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), L.nz);
+    hipLaunchKernelGGL(k_apply_op, grd, blk, 0, S(stream), L, beta, p, q);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+unsigned cell_blocks(const LevelDev<double>& L)
+{
+    dim3 blk = block2d(L.nx);
+    return ceil_div(L.nx, blk.x) * ceil_div(L.ny, blk.y) * (unsigned)L.nz;
+}
+
+int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d, double* qv, PcgState* st,
+                  const Reducer& red, void* stream)
+{
+    dim3 blk = block2d(L.nx);
+    dim3 grd(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), L.nz);
+    hipLaunchKernelGGL(k_op_dot, grd, blk, 0, S(stream), L, beta, d, qv, st, red);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
+                         double* r, PcgState* st, const Reducer& red, void* stream)
+{
+    dim3 blk = block2d(L.nx);
+    dim3 grd(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), L.nz);
+    hipLaunchKernelGGL(k_residual_init, grd, blk, 0, S(stream), L, beta, q, psi, r, st, red);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+int launch_pcg_update(size_t n, double* psi, double* r, const double* d, const double* qv, PcgState* st,
+                      const Reducer& red, void* stream)
+{
+    hipLaunchKernelGGL(k_pcg_update, dim3(ceil_div(n, 256)), dim3(256), 0, S(stream), n, psi, r, d, qv, st, red);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class TZ>
+int launch_pcg_dots(size_t n, const double* r, const TZ* z, const double* qv, int init, PcgState* st,
+                    const Reducer& red, void* stream)
+{
+    hipLaunchKernelGGL(k_pcg_dots<TZ>, dim3(ceil_div(n, 256)), dim3(256), 0, S(stream), n, r, z, qv, init, st, red);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class TZ>
+int launch_pcg_dir(size_t n, const TZ* z, double* d, int init, const PcgState* st, void* stream)
+{
+    hipLaunchKernelGGL(k_pcg_dir<TZ>, dim3(ceil_div(n, 256)), dim3(256), 0, S(stream), n, z, d, init, st);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class T, class TQ>
+int launch_rbgs(const LevelDev<T>& L, const T* beta, const TQ* q, T* p, int colour, bool first, const int* done,
+                void* stream)
+{
+    dim3 blk = block2d(L.nx >> 1);
+    dim3 grd(ceil_div(L.nx >> 1, blk.x), ceil_div(L.ny, blk.y), L.nz);
+    if (colour == RED) {
+        if (first)
+            hipLaunchKernelGGL((k_rbgs<T, TQ, RED, true>), grd, blk, 0, S(stream), L, beta, q, p, done);
+        else
+            hipLaunchKernelGGL((k_rbgs<T, TQ, RED, false>), grd, blk, 0, S(stream), L, beta, q, p, done);
+    } else {
+        if (first)
+            hipLaunchKernelGGL((k_rbgs<T, TQ, BLACK, true>), grd, blk, 0, S(stream), L, beta, q, p, done);
+        else
+            hipLaunchKernelGGL((k_rbgs<T, TQ, BLACK, false>), grd, blk, 0, S(stream), L, beta, q, p, done);
+    }
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class T, class TQ>
+int launch_resid_restrict(const LevelDev<T>& L, const T* beta, const TQ* q, const T* p, T* qc, const int* done,
+                          void* stream)
+{
+    dim3 blk = block2d(L.nx >> 1);
+    dim3 grd(ceil_div(L.nx >> 1, blk.x), ceil_div(L.ny >> 1, blk.y), L.nz >> 1);
+    hipLaunchKernelGGL((k_resid_restrict<T, TQ>), grd, blk, 0, S(stream), L, beta, q, p, qc, done);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class TF, class T>
+int launch_restrict(const LevelDev<T>& L, const TF* fine, T* coarse, void* stream)
+{
+    dim3 blk = block2d(L.nx >> 1);
+    dim3 grd(ceil_div(L.nx >> 1, blk.x), ceil_div(L.ny >> 1, blk.y), L.nz >> 1);
+    hipLaunchKernelGGL((k_restrict<TF, T>), grd, blk, 0, S(stream), L, fine, coarse);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class T>
+int launch_prolong_add(const LevelDev<T>& L, const T* pc, T* p, const int* done, void* stream)
+{
+    dim3 blk = block2d(L.nx);
+    dim3 grd(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), L.nz);
+    hipLaunchKernelGGL(k_prolong_add<T>, grd, blk, 0, S(stream), L, pc, p, done);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class TS, class TD>
+int launch_convert(size_t n, const TS* src, TD* dst, void* stream)
+{
+    hipLaunchKernelGGL((k_convert<TS, TD>), dim3(ceil_div(n, 256)), dim3(256), 0, S(stream), n, src, dst);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class T>
+int launch_tail(const TailPlan<T>* plan_dev, const int* done, void* stream)
+{
+    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), 0, S(stream), plan_dev, done);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class T>
+int launch_dense_setup(const LevelDev<double>& L, const T* beta, const double* vol, double* work, double* G,
+                       void* stream)
+{
+    hipLaunchKernelGGL(k_dense_setup<T>, dim3(1), dim3(1024), 0, S(stream), L, beta, vol, work, G);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+// explicit instantiations
+template int launch_pcg_dots<double>(size_t, const double*, const double*, const double*, int, PcgState*,
+                                     const Reducer&, void*);
+template int launch_pcg_dots<float>(size_t, const double*, const float*, const double*, int, PcgState*,
+                                    const Reducer&, void*);
+template int launch_pcg_dir<double>(size_t, const double*, double*, int, const PcgState*, void*);
+template int launch_pcg_dir<float>(size_t, const float*, double*, int, const PcgState*, void*);
+template int launch_rbgs<double, double>(const LevelDev<double>&, const double*, const double*, double*, int, bool,
+                                         const int*, void*);
+template int launch_rbgs<float, double>(const LevelDev<float>&, const float*, const double*, float*, int, bool,
+                                        const int*, void*);
+template int launch_rbgs<float, float>(const LevelDev<float>&, const float*, const float*, float*, int, bool,
+                                       const int*, void*);
+template int launch_resid_restrict<double, double>(const LevelDev<double>&, const double*, const double*,
+                                                   const double*, double*, const int*, void*);
+template int launch_resid_restrict<float, double>(const LevelDev<float>&, const float*, const double*, const float*,
+                                                  float*, const int*, void*);
+template int launch_resid_restrict<float, float>(const LevelDev<float>&, const float*, const float*, const float*,
+                                                 float*, const int*, void*);
+template int launch_restrict<double, double>(const LevelDev<double>&, const double*, double*, void*);
+template int launch_restrict<float, float>(const LevelDev<float>&, const float*, float*, void*);
+template int launch_restrict<double, float>(const LevelDev<float>&, const double*, float*, void*);
+template int launch_prolong_add<double>(const LevelDev<double>&, const double*, double*, const int*, void*);
+template int launch_prolong_add<float>(const LevelDev<float>&, const float*, float*, const int*, void*);
+template int launch_convert<double, float>(size_t, const double*, float*, void*);
+template int launch_convert<float, double>(size_t, const float*, double*, void*);
+template int launch_tail<double>(const TailPlan<double>*, const int*, void*);
+template int launch_tail<float>(const TailPlan<float>*, const int*, void*);
+template int launch_dense_setup<double>(const LevelDev<double>&, const double*, const double*, double*, double*,
+                                        void*);
+template int launch_dense_setup<float>(const LevelDev<double>&, const float*, const double*, double*, double*,
+                                       void*);
+
+}  // namespace tbfg

@@ -1,0 +1,61 @@
+// mg_kernels.hpp -- device code of the multigrid-preconditioned CG path.
+//
+// Index conventions inside kernels are 0-based: cell (i,j,k), i fastest,
+// c = i + nx*(j + ny*k).  x and z wrap periodically, y clamps to the first/last
+// interior cell (homogeneous Neumann ghost = interior, boundaryField_S.f90:713-725).
+// The red/black colour is the reference's parity of the 1-based index sum
+// (rbgs.f90:121): colour(i,j,k) = (i+j+k+3) mod 2 with 0 = red (updated first).
+#pragma once
+#include "common.hpp"
+
+namespace tbfg {
+
+constexpr int RED = 0;
+constexpr int BLACK = 1;
+constexpr int kMaxTail = 16;  // levels handled by the single-workgroup tail kernel
+
+// Scalars of one PCG solve, resident on the device so that no host round trip
+// sits between the kernels of an iteration.
+struct PcgState {
+    double delta;  // r.z of the current direction (pcg.f90:120,131)
+    double alpha;  // delta / (d.Ad)             (pcg.f90:222)
+    double gamma;  // direction update factor     (pcg.f90:132)
+    double rr;     // ||r||^2
+    double qq;     // ||q||^2
+    double res;    // ||r|| / ||q||
+    double zmean;  // arithmetic mean removed from the preconditioned residual
+    double tol;
+    int32_t iter;      // direction updates made
+    int32_t max_iter;
+    int32_t done;      // 1: stop (converged, max_iter, or numeric failure)
+    int32_t status;    // TBFG_OK / TBFG_ENOTCONVERGED / TBFG_ENUMERIC
+    int32_t flexible;  // 0 Fletcher-Reeves, 1 Polak-Ribiere
+    int32_t pad;
+};
+
+// Deterministic grid reduction: every block writes NV partial sums, the last
+// block to finish adds the partials in block order and calls the finisher.
+struct Reducer {
+    double* partials;    // >= blocks * NV
+    unsigned* counter;   // zero between uses (the last block resets it)
+};
+
+template <class T>
+struct TailLevel {
+    LevelDev<T> geo;
+    T* p;
+    T* q;
+    const T* beta;
+};
+
+template <class T>
+struct TailPlan {
+    int nlev;          // number of tail levels (index 0 = finest tail level)
+    int user_last;     // tail index of the reference's coarsest level
+    int bottom;        // tail index of the dense bottom level
+    int n_pre, n_post, coarse_cycles;
+    const double* G;   // bottom dense operator, row-major: x = G q
+    TailLevel<T> lv[kMaxTail];
+};
+
+}  // namespace tbfg

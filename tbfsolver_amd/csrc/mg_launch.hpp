@@ -1,0 +1,40 @@
+// mg_launch.hpp -- host-side launchers of the multigrid / PCG kernels.
+#pragma once
+#include "mg_kernels.hpp"
+
+namespace tbfg {
+
+unsigned cell_blocks(const LevelDev<double>& L);
+
+int launch_apply_op(const LevelDev<double>& L, const double* beta, const double* p, double* q, void* stream);
+int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d, double* qv, PcgState* st,
+                  const Reducer& red, void* stream);
+int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
+                         double* r, PcgState* st, const Reducer& red, void* stream);
+int launch_pcg_update(size_t n, double* psi, double* r, const double* d, const double* qv, PcgState* st,
+                      const Reducer& red, void* stream);
+template <class TZ>
+int launch_pcg_dots(size_t n, const double* r, const TZ* z, const double* qv, int init, PcgState* st,
+                    const Reducer& red, void* stream);
+template <class TZ>
+int launch_pcg_dir(size_t n, const TZ* z, double* d, int init, const PcgState* st, void* stream);
+
+template <class T, class TQ>
+int launch_rbgs(const LevelDev<T>& L, const T* beta, const TQ* q, T* p, int colour, bool first, const int* done,
+                void* stream);
+template <class T, class TQ>
+int launch_resid_restrict(const LevelDev<T>& L, const T* beta, const TQ* q, const T* p, T* qc, const int* done,
+                          void* stream);
+template <class TF, class T>
+int launch_restrict(const LevelDev<T>& L, const TF* fine, T* coarse, void* stream);
+template <class T>
+int launch_prolong_add(const LevelDev<T>& L, const T* pc, T* p, const int* done, void* stream);
+template <class TS, class TD>
+int launch_convert(size_t n, const TS* src, TD* dst, void* stream);
+template <class T>
+int launch_tail(const TailPlan<T>* plan_dev, const int* done, void* stream);
+template <class T>
+int launch_dense_setup(const LevelDev<double>& L, const T* beta, const double* vol, double* work, double* G,
+                       void* stream);
+
+}  // namespace tbfg

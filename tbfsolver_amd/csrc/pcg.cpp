@@ -1,0 +1,387 @@
+// pcg.cpp -- host driver of the PCG + multigrid solve.
+//
+// Reference flow (pcg.f90:103-140): r = q - A psi; d = M r; delta = r.d; loop
+// { qv = A d; alpha = delta/(d.qv); psi += alpha d; r -= alpha qv; z = M r;
+// gamma = r.z/delta; d = z + gamma d } until ||r||/||q|| <= tolPCG.  M is one
+// V-cycle (multiGrid.f90:186-232) with red/black Gauss-Seidel smoothing and an
+// exact solve on the coarsest level (poissMat.f90:289-312).
+//
+// Device design: every scalar of the iteration lives in a PcgState on the
+// device; kernels read alpha/gamma from it and the last block of each
+// reduction writes them, so an iteration is a chain of launches with one
+// host synchronisation at its end (the stop test).  Levels with at most
+// kTailCells cells, the reference's coarsest level and the sub-hierarchy that
+// approximates its exact solve run inside one single-workgroup kernel.
+#include "pcg.hpp"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "mg_launch.hpp"
+
+using namespace tbfg;
+
+namespace {
+
+size_t tail_cells_limit()
+{
+    const char* e = std::getenv("TBFG_TAIL_CELLS");
+    if (e && *e) {
+        long v = std::atol(e);
+        if (v > 0) return (size_t)v;
+    }
+    return 32768;
+}
+
+constexpr int kMaxBottomCells = 512;
+
+}  // namespace
+
+tbfg_pcg::~tbfg_pcg()
+{
+    if (hst) (void)hipHostFree(hst);
+    if (hstage) (void)hipHostFree(hstage);
+}
+
+namespace tbfg {
+
+template <class T>
+static const LevelDev<T>& geo(tbfg_pcg* h, int l);
+template <>
+const LevelDev<double>& geo<double>(tbfg_pcg* h, int l) { return h->tabd[l]->view; }
+template <>
+const LevelDev<float>& geo<float>(tbfg_pcg* h, int l) { return h->tabf[l]->view; }
+
+template <class T>
+static int build_plan(tbfg_pcg* h)
+{
+    TailPlan<T> P;
+    std::memset(&P, 0, sizeof P);
+    P.nlev = h->ntot - h->t0;
+    P.user_last = (h->nuser - 1) - h->t0;
+    P.bottom = (h->ntot - 1) - h->t0;
+    P.n_pre = h->prm.n_pre;
+    P.n_post = h->prm.n_post;
+    P.coarse_cycles = h->prm.coarse_cycles;
+    P.G = h->G.as<double>();
+    for (int m = 0; m < P.nlev; ++m) {
+        const int l = h->t0 + m;
+        P.lv[m].geo = geo<T>(h, l);
+        P.lv[m].p = h->p[l].as<T>();
+        P.lv[m].q = h->q[l].as<T>();
+        P.lv[m].beta = h->beta[l].as<T>();
+    }
+    return upload(h->plan, &P, sizeof P);
+}
+
+int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg** out)
+{
+    if (!out || !prm_in) {
+        set_error("tbfg_pcg_create: NULL argument");
+        return TBFG_EINVAL;
+    }
+    *out = nullptr;
+    TBFG_TRY(validate_grid(g));
+    tbfg_pcg_params prm = *prm_in;
+    if (prm.n_levels < 2 || prm.n_levels > 12) {
+        set_error("n_levels = %d (expected 2..12)", prm.n_levels);
+        return TBFG_EINVAL;
+    }
+    if (prm.n_pre < 0 || prm.n_post < 0 || prm.n_pre + prm.n_post < 1 || prm.mg_cycles < 1 || prm.max_iter < 0 ||
+        !(prm.tol >= 0.0) || prm.coarse_cycles < 0) {
+        set_error("invalid PCG/MG parameters (pre %d post %d cycles %d max_iter %d tol %g coarse %d)", prm.n_pre,
+                  prm.n_post, prm.mg_cycles, prm.max_iter, prm.tol, prm.coarse_cycles);
+        return TBFG_EINVAL;
+    }
+    if (prm.precision != TBFG_PREC_FP64 && prm.precision != TBFG_PREC_MIXED) {
+        set_error("precision = %d", prm.precision);
+        return TBFG_EINVAL;
+    }
+    // multiGrid.f90:116-145: every axis divisible by 2^(nLevels-1)
+    for (int a = 0; a < 3; ++a)
+        if (g->n[a] % (1 << (prm.n_levels - 1)) != 0) {
+            set_error("Attempt to initialize a MG grid with a non-integer number of cells along axis %d "
+                      "(n=%d, levels=%d)",
+                      a, g->n[a], prm.n_levels);
+            return TBFG_EINVAL;
+        }
+    std::unique_ptr<tbfg_pcg> h(new tbfg_pcg());
+    h->g = *g;
+    h->prm = prm;
+    h->mixed = prm.precision == TBFG_PREC_MIXED;
+    if (h->prm.flexible < 0) h->prm.flexible = h->mixed ? 1 : 0;
+    h->nuser = prm.n_levels;
+    // sub-hierarchy under the coarsest level: halve while every axis is even
+    int ntot = h->nuser;
+    if (prm.coarse_cycles > 0) {
+        for (;;) {
+            bool ok = true;
+            size_t cells = 1;
+            for (int a = 0; a < 3; ++a) {
+                int nn = g->n[a] >> (ntot - 1);
+                cells *= (size_t)nn;
+                if (nn % 2 != 0 || nn < 2) ok = false;
+            }
+            if (!ok || cells <= 64 || ntot >= 20) break;
+            ntot++;
+        }
+    }
+    h->ntot = ntot;
+    {
+        size_t cb = 1;
+        for (int a = 0; a < 3; ++a) cb *= (size_t)(g->n[a] >> (ntot - 1));
+        if (cb > (size_t)kMaxBottomCells) {
+            set_error("the coarsest grid has %zu cells; the dense coarsest solve needs <= %d (use more levels "
+                      "or coarse_cycles > 0 on an axis-even grid)",
+                      cb, kMaxBottomCells);
+            return TBFG_EINVAL;
+        }
+    }
+    const size_t tail_lim = tail_cells_limit();
+    int t0 = h->nuser - 1;
+    for (int l = 1; l < h->nuser; ++l) {
+        size_t cells = 1;
+        for (int a = 0; a < 3; ++a) cells *= (size_t)(g->n[a] >> l);
+        if (cells <= tail_lim) {
+            t0 = l;
+            break;
+        }
+    }
+    h->t0 = t0;
+    if (h->ntot - h->t0 > kMaxTail) {
+        set_error("hierarchy too deep for the tail kernel (%d levels)", h->ntot - h->t0);
+        return TBFG_EINVAL;
+    }
+    for (int l = 0; l < ntot; ++l) h->hl.push_back(make_level(g, l));
+    h->n0 = h->hl[0].cells();
+    // tables
+    h->tabd.resize(ntot);
+    for (int l = 0; l < ntot; ++l) {
+        h->tabd[l].reset(new LevelTables<double>());
+        TBFG_TRY(h->tabd[l]->build(h->hl[l], l + 1 < ntot ? &h->hl[l + 1] : nullptr));
+    }
+    if (h->mixed) {
+        h->tabf.resize(ntot);
+        for (int l = 0; l < ntot; ++l) {
+            h->tabf[l].reset(new LevelTables<float>());
+            TBFG_TRY(h->tabf[l]->build(h->hl[l], l + 1 < ntot ? &h->hl[l + 1] : nullptr));
+        }
+    }
+    const size_t es = h->mixed ? sizeof(float) : sizeof(double);
+    h->p.resize(ntot);
+    h->q.resize(ntot);
+    h->beta.resize(ntot);
+    for (int l = 0; l < ntot; ++l) {
+        const size_t n = h->hl[l].cells();
+        TBFG_TRY(h->p[l].alloc(n * es));
+        if (l > 0) TBFG_TRY(h->q[l].alloc(n * es));
+        if (l > 0 || h->mixed) TBFG_TRY(h->beta[l].alloc(n * es));
+    }
+    TBFG_TRY(h->r.alloc(h->n0 * 8));
+    TBFG_TRY(h->d.alloc(h->n0 * 8));
+    TBFG_TRY(h->qv.alloc(h->n0 * 8));
+    TBFG_TRY(h->st.alloc(sizeof(PcgState)));
+    const unsigned nb = std::max(cell_blocks(h->tabd[0]->view), ceil_div(h->n0, 256));
+    TBFG_TRY(h->partials.alloc((size_t)nb * 8 * sizeof(double)));
+    TBFG_TRY(h->counter.alloc(256));
+    TBFG_HIP(hipMemset(h->counter.p, 0, 256));
+    // dense bottom operator
+    const HostLevel& B = h->hl[ntot - 1];
+    const size_t nbc = B.cells();
+    TBFG_TRY(h->G.alloc(nbc * nbc * 8));
+    TBFG_TRY(h->dwork.alloc((2 * nbc * nbc + nbc + 16) * 8));
+    {
+        std::vector<double> vol(nbc);
+        for (int k = 1; k <= B.n[2]; ++k)
+            for (int j = 1; j <= B.n[1]; ++j)
+                for (int i = 1; i <= B.n[0]; ++i)
+                    vol[(i - 1) + (size_t)B.n[0] * ((j - 1) + (size_t)B.n[1] * (k - 1))] =
+                        B.ax[0].df(i) * B.ax[1].df(j) * B.ax[2].df(k);
+        TBFG_TRY(upload(h->vol, vol.data(), nbc * 8));
+    }
+    if (h->mixed)
+        TBFG_TRY(build_plan<float>(h.get()));
+    else
+        TBFG_TRY(build_plan<double>(h.get()));
+    TBFG_HIP(hipHostMalloc((void**)&h->hst, sizeof(PcgState), hipHostMallocDefault));
+    std::memset(h->hst, 0, sizeof(PcgState));
+    // footprint: every device buffer of the handle
+    int64_t fp = 0;
+    for (auto& b : h->p) fp += (int64_t)b.bytes;
+    for (auto& b : h->q) fp += (int64_t)b.bytes;
+    for (auto& b : h->beta) fp += (int64_t)b.bytes;
+    for (auto& t : h->tabd) fp += (int64_t)t->buf.bytes;
+    for (auto& t : h->tabf) fp += (int64_t)t->buf.bytes;
+    fp += (int64_t)(h->r.bytes + h->d.bytes + h->qv.bytes + h->st.bytes + h->partials.bytes + h->counter.bytes +
+                    h->plan.bytes + h->G.bytes + h->dwork.bytes + h->vol.bytes);
+    h->footprint = fp;
+    *out = h.release();
+    return TBFG_OK;
+}
+
+// beta on every level (restricted once per solve; multiGrid.f90:210 restricts it
+// every V-cycle, but it does not change within a solve) and the dense bottom.
+template <class T>
+static int setup_levels(tbfg_pcg* h, const double* beta0, void* stream)
+{
+    const T* prev;
+    if constexpr (sizeof(T) == 4) {
+        TBFG_TRY(launch_convert<double, float>(h->n0, beta0, h->beta[0].as<float>(), stream));
+        prev = h->beta[0].as<T>();
+    } else {
+        prev = beta0;
+    }
+    for (int l = 1; l < h->ntot; ++l) {
+        TBFG_TRY(launch_restrict<T, T>(geo<T>(h, l - 1), prev, h->beta[l].as<T>(), stream));
+        prev = h->beta[l].as<T>();
+    }
+    TBFG_TRY(launch_dense_setup<T>(h->tabd[h->ntot - 1]->view, prev, h->vol.as<double>(), h->dwork.as<double>(),
+                                   h->G.as<double>(), stream));
+    return TBFG_OK;
+}
+
+// One preconditioner application z = M r (multiGrid.f90:165-232, fixed cycles).
+template <class T>
+static int vcycle(tbfg_pcg* h, const double* r, T* z, const T* beta0, const int* done, void* stream)
+{
+    const int t0 = h->t0;
+    for (int cyc = 0; cyc < h->prm.mg_cycles; ++cyc) {
+        for (int l = 0; l < t0; ++l) {
+            const bool first = (l > 0) || (cyc == 0);
+            T* p = (l == 0) ? z : h->p[l].as<T>();
+            const T* b = (l == 0) ? beta0 : h->beta[l].as<T>();
+            if (h->prm.n_pre == 0 && first) TBFG_HIP(hipMemsetAsync(p, 0, h->hl[l].cells() * sizeof(T), (hipStream_t)stream));
+            for (int s = 0; s < h->prm.n_pre; ++s) {
+                const bool f0 = first && s == 0;
+                if (l == 0) {
+                    TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, RED, f0, done, stream));
+                    TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, BLACK, f0, done, stream));
+                } else {
+                    const T* q = h->q[l].as<T>();
+                    TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, RED, f0, done, stream));
+                    TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, BLACK, f0, done, stream));
+                }
+            }
+            if (l == 0)
+                TBFG_TRY(launch_resid_restrict<T, double>(geo<T>(h, 0), b, r, p, h->q[1].as<T>(), done, stream));
+            else
+                TBFG_TRY(launch_resid_restrict<T, T>(geo<T>(h, l), b, h->q[l].as<T>(), p, h->q[l + 1].as<T>(), done,
+                                                     stream));
+        }
+        TBFG_TRY(launch_tail<T>(h->plan.as<TailPlan<T>>(), done, stream));
+        for (int l = t0 - 1; l >= 0; --l) {
+            T* p = (l == 0) ? z : h->p[l].as<T>();
+            const T* b = (l == 0) ? beta0 : h->beta[l].as<T>();
+            TBFG_TRY(launch_prolong_add<T>(geo<T>(h, l), h->p[l + 1].as<T>(), p, done, stream));
+            for (int s = 0; s < h->prm.n_post; ++s) {
+                if (l == 0) {
+                    TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, RED, false, done, stream));
+                    TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, BLACK, false, done, stream));
+                } else {
+                    const T* q = h->q[l].as<T>();
+                    TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, RED, false, done, stream));
+                    TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, BLACK, false, done, stream));
+                }
+            }
+        }
+    }
+    return TBFG_OK;
+}
+
+template <class T>
+static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t* iters, double* res,
+                   void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    PcgState* st = h->st.as<PcgState>();
+    const int* done = &st->done;
+    Reducer red{h->partials.as<double>(), h->counter.as<unsigned>()};
+    std::memset(h->hst, 0, sizeof(PcgState));
+    h->hst->tol = h->prm.tol;
+    h->hst->max_iter = h->prm.max_iter;
+    h->hst->flexible = h->prm.flexible;
+    TBFG_HIP(hipMemcpyAsync(st, h->hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    TBFG_TRY(setup_levels<T>(h, beta, stream));
+    const T* beta0;
+    if constexpr (sizeof(T) == 4)
+        beta0 = h->beta[0].as<T>();
+    else
+        beta0 = beta;
+    T* z = h->p[0].as<T>();
+    const LevelDev<double>& L0 = h->tabd[0]->view;
+    const size_t n = h->n0;
+    double* r = h->r.as<double>();
+    double* d = h->d.as<double>();
+    double* qv = h->qv.as<double>();
+    TBFG_TRY(launch_residual_init(L0, beta, q, psi, r, st, red, stream));
+    TBFG_TRY(vcycle<T>(h, r, z, beta0, done, stream));
+    TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 1, st, red, stream));
+    TBFG_TRY(launch_pcg_dir<T>(n, z, d, 1, st, stream));
+    TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    TBFG_HIP(hipStreamSynchronize(s));
+    while (!h->hst->done) {
+        TBFG_TRY(launch_op_dot(L0, beta, d, qv, st, red, stream));
+        TBFG_TRY(launch_pcg_update(n, psi, r, d, qv, st, red, stream));
+        TBFG_TRY(vcycle<T>(h, r, z, beta0, done, stream));
+        TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 0, st, red, stream));
+        TBFG_TRY(launch_pcg_dir<T>(n, z, d, 0, st, stream));
+        TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        TBFG_HIP(hipStreamSynchronize(s));
+    }
+    if (iters) *iters = h->hst->iter;
+    if (res) *res = h->hst->res;
+    if (h->hst->status == TBFG_ENUMERIC) {
+        set_error("PCG: non-finite value (res=%g) after %d iterations", h->hst->res, h->hst->iter);
+        return TBFG_ENUMERIC;
+    }
+    if (h->hst->status == TBFG_ENOTCONVERGED) {
+        set_error("EXIT PCG iterations: max iter reached (%d, res=%g)", h->hst->iter, h->hst->res);
+        return TBFG_ENOTCONVERGED;
+    }
+    return TBFG_OK;
+}
+
+int pcg_solve_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t* iters, double* res,
+                  void* stream)
+{
+    if (!h || !psi || !beta || !q) {
+        set_error("tbfg_pcg_solve_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    return h->mixed ? solve_t<float>(h, psi, beta, q, iters, res, stream)
+                    : solve_t<double>(h, psi, beta, q, iters, res, stream);
+}
+
+template <class T>
+static int precond_t(tbfg_pcg* h, const double* beta, const double* r, double* z, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    PcgState* st = h->st.as<PcgState>();
+    Reducer red{h->partials.as<double>(), h->counter.as<unsigned>()};
+    std::memset(h->hst, 0, sizeof(PcgState));
+    TBFG_HIP(hipMemcpyAsync(st, h->hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    TBFG_TRY(setup_levels<T>(h, beta, stream));
+    const T* beta0;
+    if constexpr (sizeof(T) == 4)
+        beta0 = h->beta[0].as<T>();
+    else
+        beta0 = beta;
+    T* zt = h->p[0].as<T>();
+    TBFG_TRY(vcycle<T>(h, r, zt, beta0, nullptr, stream));
+    // z with its arithmetic mean removed, as the reference's sweeps leave it
+    TBFG_TRY(launch_pcg_dots<T>(h->n0, r, zt, nullptr, 1, st, red, stream));
+    TBFG_TRY(launch_pcg_dir<T>(h->n0, zt, z, 1, st, stream));
+    return TBFG_OK;
+}
+
+int pcg_precondition_dev(tbfg_pcg* h, const double* beta, const double* r, double* z, void* stream)
+{
+    if (!h || !beta || !r || !z) {
+        set_error("tbfg_pcg_precondition_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    return h->mixed ? precond_t<float>(h, beta, r, z, stream) : precond_t<double>(h, beta, r, z, stream);
+}
+
+}  // namespace tbfg

@@ -1,0 +1,41 @@
+// pcg.hpp -- the MG_MODE solver handle: PCG with a geometric multigrid
+// preconditioner (pcg.f90, multiGrid.f90, rbgs.f90, poissMat.f90 of the reference).
+#pragma once
+#include <memory>
+
+#include "mg_kernels.hpp"
+
+struct tbfg_pcg {
+    tbfg_grid_desc g{};
+    tbfg_pcg_params prm{};
+    bool mixed = false;
+    int nuser = 0;   // reference levels (nLevels)
+    int ntot = 0;    // reference levels + sub-levels of the coarsest-solve hierarchy
+    int t0 = 0;      // first level run by the tail kernel
+    size_t n0 = 0;   // finest cells
+
+    std::vector<tbfg::HostLevel> hl;
+    std::vector<std::unique_ptr<tbfg::LevelTables<double>>> tabd;  // FP64 tables (all levels)
+    std::vector<std::unique_ptr<tbfg::LevelTables<float>>> tabf;   // FP32 tables (mixed)
+
+    // per-level MG arrays (element type float in mixed mode, double otherwise);
+    // level 0: p = z, q = r (double), beta = caller's (FP64) or a float copy (mixed)
+    std::vector<tbfg::DevBuf> p, q, beta;
+    tbfg::DevBuf r, d, qv;              // CG vectors (FP64)
+    tbfg::DevBuf st, partials, counter; // PcgState + reduction scratch
+    tbfg::DevBuf plan;                  // TailPlan<T>
+    tbfg::DevBuf G, dwork, vol;         // dense bottom operator and its set-up scratch
+    tbfg::DevBuf hpsi, hbeta, hq;       // device copies for the host-array entry point
+    tbfg::PcgState* hst = nullptr;      // pinned mirror of the device state
+    double* hstage = nullptr;           // pinned staging (3 * n0 doubles)
+    int64_t footprint = 0;
+
+    ~tbfg_pcg();
+};
+
+namespace tbfg {
+int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm, tbfg_pcg** out);
+int pcg_solve_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t* iters, double* res,
+                  void* stream);
+int pcg_precondition_dev(tbfg_pcg* h, const double* beta, const double* r, double* z, void* stream);
+}  // namespace tbfg

@@ -1,0 +1,195 @@
+"""GPU parity of the MG_MODE path (PCG + multigrid) against the CPU oracle.
+
+Every call goes through the C ABI of libtbfgpu.so.  Tolerances: the 7-point
+operator and the field kernels agree with the oracle to round-off (1e-13
+relative); the converged FP64 solves agree to a relative L2 error of 1e-10 on
+the mean-free pressure (both solved to 1e-12); the mixed-precision solve is
+checked against the FP64 oracle with the bound its tolerance implies.
+"""
+import numpy as np
+import pytest
+
+import tbfsolver_amd as T
+from tbfsolver_amd import fields as F
+from oracle import pyoracle as O
+from conftest import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(77)
+
+
+def grids(n, stretch=0, gamma=1.5):
+    return T.channel_grid(n, stretch_y=stretch, gamma=gamma), O.desc(n, stretch=(0, stretch, 0), gamma=gamma)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def mean_free(a):
+    return a - a.mean()
+
+
+def bubble_case(n, nb, stretch, seed, adt=1e-3):
+    g, go = grids(n, stretch)
+    rho, _ = F.make_bubble_density(g, nb, 0.3, seed, nref=10)
+    ux, uy, uz = F.make_velocity(g, seed)
+    q = O.divergence(go, ux, uy, uz)
+    beta = O.beta_from_rho(go, adt, rho)
+    return g, go, rho, (ux, uy, uz), q, beta
+
+
+@pytest.mark.parametrize("n,stretch", [((64, 64, 64), 0), ((48, 40, 32), 1), ((32, 24, 16), 1)])
+def test_operator_matches_oracle(torch_gpu, n, stretch):
+    print("case operator", n, stretch, flush=True)
+    g, go = grids(n, stretch)
+    beta = np.exp(rng.standard_normal(n))
+    p = rng.standard_normal(n)
+    ref = O.apply_op(go, beta, p)
+    q = torch_gpu.zeros_like(to_dev(p, torch_gpu))
+    F.apply_operator(g, to_dev(beta, torch_gpu), to_dev(p, torch_gpu), q)
+    torch_gpu.cuda.synchronize()
+    out = to_host(q)
+    assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_fields_match_oracle(torch_gpu):
+    print("case fields", flush=True)
+    n = (32, 24, 16)
+    g, go = grids(n, 1)
+    nx, ny, nz = n
+    rho, _ = F.make_bubble_density(g, 4, 0.3, 3, nref=8)
+    ux, uy, uz = F.make_velocity(g, 3)
+    psi = rng.standard_normal(n)
+    adt = 2e-3
+    dux, duy, duz = (to_dev(a, torch_gpu) for a in (ux, uy, uz))
+    drho, dpsi = to_dev(rho, torch_gpu), to_dev(psi, torch_gpu)
+    div = torch_gpu.zeros_like(drho)
+    F.computeSource(g, dux, duy, duz, div)
+    beta = torch_gpu.zeros_like(drho)
+    F.computeBeta(g, adt, drho, beta)
+    torch_gpu.cuda.synchronize()
+    assert np.abs(to_host(div) - O.divergence(go, ux, uy, uz)).max() <= 1e-13 * np.abs(to_host(div)).max()
+    assert np.array_equal(to_host(beta), O.beta_from_rho(go, adt, rho))
+    F.makeVelocityDivFree(g, adt, drho, dpsi, dux, duy, duz)
+    torch_gpu.cuda.synchronize()
+    rx, ry, rz = O.project_mg(go, adt, rho, psi, ux, uy, uz)
+    for a, b in ((dux, rx), (duy, ry), (duz, rz)):
+        assert np.abs(to_host(a) - b).max() <= 1e-14 * np.abs(b).max()
+    md = F.max_divergence(g, dux, duy, duz)
+    assert np.isclose(md, np.abs(O.divergence(go, rx, ry, rz)).max(), rtol=1e-12)
+
+
+@pytest.mark.parametrize("n,stretch,nlev", [((32, 32, 32), 0, 3), ((64, 48, 64), 1, 4)])
+def test_preconditioner_matches_oracle(torch_gpu, n, stretch, nlev):
+    print("case precondition", n, stretch, nlev, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 6, stretch, 11)
+    r = q.copy()
+    s = T.PCG(g, nLevels=nlev, coarse_cycles=6)
+    z = torch_gpu.zeros_like(to_dev(r, torch_gpu))
+    s.precondition(to_dev(beta, torch_gpu), to_dev(r, torch_gpu), z)
+    zr = O.mg_apply(go, O.params(n_levels=nlev, coarse_mode=1, coarse_cycles=6), beta, r)
+    out = to_host(z)
+    assert abs(out.mean()) < 1e-12 * np.abs(out).max()
+    assert rel(mean_free(out), mean_free(zr)) < 1e-11
+
+
+@pytest.mark.parametrize("n,stretch,nlev,nb", [((64, 64, 64), 0, 4, 18), ((64, 48, 64), 1, 4, 20)])
+def test_pcg_fp64_matches_oracle(torch_gpu, n, stretch, nlev, nb):
+    print("case pcg fp64", n, stretch, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, nb, stretch, 21)
+    for flexible in (0, 1):
+        s = T.PCG(g, nLevels=nlev, tolPCG=1e-12, maxIterPCG=200, coarse_cycles=6, flexible=flexible)
+        psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+        ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=nlev, tol=1e-12, max_iter=200, coarse_mode=1,
+                                                    coarse_cycles=6, flexible=flexible), beta, q)
+        assert rc == 0 and s.converged
+        assert abs(s.iterations - it) <= 1, (s.iterations, it)
+        out = to_host(psi)
+        assert rel(mean_free(out), mean_free(ref)) < 1e-10
+        true_res = np.linalg.norm(q - O.apply_op(go, beta, out)) / np.linalg.norm(q)
+        assert true_res < 2e-12
+        # reference semantics of the exact coarsest solve give the same pressure
+        ref0, _, _, _ = O.pcg_solve(go, O.params(n_levels=nlev, tol=1e-12, max_iter=200, coarse_mode=0,
+                                                 flexible=flexible), beta, q)
+        assert rel(mean_free(out), mean_free(ref0)) < 1e-10
+
+
+@pytest.mark.parametrize("n,stretch,nb", [((64, 64, 64), 0, 18), ((64, 48, 64), 1, 20)])
+def test_pcg_mixed_vs_fp64_oracle(torch_gpu, n, stretch, nb):
+    print("case pcg mixed", n, stretch, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, nb, stretch, 31)
+    ref, _, _, _ = O.pcg_solve(go, O.params(n_levels=4, tol=1e-13, max_iter=300, flexible=1), beta, q)
+    for tol in (1e-8, 1e-11):
+        s = T.PCG(g, nLevels=4, tolPCG=tol, maxIterPCG=300, precision="mixed", coarse_cycles=6)
+        psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+        assert s.converged and s.res_ <= tol
+        out = to_host(psi)
+        true_res = np.linalg.norm(q - O.apply_op(go, beta, out)) / np.linalg.norm(q)
+        assert true_res < 2 * tol
+        # error against the FP64 oracle is bounded by the residual times a
+        # modest condition factor
+        assert rel(mean_free(out), mean_free(ref)) < 1e3 * tol
+
+
+def test_host_array_entry_point_with_halo(torch_gpu):
+    print("case pcg host arrays", flush=True)
+    n = (32, 32, 32)
+    g, go, rho, _, q, beta = bubble_case(n, 6, 0, 41)
+    s = T.PCG(g, nLevels=3, tolPCG=1e-12, maxIterPCG=100, coarse_cycles=6)
+    pad = lambda a: np.asfortranarray(np.pad(a, 1))
+    psi_h = np.zeros((34, 34, 34), order="F")
+    s.solvePCG(psi_h, pad(beta), pad(q))
+    it_host = s.iterations
+    psi_d = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+    s.solvePCG(psi_d, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+    assert it_host == s.iterations
+    assert np.array_equal(psi_h[1:-1, 1:-1, 1:-1], to_host(psi_d))
+    # halo: periodic in x and z, Neumann (ghost = interior) in y
+    assert np.array_equal(psi_h[0, 1:-1, 1:-1], psi_h[-2, 1:-1, 1:-1])
+    assert np.array_equal(psi_h[1:-1, 0, 1:-1], psi_h[1:-1, 1, 1:-1])
+    assert np.array_equal(psi_h[1:-1, 1:-1, -1], psi_h[1:-1, 1:-1, 1])
+    assert psi_h[0, 0, 0] == psi_h[-2, 1, -2]
+
+
+def test_max_iter_status_and_warm_start(torch_gpu):
+    print("case pcg status", flush=True)
+    n = (32, 32, 32)
+    g, go, rho, _, q, beta = bubble_case(n, 6, 0, 51)
+    s = T.PCG(g, nLevels=3, tolPCG=1e-14, maxIterPCG=2, coarse_cycles=4)
+    psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+    s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+    assert not s.converged and s.iterations == 2 and s.iter_ == 2
+    s2 = T.PCG(g, nLevels=3, tolPCG=1e-9, maxIterPCG=50, coarse_cycles=4)
+    s2.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+    warm = s2.iterations
+    psi0 = torch_gpu.zeros_like(psi)
+    s2.solvePCG(psi0, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+    assert s2.converged and warm < s2.iterations
+    with pytest.raises(T.TbfgError):
+        bad = to_dev(q, torch_gpu).clone()
+        bad.view(-1)[5] = float("nan")
+        s2.solvePCG(torch_gpu.zeros_like(psi), to_dev(beta, torch_gpu), bad)
+
+
+def test_projection_step_divergence_free(torch_gpu):
+    """solvePCG -> makeVelocityDivFree: div u_new equals the PCG residual."""
+    print("case projection step", flush=True)
+    n = (64, 48, 64)
+    g, go, rho, (ux, uy, uz), q, beta = bubble_case(n, 20, 1, 61)
+    adt = 1e-3
+    dux, duy, duz = (to_dev(a, torch_gpu) for a in (ux, uy, uz))
+    drho = to_dev(rho, torch_gpu)
+    dq = torch_gpu.zeros_like(drho)
+    F.computeSource(g, dux, duy, duz, dq)
+    dbeta = torch_gpu.zeros_like(drho)
+    F.computeBeta(g, adt, drho, dbeta)
+    s = T.PCG(g, nLevels=4, tolPCG=1e-12, maxIterPCG=200, coarse_cycles=6)
+    psi = torch_gpu.zeros_like(drho)
+    s.solvePCG(psi, dbeta, dq)
+    F.makeVelocityDivFree(g, adt, drho, psi, dux, duy, duz)
+    md = F.max_divergence(g, dux, duy, duz)
+    assert md <= 2e-12 * np.linalg.norm(q), (md, np.linalg.norm(q))   # |r|_max <= ||r||_2 <= tol ||q||_2

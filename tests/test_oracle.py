@@ -1,0 +1,420 @@
+"""Pins the CPU oracle (oracle/liboracle.so) before it is trusted as the checker.
+
+The reference ships no tests or golden vectors for this path, so each oracle
+function is pinned by what the mathematics fixes: closed-form coordinates and
+eigenvalues, numpy's FFT, dense linear-algebra solves on tiny grids, symmetry,
+null space and conservation invariants, and exactness of the transfer
+operators on linear functions.  Each check is chosen so that a plausible
+mistake (wrong index or metric, sign, face average, colour order) fails it.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+TWO_PI = 2 * np.pi
+
+
+def grid(n, stretch_y=0, gamma=1.4, L=(TWO_PI, 2.0, np.pi)):
+    return O.desc(n, L=L, stretch=(0, stretch_y, 0), gamma=gamma)
+
+
+def tanh_c(i, n, L, g):
+    return 0.5 * L * (1 + np.tanh(-g + 2 * g * (i - 0.5) / n) / np.tanh(g))
+
+
+def tanh_f(i, n, L, g):
+    return 0.5 * L * (1 + np.tanh(-g + 2 * g * i / n) / np.tanh(g))
+
+
+def centres(g, a):
+    n = g.n[a]
+    xc, _ = O.axis_coords(n, g.L[a], g.stretch[a], g.gamma, 0)
+    return xc
+
+
+def faces(g, a):
+    n = g.n[a]
+    _, xf = O.axis_coords(n, g.L[a], g.stretch[a], g.gamma, 0)
+    return xf  # f(0..n)
+
+
+def vol(g):
+    dx = np.diff(faces(g, 0))
+    dy = np.diff(faces(g, 1))
+    dz = np.diff(faces(g, 2))
+    return dx[:, None, None] * dy[None, :, None] * dz[None, None, :]
+
+
+def dense(fn, shape):
+    n = int(np.prod(shape))
+    M = np.zeros((n, n))
+    for c in range(n):
+        e = np.zeros(n)
+        e[c] = 1.0
+        M[:, c] = fn(e.reshape(shape, order="F")).ravel(order="F")
+    return M
+
+
+rng = np.random.default_rng(1234)
+
+
+# ------------------------------------------------------------ coordinates (a30)
+
+@pytest.mark.parametrize("stretch,gamma", [(0, 1.4), (1, 1.4), (1, 1.5)])
+def test_axis_coordinates_closed_form(stretch, gamma):
+    n, L, G = 24, 2.0, 2
+    xc, xf = O.axis_coords(n, L, stretch, gamma, G)
+    i_c = np.arange(1 - G, n + G + 1)
+    i_f = np.arange(-G, n + G + 1)
+    if stretch:
+        ec, ef = tanh_c(i_c, n, L, gamma), tanh_f(i_f, n, L, gamma)
+    else:
+        d = L / n
+        ec, ef = (i_c - 0.5) * d, i_f * d
+    assert np.allclose(xc, ec, rtol=0, atol=1e-15)
+    assert np.allclose(xf, ef, rtol=0, atol=1e-15)
+    # walls at faces 0 and n, coarse centres coincide with fine faces 2I-1
+    assert abs(xf[G]) < 1e-15 and abs(xf[G + n] - L) < 1e-14
+    xc2, _ = O.axis_coords(n // 2, L, stretch, gamma, 0)
+    assert np.allclose(xc2, xf[G + 1:G + n:2], atol=1e-15)
+
+
+# ------------------------------------------------------------------ FFT (a2)
+
+@pytest.mark.parametrize("n", [2, 3, 4, 6, 8, 12, 32, 64, 96, 128, 192, 256])
+def test_fft_matches_numpy(n):
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.allclose(O.fft_c2c(x, -1), np.fft.fft(x), atol=1e-12 * n)
+    assert np.allclose(O.fft_c2c(x, +1), np.fft.ifft(x) * n, atol=1e-12 * n)
+
+
+# ------------------------------------------------------- operator (a16, a24)
+
+def test_operator_eigenmodes_uniform():
+    g = grid((16, 12, 8))
+    nx, ny, nz = g.shape if hasattr(g, "shape") else O.shape(g)
+    xc, yc, zc = centres(g, 0), centres(g, 1), centres(g, 2)
+    dx, dy, dz = TWO_PI / nx, 2.0 / ny, np.pi / nz
+    beta = np.ones((nx, ny, nz), order="F")
+    for m in (1, 3):
+        v = np.cos(TWO_PI * m * xc / TWO_PI)[:, None, None] * np.ones((1, ny, nz))
+        lam = 2 * (np.cos(TWO_PI * m / nx) - 1) / dx ** 2
+        assert np.allclose(O.apply_op(g, beta, v), lam * v, atol=1e-11)
+        v = np.ones((nx, 1, 1)) * np.cos(np.pi * m * yc / 2.0)[None, :, None] * np.ones((1, 1, nz))
+        lam = 2 * (np.cos(np.pi * m / ny) - 1) / dy ** 2
+        assert np.allclose(O.apply_op(g, beta, v), lam * v, atol=1e-11)
+        v = np.ones((nx, ny, 1)) * np.cos(TWO_PI * m * zc / np.pi)[None, None, :]
+        lam = 2 * (np.cos(TWO_PI * m / nz) - 1) / dz ** 2
+        assert np.allclose(O.apply_op(g, beta, v), lam * v, atol=1e-10)
+
+
+def test_operator_quadratic_on_tanh_walls():
+    g = grid((4, 20, 4), stretch_y=1, gamma=1.5)
+    ny = 20
+    yc = tanh_c(np.arange(1, ny + 1), ny, 2.0, 1.5)
+    yf = tanh_f(np.arange(0, ny + 1), ny, 2.0, 1.5)
+    p = np.ones((4, 1, 4)) * (yc ** 2)[None, :, None]
+    q = O.apply_op(g, np.ones((4, ny, 4), order="F"), p)[0, :, 0]
+    dyf = np.diff(yf)
+    expect = np.empty(ny)
+    expect[1:-1] = (yc[2:] - yc[:-2]) / dyf[1:-1]       # flux (y_{j+1}+y_j) differences
+    expect[0] = (yc[1] + yc[0]) / dyf[0]                # no flux through the wall
+    expect[-1] = -(yc[-1] + yc[-2]) / dyf[-1]
+    assert np.allclose(q, expect, rtol=1e-12, atol=1e-10)
+
+
+def test_operator_arithmetic_face_average():
+    g = grid((16, 4, 4))
+    nx = 16
+    dx = TWO_PI / nx
+    beta = np.ones((nx, 4, 4), order="F")
+    beta[8:] = 2.0
+    xc = centres(g, 0)
+    p = xc[:, None, None] * np.ones((1, 4, 4))
+    q = O.apply_op(g, beta, p)
+    assert np.allclose(q[7], (1.5 - 1.0) / dx, atol=1e-11)   # interface cell, left side
+    assert np.allclose(q[8], (2.0 - 1.5) / dx, atol=1e-11)
+    assert np.allclose(q[3], 0.0, atol=1e-11)
+
+
+@pytest.mark.parametrize("stretch", [0, 1])
+def test_operator_symmetry_nullspace_conservation(stretch):
+    g = grid((6, 8, 4), stretch_y=stretch)
+    shape = O.shape(g)
+    beta = np.exp(rng.standard_normal(shape))
+    V = vol(g)
+    A = dense(lambda e: O.apply_op(g, beta, e), shape)
+    W = np.diag(V.ravel(order="F"))
+    VA = W @ A
+    assert np.allclose(VA, VA.T, atol=1e-10 * np.abs(VA).max())
+    assert np.allclose(A @ np.ones(A.shape[0]), 0, atol=1e-10 * np.abs(A).max())
+    ev = np.linalg.eigvalsh(0.5 * (VA + VA.T))
+    assert ev.max() < 1e-9 * np.abs(ev).max() and np.sum(np.abs(ev) < 1e-9 * np.abs(ev).max()) == 1
+    u = rng.standard_normal(shape)
+    assert abs(np.sum(V * O.apply_op(g, beta, u))) < 1e-10 * np.abs(VA).max()
+
+
+# --------------------------------------------------------------- FFT path (a3-a7)
+
+def test_fft_operator_equals_unit_beta_operator():
+    for stretch in (0, 1):
+        g = grid((8, 10, 6), stretch_y=stretch)
+        p = rng.standard_normal(O.shape(g))
+        q1 = O.fft_operator(g, p)
+        q2 = O.apply_op(g, np.ones(O.shape(g)), p)
+        assert np.allclose(q1, q2, rtol=1e-12, atol=1e-10)
+
+
+def test_tridiag_vs_dense():
+    g = grid((4, 24, 4), stretch_y=1, gamma=1.5)
+    ny = 24
+    yc = tanh_c(np.arange(0, ny + 2), ny, 2.0, 1.5)   # c(0..ny+1)
+    yf = tanh_f(np.arange(0, ny + 1), ny, 2.0, 1.5)
+    dyc = np.diff(yc)          # dyc(j) for j=1..ny+1
+    dyf = np.diff(yf)          # dyf(j) for j=1..ny
+    am = 1 / (dyc[:ny] * dyf)
+    ap = 1 / (dyc[1:] * dyf)
+    am[0] = 0
+    ap[-1] = 0
+    d = rng.standard_normal(ny) + 1j * rng.standard_normal(ny)
+    for lam in (-7.0, -1e-4):
+        M = np.diag(lam - am - ap) + np.diag(am[1:], -1) + np.diag(ap[:-1], 1)
+        x = O.tridiag(g, lam, 0, d)
+        # backward error (the lam -> 0 system has condition ~1e7)
+        assert np.linalg.norm(M @ x - d) < 1e-13 * np.abs(M).max() * np.linalg.norm(x)
+        if lam < -1:
+            assert np.allclose(x, np.linalg.solve(M, d), rtol=1e-12, atol=1e-14)
+    M = np.diag(-am - ap) + np.diag(am[1:], -1) + np.diag(ap[:-1], 1)
+    x = O.tridiag(g, 0.0, 1, d)
+    assert x[-1] == 0
+    assert np.allclose((M @ x)[:-1], d[:-1], atol=1e-9)
+
+
+@pytest.mark.parametrize("stretch", [0, 1])
+def test_fft_solve_dense_tiny(stretch):
+    g = grid((8, 6, 4), stretch_y=stretch)
+    shape = O.shape(g)
+    V = vol(g)
+    s = rng.standard_normal(shape)
+    s -= np.sum(V * s) / V.sum()              # compatible right-hand side
+    p = O.fft_solve(g, s)
+    L = dense(lambda e: O.fft_operator(g, e), shape)
+    n = L.shape[0]
+    aug = np.zeros((n + 1, n + 1))
+    aug[:n, :n] = L
+    aug[:n, n] = V.ravel(order="F")
+    aug[n, :n] = V.ravel(order="F")
+    ref = np.linalg.solve(aug, np.r_[s.ravel(order="F"), 0.0])[:n].reshape(shape, order="F")
+    assert np.allclose(p, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+    assert abs(O.volume_mean(g, p)) < 1e-14 * np.abs(p).max()
+
+
+@pytest.mark.parametrize("n,stretch", [((64, 64, 64), 0), ((48, 32, 24), 1)])
+def test_fft_solve_residual(n, stretch):
+    g = grid(n, stretch_y=stretch, gamma=1.5)
+    V = vol(g)
+    s = rng.standard_normal(O.shape(g))
+    s -= np.sum(V * s) / V.sum()
+    p = O.fft_solve(g, s)
+    r = O.fft_operator(g, p) - s
+    assert np.linalg.norm(r) < 1e-12 * np.linalg.norm(s)
+    assert abs(O.volume_mean(g, p)) < 1e-12 * np.abs(p).max()
+
+
+# -------------------------------------------------------- transfer ops (a27-a28)
+
+@pytest.mark.parametrize("stretch", [0, 1])
+def test_restriction_exact_on_linear(stretch):
+    g = grid((8, 16, 8), stretch_y=stretch, gamma=1.5)
+    xc, yc, zc = centres(g, 0), centres(g, 1), centres(g, 2)
+    f = 1.5 + 0.3 * xc[:, None, None] - 0.7 * yc[None, :, None] + 0.2 * zc[None, None, :]
+    c = O.restrict(g, 0, f)
+    gc = grid((4, 8, 4), stretch_y=stretch, gamma=1.5)
+    Xc, Yc, Zc = centres(gc, 0), centres(gc, 1), centres(gc, 2)
+    expect = 1.5 + 0.3 * Xc[:, None, None] - 0.7 * Yc[None, :, None] + 0.2 * Zc[None, None, :]
+    assert np.allclose(c, expect, atol=1e-13)
+    if not stretch:   # uniform: plain average of the 8 children
+        avg = f.reshape(2, 4, 2, 8, 2, 4, order="F").mean(axis=(0, 2, 4))
+        assert np.allclose(c, avg, atol=1e-14)
+
+
+@pytest.mark.parametrize("stretch", [0, 1])
+def test_prolongation(stretch):
+    g = grid((8, 16, 8), stretch_y=stretch, gamma=1.5)
+    cs = O.shape(g, 1)
+    fine0 = rng.standard_normal(O.shape(g))
+    out = O.prolong_add(g, 0, 2.5 * np.ones(cs), fine0)
+    assert np.allclose(out - fine0, 2.5, atol=1e-13)          # constants, walls and wrap included
+    gc = grid((4, 8, 4), stretch_y=stretch, gamma=1.5)
+    Yc, yc = centres(gc, 1), centres(g, 1)
+    lin = np.ones((4, 1, 4)) * Yc[None, :, None]
+    out = O.prolong_add(g, 0, lin, np.zeros(O.shape(g)))
+    inner = slice(1, 15)                                       # away from the wall ghosts
+    assert np.allclose(out[:, inner, :], np.ones((8, 1, 8)) * yc[None, inner, None], atol=1e-13)
+    if not stretch:
+        delta = np.zeros(cs)
+        delta[1, 3, 1] = 1.0
+        out = O.prolong_add(g, 0, delta, np.zeros(O.shape(g)))
+        vals = sorted(set(np.round(out[out > 0] * 64, 10)))
+        assert vals == [1.0, 3.0, 9.0, 27.0]
+        assert np.isclose(out.sum(), 8.0)
+
+
+# --------------------------------------------------------- smoother (a22-a25)
+
+def test_rbgs_ordering_and_fixed_point():
+    g = grid((8, 8, 8), stretch_y=1)
+    shape = O.shape(g)
+    beta = np.exp(0.5 * rng.standard_normal(shape))
+    V = vol(g)
+    q = rng.standard_normal(shape)
+    q -= np.sum(V * q) / V.sum()           # compatible: sum V q = 0
+    p0 = rng.standard_normal(shape)
+    p1 = O.rbgs(g, 0, beta, q, p0, 1, remove_mean=0)
+    r = O.residual(g, 0, beta, q, p1)
+    i, j, k = np.meshgrid(np.arange(1, 9), np.arange(1, 9), np.arange(1, 9), indexing="ij")
+    black = ((i + j + k) % 2) == 1
+    interior = (j > 1) & (j < 8)
+    # black cells were updated last with final red neighbours: exact there
+    assert np.abs(r[black & interior]).max() < 1e-12 * np.abs(q).max()
+    # red cells see new black values: generally not exact
+    assert np.abs(r[~black & interior]).max() > 1e-6
+    # at a wall the cell's own old value is its ghost (Jacobi-like wall term)
+    A = dense(lambda e: O.apply_op(g, beta, e), shape)
+    sol = np.linalg.lstsq(A, q.ravel(order="F"), rcond=None)[0].reshape(shape, order="F")
+    p2 = O.rbgs(g, 0, beta, q, sol, 3, remove_mean=1)
+    assert np.allclose(p2, sol - sol.mean(), atol=1e-10 * np.abs(sol).max())
+
+
+# ------------------------------------------------------- coarse solve (a26)
+
+def test_coarse_exact_dense():
+    g = grid((4, 6, 4), stretch_y=1, gamma=1.5)
+    shape = O.shape(g)
+    beta = np.exp(rng.standard_normal(shape))
+    q = rng.standard_normal(shape)
+    V = vol(g).ravel(order="F")
+    A = dense(lambda e: O.apply_op(g, beta, e), shape)
+    M = -V[:, None] * A
+    n = M.shape[0]
+    aug = np.zeros((n + 1, n + 1))
+    aug[:n, :n] = M
+    aug[:n, n] = 1.0
+    aug[n, :n] = 1.0
+    ref = np.linalg.solve(aug, np.r_[-V * q.ravel(order="F"), 0.0])[:n].reshape(shape, order="F")
+    got = O.coarse_exact(g, 0, beta, q)
+    assert np.allclose(got, ref, atol=1e-11 * np.abs(ref).max())
+
+
+# ------------------------------------------------------------ PCG (a15, a21)
+
+@pytest.mark.parametrize("stretch", [0, 1])
+def test_pcg_dense_tiny(stretch):
+    g = grid((8, 8, 8), stretch_y=stretch)
+    shape = O.shape(g)
+    beta = np.exp(rng.standard_normal(shape))
+    V = vol(g)
+    q = rng.standard_normal(shape)
+    q -= np.sum(V * q) / V.sum()
+    A = dense(lambda e: O.apply_op(g, beta, e), shape)
+    ref = np.linalg.lstsq(A, q.ravel(order="F"), rcond=None)[0].reshape(shape, order="F")
+    ref -= ref.mean()
+    for coarse_mode, flexible in ((0, 0), (1, 0), (1, 1)):
+        prm = O.params(n_levels=3, tol=1e-13, max_iter=200, coarse_mode=coarse_mode, coarse_cycles=6,
+                       flexible=flexible)
+        psi, it, res, rc = O.pcg_solve(g, prm, beta, q)
+        assert rc == 0 and res <= 1e-13
+        r = q - O.apply_op(g, beta, psi)
+        assert abs(np.linalg.norm(r) / np.linalg.norm(q) - res) < 1e-14   # recursive vs true residual
+        assert np.allclose(psi - psi.mean(), ref, atol=1e-10 * np.abs(ref).max())
+
+
+def test_pcg_max_iter_and_warm_start():
+    g = grid((16, 16, 16))
+    shape = O.shape(g)
+    beta = np.ones(shape)
+    q = rng.standard_normal(shape)
+    q -= q.mean()
+    psi, it, res, rc = O.pcg_solve(g, O.params(n_levels=3, tol=1e-14, max_iter=2), beta, q)
+    assert rc == 1 and it == 2
+    psi2, it2, res2, rc2 = O.pcg_solve(g, O.params(n_levels=3, tol=1e-8, max_iter=50), beta, q, psi0=psi)
+    assert rc2 == 0 and res2 <= 1e-8
+    psi3, it3, _, _ = O.pcg_solve(g, O.params(n_levels=3, tol=1e-8, max_iter=50), beta, q)
+    assert it2 < it3
+
+
+# ---------------------------------------------- projection and divergence (a33)
+
+def test_divergence_manufactured():
+    g = grid((16, 8, 8), stretch_y=1, gamma=1.5)
+    nx, ny, nz = O.shape(g)
+    xf, yf, zf = faces(g, 0), faces(g, 1), faces(g, 2)
+    ux = np.sin(TWO_PI * xf[1:] / TWO_PI)[:, None, None] * np.ones((1, ny, nz))
+    uy = np.ones((nx, 1, nz)) * (yf ** 2)[None, :, None]
+    uz = np.ones((nx, ny, 1)) * np.cos(TWO_PI * zf[1:] / np.pi)[None, None, :]
+    d = O.divergence(g, ux, uy, uz)
+    ex = (np.sin(xf[1:]) - np.sin(xf[:-1])) / np.diff(xf)
+    ey = (yf[1:] ** 2 - yf[:-1] ** 2) / np.diff(yf)
+    ez = (np.cos(2 * zf[1:]) - np.cos(2 * zf[:-1])) / np.diff(zf)
+    expect = ex[:, None, None] + ey[None, :, None] + ez[None, None, :]
+    assert np.allclose(d, expect, atol=1e-12)
+
+
+def test_projection_identity_mg():
+    g = grid((16, 16, 16), stretch_y=1, gamma=1.5)
+    shape = O.shape(g)
+    nx, ny, nz = shape
+    ux, uz = rng.standard_normal(shape), rng.standard_normal(shape)
+    uy = rng.standard_normal((nx, ny + 1, nz))
+    uy[:, 0, :] = 0
+    uy[:, -1, :] = 0
+    rho = np.where(rng.random(shape) < 0.2, 1 / 20, 1.0)
+    adt = 1e-3
+    q = O.divergence(g, ux, uy, uz)
+    beta = O.beta_from_rho(g, adt, rho)
+    # a cell-by-cell random density with contrast 20 makes the reference's
+    # Fletcher-Reeves PCG with its red/black V-cycle stagnate; the identity
+    # below does not depend on the CG variant, so the flexible one is used
+    psi, it, res, rc = O.pcg_solve(g, O.params(n_levels=3, tol=1e-12, max_iter=300, flexible=1), beta, q)
+    assert rc == 0
+    ux2, uy2, uz2 = O.project_mg(g, adt, rho, psi, ux, uy, uz)
+    div = O.divergence(g, ux2, uy2, uz2)
+    r = q - O.apply_op(g, beta, psi)
+    assert np.allclose(div, r, atol=1e-12 * np.abs(q).max())     # div u_new = q - A psi
+    assert np.abs(div).max() < 1e-10 * np.abs(q).max()
+    assert np.all(uy2[:, 0, :] == 0) and np.all(uy2[:, -1, :] == 0)
+
+
+def test_fast_source_and_projection_constant_density():
+    g = grid((32, 32, 32))
+    shape = O.shape(g)
+    nx, ny, nz = shape
+    ux, uz = rng.standard_normal(shape), rng.standard_normal(shape)
+    uy = rng.standard_normal((nx, ny + 1, nz))
+    uy[:, 0, :] = 0
+    uy[:, -1, :] = 0
+    rho = np.ones(shape)
+    z = np.zeros(shape)
+    dt = 1e-3
+    s = O.source_fast(g, dt, 1.0, rho, ux, uy, uz, z, z)
+    assert np.allclose(s, O.divergence(g, ux, uy, uz) / dt, rtol=1e-13, atol=1e-9)
+    p = O.fft_solve(g, s)
+    ux2, uy2, uz2 = O.project_fast(g, dt, 1.0, rho, p, z, z, ux, uy, uz)
+    d = O.divergence(g, ux2, uy2, uz2)
+    assert np.abs(d).max() < 1e-10
+
+
+def test_fast_source_variable_density_is_conservative():
+    g = grid((16, 16, 8), stretch_y=1)
+    shape = O.shape(g)
+    nx, ny, nz = shape
+    z3 = np.zeros(shape)
+    zy = np.zeros((nx, ny + 1, nz))
+    rho = np.where(rng.random(shape) < 0.3, 1 / 20, 1.0)
+    psi1, psi2 = rng.standard_normal(shape), rng.standard_normal(shape)
+    s = O.source_fast(g, 1e-3, 1 / 20, rho, z3, zy, z3, psi1, psi2)
+    assert np.abs(s).max() > 0
+    assert abs(np.sum(vol(g) * s)) < 1e-11 * np.abs(vol(g) * s).sum()
+    s0 = O.source_fast(g, 1e-3, 1.0, np.ones(shape), z3, zy, z3, psi1, psi2)
+    assert np.abs(s0).max() == 0
