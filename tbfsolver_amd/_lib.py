@@ -52,7 +52,8 @@ class FpsOpts(C.Structure):
 class PcgParams(C.Structure):
     _fields_ = [("n_levels", C.c_int32), ("n_pre", C.c_int32), ("n_post", C.c_int32),
                 ("max_iter", C.c_int32), ("tol", C.c_double), ("mg_cycles", C.c_int32),
-                ("precision", C.c_int32), ("coarse_cycles", C.c_int32), ("flexible", C.c_int32)]
+                ("precision", C.c_int32), ("coarse_max_iter", C.c_int32), ("flexible", C.c_int32),
+                ("coarse_tol", C.c_double)]
 
 
 _P = C.c_void_p

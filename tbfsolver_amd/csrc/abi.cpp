@@ -192,8 +192,9 @@ void tbfg_pcg_default_params(tbfg_pcg_params* p)
     p->tol = 1e-8;
     p->mg_cycles = 1;
     p->precision = TBFG_PREC_FP64;
-    p->coarse_cycles = 4;
+    p->coarse_max_iter = 40;
     p->flexible = -1;
+    p->coarse_tol = 0.0;
 }
 
 int tbfg_pcg_create(const tbfg_grid_desc* grid, const tbfg_pcg_params* prm, tbfg_pcg** out)

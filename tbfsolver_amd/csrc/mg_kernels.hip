@@ -387,7 +387,7 @@ __global__ void k_convert(size_t n, const TS* __restrict__ src, TD* __restrict__
 // microseconds of work, less than a kernel launch.  One workgroup runs all of
 // them back to back with workgroup barriers; their arrays stay in L2.
 template <class T>
-__device__ void tail_smooth(const TailLevel<T>& lv, int sweeps, bool first)
+__device__ void tail_smooth(const TailLevel<T>& lv, int sweeps, bool first, bool reverse = false)
 {
     const LevelDev<T>& L = lv.geo;
     const int half = L.nx >> 1;
@@ -401,24 +401,25 @@ __device__ void tail_smooth(const TailLevel<T>& lv, int sweeps, bool first)
     }
     for (int sw = 0; sw < sweeps; ++sw) {
         const bool f0 = first && sw == 0;
-        for (int t = threadIdx.x; t < work; t += blockDim.x) {
-            const int h = t % half, row = t / half;
-            const int j = row % L.ny, k = row / L.ny;
-            if (f0)
-                rbgs_cell<T, T, RED, true>(L, lv.beta, lv.q, lv.p, h, j, k);
-            else
-                rbgs_cell<T, T, RED, false>(L, lv.beta, lv.q, lv.p, h, j, k);
+        for (int half_sweep = 0; half_sweep < 2; ++half_sweep) {
+            const bool red = (half_sweep == 0) != reverse;
+            for (int t = threadIdx.x; t < work; t += blockDim.x) {
+                const int h = t % half, row = t / half;
+                const int j = row % L.ny, k = row / L.ny;
+                if (red) {
+                    if (f0)
+                        rbgs_cell<T, T, RED, true>(L, lv.beta, lv.q, lv.p, h, j, k);
+                    else
+                        rbgs_cell<T, T, RED, false>(L, lv.beta, lv.q, lv.p, h, j, k);
+                } else {
+                    if (f0)
+                        rbgs_cell<T, T, BLACK, true>(L, lv.beta, lv.q, lv.p, h, j, k);
+                    else
+                        rbgs_cell<T, T, BLACK, false>(L, lv.beta, lv.q, lv.p, h, j, k);
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (int t = threadIdx.x; t < work; t += blockDim.x) {
-            const int h = t % half, row = t / half;
-            const int j = row % L.ny, k = row / L.ny;
-            if (f0)
-                rbgs_cell<T, T, BLACK, true>(L, lv.beta, lv.q, lv.p, h, j, k);
-            else
-                rbgs_cell<T, T, BLACK, false>(L, lv.beta, lv.q, lv.p, h, j, k);
-        }
-        __syncthreads();
     }
 }
 
@@ -462,41 +463,134 @@ __device__ void tail_dense(const TailPlan<T>& P, const TailLevel<T>& lv)
     __syncthreads();
 }
 
-// V-cycle over tail levels top..bot (multiGrid.f90:186-232); at `bot` either the
-// dense solve or, at the reference's coarsest level, coarse_cycles inner cycles
-// over the sub-hierarchy below it.
-template <class T>
-__device__ void tail_vcycle(const TailPlan<T>& P, int top, int bot, bool reset_top);
-
-template <class T>
-__device__ void tail_bottom(const TailPlan<T>& P, int bot)
+// workgroup-wide deterministic sum of NV values (one workgroup, <= 1024 threads)
+template <int NV>
+__device__ void wg_sum(double (&v)[NV])
 {
-    if (bot == P.user_last && P.bottom > P.user_last) {
-        for (int c = 0; c < P.coarse_cycles; ++c) {
-            // first inner cycle starts from p = 0 at the coarsest user level
-            const TailPlan<T>& Q = P;
-            const int top = P.user_last;
-            const bool reset = (c == 0);
-            for (int l = top; l < Q.bottom; ++l) {
-                tail_smooth<T>(Q.lv[l], Q.n_pre, (l > top) || reset);
-                tail_resid_restrict<T>(Q.lv[l], Q.lv[l + 1]);
-            }
-            tail_dense<T>(Q, Q.lv[Q.bottom]);
-            for (int l = Q.bottom - 1; l >= top; --l) {
-                tail_prolong<T>(Q.lv[l], Q.lv[l + 1]);
-                tail_smooth<T>(Q.lv[l], Q.n_post, false);
-            }
+    __shared__ double part[16 * 4];
+    __shared__ double res[4];
+    for (int q = 0; q < NV; ++q)
+        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < NV; ++q) part[wave * 4 + q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int q = 0; q < NV; ++q) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += part[w * 4 + q];
+            res[q] = s;
         }
-        if (P.coarse_cycles <= 0) {
-            const int cells = P.lv[bot].geo.nx * P.lv[bot].geo.ny * P.lv[bot].geo.nz;
-            for (int t = threadIdx.x; t < cells; t += blockDim.x) P.lv[bot].p[t] = (T)0;
-            __syncthreads();
-        }
-    } else {
-        tail_dense<T>(P, P.lv[bot]);
+    }
+    __syncthreads();
+    for (int q = 0; q < NV; ++q) v[q] = res[q];
+    __syncthreads();
+}
+
+// z = M_c r on the reference's coarsest level U: one V-cycle from zero over the
+// sub-hierarchy U..bottom (red/black pre-smoothing, black/red post-smoothing,
+// dense exact solve at the bottom).  Input in lv[U].q, output in lv[U].p.
+template <class T>
+__device__ void tail_inner_vcycle(const TailPlan<T>& P)
+{
+    const int top = P.user_last;
+    for (int l = top; l < P.bottom; ++l) {
+        tail_smooth<T>(P.lv[l], P.n_pre, true);
+        tail_resid_restrict<T>(P.lv[l], P.lv[l + 1]);
+    }
+    tail_dense<T>(P, P.lv[P.bottom]);
+    for (int l = P.bottom - 1; l >= top; --l) {
+        tail_prolong<T>(P.lv[l], P.lv[l + 1]);
+        tail_smooth<T>(P.lv[l], P.n_post, false, true);
     }
 }
 
+// Coarsest-level solve replacing the reference's MUMPS factor+solve
+// (poissMat.f90:289-312): the Lagrange-constrained system (poissMat.f90:246-259)
+// is solved by flexible conjugate gradients in the V-weighted inner product
+// (V A is symmetric), preconditioned by the inner V-cycle, to the relative
+// residual coarse_tol.  The right-hand side is first projected onto the range
+// of A as the Lagrange multiplier does: q' = q - (sum V q)/(N V).
+template <class T>
+__device__ void tail_coarse_pcg(const TailPlan<T>& P)
+{
+    const TailLevel<T>& U = P.lv[P.user_last];
+    const LevelDev<T>& L = U.geo;
+    const int n = L.nx * L.ny * L.nz;
+    T *x = P.cx, *r = P.cr, *d = P.cd, *ad = P.cad;
+    const T* vol = P.cvol;
+    double s1[1] = {0.0};
+    for (int t = threadIdx.x; t < n; t += blockDim.x) s1[0] += (double)vol[t] * (double)U.q[t];
+    wg_sum<1>(s1);
+    const double shift = s1[0] / (double)n;
+    double s2[1] = {0.0};
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const T rv = U.q[t] - (T)(shift / (double)vol[t]);
+        r[t] = rv;
+        x[t] = (T)0;
+        U.q[t] = rv;
+        s2[0] += (double)vol[t] * (double)rv * (double)rv;
+    }
+    wg_sum<1>(s2);
+    const double rr0 = s2[0];
+    if (!(rr0 > 0.0)) {
+        for (int t = threadIdx.x; t < n; t += blockDim.x) U.p[t] = (T)0;
+        __syncthreads();
+        return;
+    }
+    tail_inner_vcycle<T>(P);
+    double s3[1] = {0.0};
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        d[t] = U.p[t];
+        s3[0] += (double)vol[t] * (double)r[t] * (double)U.p[t];
+    }
+    wg_sum<1>(s3);
+    double delta = s3[0];
+    const double tol2 = (double)P.ctol * (double)P.ctol;
+    for (int it = 0; it < P.cmaxit; ++it) {
+        double dq[1] = {0.0};
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            const int i = t % L.nx, rr = t / L.nx, j = rr % L.ny, k = rr / L.ny;
+            size_t c, e, w, nn, s, f, b;
+            neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, nn, s, f, b);
+            Coef<T> a = coefs<T>(L, U.beta, i, j, k, c, e, w, nn, s, f, b);
+            const T v = apply_cell<T>(a, d, c, e, w, nn, s, f, b);
+            ad[t] = v;
+            dq[0] += (double)vol[t] * (double)d[t] * (double)v;
+        }
+        wg_sum<1>(dq);
+        if (!(dq[0] != 0.0)) break;
+        const double alpha = delta / dq[0];
+        double rn[1] = {0.0};
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            x[t] = x[t] + (T)alpha * d[t];
+            const T rv = r[t] - (T)alpha * ad[t];
+            r[t] = rv;
+            U.q[t] = rv;
+            rn[0] += (double)vol[t] * (double)rv * (double)rv;
+        }
+        wg_sum<1>(rn);
+        if (rn[0] <= tol2 * rr0) break;
+        tail_inner_vcycle<T>(P);
+        double dz[2] = {0.0, 0.0};
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            const double z = (double)U.p[t], vv = (double)vol[t];
+            dz[0] += vv * (double)r[t] * z;
+            dz[1] += vv * (double)ad[t] * z;
+        }
+        wg_sum<2>(dz);
+        const double gmm = -alpha * dz[1] / delta;  // Polak-Ribiere
+        delta = dz[0];
+        for (int t = threadIdx.x; t < n; t += blockDim.x) d[t] = U.p[t] + (T)gmm * d[t];
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < n; t += blockDim.x) U.p[t] = x[t];
+    __syncthreads();
+}
+
+// V-cycle over tail levels top..bot (multiGrid.f90:186-232) with the coarsest
+// level solved by tail_coarse_pcg (or the dense solve when it is the bottom).
 template <class T>
 __device__ void tail_vcycle(const TailPlan<T>& P, int top, int bot, bool reset_top)
 {
@@ -504,7 +598,10 @@ __device__ void tail_vcycle(const TailPlan<T>& P, int top, int bot, bool reset_t
         tail_smooth<T>(P.lv[l], P.n_pre, (l > top) || reset_top);
         tail_resid_restrict<T>(P.lv[l], P.lv[l + 1]);
     }
-    tail_bottom<T>(P, bot);
+    if (P.bottom > P.user_last)
+        tail_coarse_pcg<T>(P);
+    else
+        tail_dense<T>(P, P.lv[bot]);
     for (int l = bot - 1; l >= top; --l) {
         tail_prolong<T>(P.lv[l], P.lv[l + 1]);
         tail_smooth<T>(P.lv[l], P.n_post, false);

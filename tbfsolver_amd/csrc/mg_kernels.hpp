@@ -53,7 +53,11 @@ struct TailPlan {
     int nlev;          // number of tail levels (index 0 = finest tail level)
     int user_last;     // tail index of the reference's coarsest level
     int bottom;        // tail index of the dense bottom level
-    int n_pre, n_post, coarse_cycles;
+    int n_pre, n_post;
+    int cmaxit;        // inner PCG iterations on the coarsest user level
+    T ctol;            // its relative residual target
+    T *cx, *cr, *cd, *cad;  // inner PCG vectors (coarsest user level)
+    const T* cvol;     // cell volumes of that level
     const double* G;   // bottom dense operator, row-major: x = G q
     TailLevel<T> lv[kMaxTail];
 };

@@ -63,7 +63,13 @@ static int build_plan(tbfg_pcg* h)
     P.bottom = (h->ntot - 1) - h->t0;
     P.n_pre = h->prm.n_pre;
     P.n_post = h->prm.n_post;
-    P.coarse_cycles = h->prm.coarse_cycles;
+    P.cmaxit = h->prm.coarse_max_iter;
+    P.ctol = (T)h->prm.coarse_tol;
+    P.cx = h->cx.as<T>();
+    P.cr = h->cr.as<T>();
+    P.cd = h->cd.as<T>();
+    P.cad = h->cad.as<T>();
+    P.cvol = h->cvol.as<T>();
     P.G = h->G.as<double>();
     for (int m = 0; m < P.nlev; ++m) {
         const int l = h->t0 + m;
@@ -89,9 +95,9 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         return TBFG_EINVAL;
     }
     if (prm.n_pre < 0 || prm.n_post < 0 || prm.n_pre + prm.n_post < 1 || prm.mg_cycles < 1 || prm.max_iter < 0 ||
-        !(prm.tol >= 0.0) || prm.coarse_cycles < 0) {
-        set_error("invalid PCG/MG parameters (pre %d post %d cycles %d max_iter %d tol %g coarse %d)", prm.n_pre,
-                  prm.n_post, prm.mg_cycles, prm.max_iter, prm.tol, prm.coarse_cycles);
+        !(prm.tol >= 0.0) || prm.coarse_max_iter < 0 || !(prm.coarse_tol >= 0.0)) {
+        set_error("invalid PCG/MG parameters (pre %d post %d cycles %d max_iter %d tol %g coarse %d/%g)", prm.n_pre,
+                  prm.n_post, prm.mg_cycles, prm.max_iter, prm.tol, prm.coarse_max_iter, prm.coarse_tol);
         return TBFG_EINVAL;
     }
     if (prm.precision != TBFG_PREC_FP64 && prm.precision != TBFG_PREC_MIXED) {
@@ -111,10 +117,11 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     h->prm = prm;
     h->mixed = prm.precision == TBFG_PREC_MIXED;
     if (h->prm.flexible < 0) h->prm.flexible = h->mixed ? 1 : 0;
+    if (!(h->prm.coarse_tol > 0.0)) h->prm.coarse_tol = h->mixed ? 1e-5 : 1e-10;
     h->nuser = prm.n_levels;
     // sub-hierarchy under the coarsest level: halve while every axis is even
     int ntot = h->nuser;
-    if (prm.coarse_cycles > 0) {
+    if (prm.coarse_max_iter > 0) {
         for (;;) {
             bool ok = true;
             size_t cells = 1;
@@ -133,7 +140,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         for (int a = 0; a < 3; ++a) cb *= (size_t)(g->n[a] >> (ntot - 1));
         if (cb > (size_t)kMaxBottomCells) {
             set_error("the coarsest grid has %zu cells; the dense coarsest solve needs <= %d (use more levels "
-                      "or coarse_cycles > 0 on an axis-even grid)",
+                      "or coarse_max_iter > 0 on an axis-even grid)",
                       cb, kMaxBottomCells);
             return TBFG_EINVAL;
         }
@@ -200,6 +207,27 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
                         B.ax[0].df(i) * B.ax[1].df(j) * B.ax[2].df(k);
         TBFG_TRY(upload(h->vol, vol.data(), nbc * 8));
     }
+    // inner CG vectors and cell volumes of the coarsest user level
+    {
+        const HostLevel& U = h->hl[h->nuser - 1];
+        const size_t nu = U.cells();
+        TBFG_TRY(h->cx.alloc(nu * es));
+        TBFG_TRY(h->cr.alloc(nu * es));
+        TBFG_TRY(h->cd.alloc(nu * es));
+        TBFG_TRY(h->cad.alloc(nu * es));
+        std::vector<double> v(nu);
+        for (int k = 1; k <= U.n[2]; ++k)
+            for (int j = 1; j <= U.n[1]; ++j)
+                for (int i = 1; i <= U.n[0]; ++i)
+                    v[(i - 1) + (size_t)U.n[0] * ((j - 1) + (size_t)U.n[1] * (k - 1))] =
+                        U.ax[0].df(i) * U.ax[1].df(j) * U.ax[2].df(k);
+        if (h->mixed) {
+            std::vector<float> vf(v.begin(), v.end());
+            TBFG_TRY(upload(h->cvol, vf.data(), nu * 4));
+        } else {
+            TBFG_TRY(upload(h->cvol, v.data(), nu * 8));
+        }
+    }
     if (h->mixed)
         TBFG_TRY(build_plan<float>(h.get()));
     else
@@ -214,7 +242,8 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     for (auto& t : h->tabd) fp += (int64_t)t->buf.bytes;
     for (auto& t : h->tabf) fp += (int64_t)t->buf.bytes;
     fp += (int64_t)(h->r.bytes + h->d.bytes + h->qv.bytes + h->st.bytes + h->partials.bytes + h->counter.bytes +
-                    h->plan.bytes + h->G.bytes + h->dwork.bytes + h->vol.bytes);
+                    h->plan.bytes + h->G.bytes + h->dwork.bytes + h->vol.bytes + h->cx.bytes + h->cr.bytes +
+                    h->cd.bytes + h->cad.bytes + h->cvol.bytes);
     h->footprint = fp;
     *out = h.release();
     return TBFG_OK;

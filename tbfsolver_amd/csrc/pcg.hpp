@@ -25,6 +25,7 @@ struct tbfg_pcg {
     tbfg::DevBuf st, partials, counter; // PcgState + reduction scratch
     tbfg::DevBuf plan;                  // TailPlan<T>
     tbfg::DevBuf G, dwork, vol;         // dense bottom operator and its set-up scratch
+    tbfg::DevBuf cx, cr, cd, cad, cvol; // inner CG of the coarsest-level solve
     tbfg::DevBuf hpsi, hbeta, hq;       // device copies for the host-array entry point
     tbfg::PcgState* hst = nullptr;      // pinned mirror of the device state
     double* hstage = nullptr;           // pinned staging (3 * n0 doubles)

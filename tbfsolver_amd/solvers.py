@@ -124,14 +124,15 @@ class PCG:
     `iterations` the number of direction updates."""
 
     def __init__(self, grid, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=100,
-                 precision="fp64", mg_cycles=1, coarse_cycles=4, flexible=None):
+                 precision="fp64", mg_cycles=1, coarse_max_iter=40, coarse_tol=0.0, flexible=None):
         # pcgCTOR + multiGridCTOR
         self.grid = grid
         p = default_params()
         p.n_levels, p.n_pre, p.n_post = int(nLevels), int(nPreSweep), int(nPostSweep)
         p.tol, p.max_iter, p.mg_cycles = float(tolPCG), int(maxIterPCG), int(mg_cycles)
         p.precision = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
-        p.coarse_cycles = int(coarse_cycles)
+        p.coarse_max_iter = int(coarse_max_iter)
+        p.coarse_tol = float(coarse_tol)
         p.flexible = -1 if flexible is None else int(flexible)
         self.params = p
         h = C.c_void_p()

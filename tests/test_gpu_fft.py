@@ -69,7 +69,7 @@ def test_fft_solve_mixed(torch_gpu, n, stretch):
     ref_solver = T.FastPoissonSolver(g)
     pr = torch_gpu.zeros_like(to_dev(s, torch_gpu))
     ref_solver.solveFPS(pr, to_dev(s, torch_gpu))
-    for sweeps, bound in ((0, 1e-4), (1, 1e-8), (2, 1e-10)):
+    for sweeps, bound in ((0, 1e-3), (1, 1e-7), (2, 1e-10)):
         fs = T.FastPoissonSolver(g, precision="mixed", refine_sweeps=sweeps)
         p = torch_gpu.zeros_like(pr)
         fs.solveFPS(p, to_dev(s, torch_gpu))

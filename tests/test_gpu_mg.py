@@ -86,13 +86,15 @@ def test_preconditioner_matches_oracle(torch_gpu, n, stretch, nlev):
     print("case precondition", n, stretch, nlev, flush=True)
     g, go, rho, _, q, beta = bubble_case(n, 6, stretch, 11)
     r = q.copy()
-    s = T.PCG(g, nLevels=nlev, coarse_cycles=6)
+    s = T.PCG(g, nLevels=nlev)
     z = torch_gpu.zeros_like(to_dev(r, torch_gpu))
     s.precondition(to_dev(beta, torch_gpu), to_dev(r, torch_gpu), z)
-    zr = O.mg_apply(go, O.params(n_levels=nlev, coarse_mode=1, coarse_cycles=6), beta, r)
+    # the reference's V-cycle with the exact coarsest solve; the GPU solves the
+    # coarsest level iteratively to a relative residual of 1e-10
+    zr = O.mg_apply(go, O.params(n_levels=nlev, coarse_mode=0), beta, r)
     out = to_host(z)
     assert abs(out.mean()) < 1e-12 * np.abs(out).max()
-    assert rel(mean_free(out), mean_free(zr)) < 1e-11
+    assert rel(mean_free(out), mean_free(zr)) < 1e-8
 
 
 @pytest.mark.parametrize("n,stretch,nlev,nb", [((64, 64, 64), 0, 4, 18), ((64, 48, 64), 1, 4, 20)])
@@ -100,21 +102,19 @@ def test_pcg_fp64_matches_oracle(torch_gpu, n, stretch, nlev, nb):
     print("case pcg fp64", n, stretch, flush=True)
     g, go, rho, _, q, beta = bubble_case(n, nb, stretch, 21)
     for flexible in (0, 1):
-        s = T.PCG(g, nLevels=nlev, tolPCG=1e-12, maxIterPCG=200, coarse_cycles=6, flexible=flexible)
+        s = T.PCG(g, nLevels=nlev, tolPCG=1e-12, maxIterPCG=200, flexible=flexible)
         psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
         s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
-        ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=nlev, tol=1e-12, max_iter=200, coarse_mode=1,
-                                                    coarse_cycles=6, flexible=flexible), beta, q)
+        # the reference algorithm: PCG + V-cycle with the exact coarsest solve
+        ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=nlev, tol=1e-12, max_iter=200, coarse_mode=0,
+                                                    flexible=flexible), beta, q)
+        print("  flexible", flexible, "gpu iterations", s.iterations, "oracle", it, flush=True)
         assert rc == 0 and s.converged
         assert abs(s.iterations - it) <= 1, (s.iterations, it)
         out = to_host(psi)
         assert rel(mean_free(out), mean_free(ref)) < 1e-10
         true_res = np.linalg.norm(q - O.apply_op(go, beta, out)) / np.linalg.norm(q)
         assert true_res < 2e-12
-        # reference semantics of the exact coarsest solve give the same pressure
-        ref0, _, _, _ = O.pcg_solve(go, O.params(n_levels=nlev, tol=1e-12, max_iter=200, coarse_mode=0,
-                                                 flexible=flexible), beta, q)
-        assert rel(mean_free(out), mean_free(ref0)) < 1e-10
 
 
 @pytest.mark.parametrize("n,stretch,nb", [((64, 64, 64), 0, 18), ((64, 48, 64), 1, 20)])
@@ -123,7 +123,7 @@ def test_pcg_mixed_vs_fp64_oracle(torch_gpu, n, stretch, nb):
     g, go, rho, _, q, beta = bubble_case(n, nb, stretch, 31)
     ref, _, _, _ = O.pcg_solve(go, O.params(n_levels=4, tol=1e-13, max_iter=300, flexible=1), beta, q)
     for tol in (1e-8, 1e-11):
-        s = T.PCG(g, nLevels=4, tolPCG=tol, maxIterPCG=300, precision="mixed", coarse_cycles=6)
+        s = T.PCG(g, nLevels=4, tolPCG=tol, maxIterPCG=300, precision="mixed")
         psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
         s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
         assert s.converged and s.res_ <= tol
@@ -139,7 +139,7 @@ def test_host_array_entry_point_with_halo(torch_gpu):
     print("case pcg host arrays", flush=True)
     n = (32, 32, 32)
     g, go, rho, _, q, beta = bubble_case(n, 6, 0, 41)
-    s = T.PCG(g, nLevels=3, tolPCG=1e-12, maxIterPCG=100, coarse_cycles=6)
+    s = T.PCG(g, nLevels=3, tolPCG=1e-12, maxIterPCG=100)
     pad = lambda a: np.asfortranarray(np.pad(a, 1))
     psi_h = np.zeros((34, 34, 34), order="F")
     s.solvePCG(psi_h, pad(beta), pad(q))
@@ -159,11 +159,11 @@ def test_max_iter_status_and_warm_start(torch_gpu):
     print("case pcg status", flush=True)
     n = (32, 32, 32)
     g, go, rho, _, q, beta = bubble_case(n, 6, 0, 51)
-    s = T.PCG(g, nLevels=3, tolPCG=1e-14, maxIterPCG=2, coarse_cycles=4)
+    s = T.PCG(g, nLevels=3, tolPCG=1e-14, maxIterPCG=2)
     psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
     s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
     assert not s.converged and s.iterations == 2 and s.iter_ == 2
-    s2 = T.PCG(g, nLevels=3, tolPCG=1e-9, maxIterPCG=50, coarse_cycles=4)
+    s2 = T.PCG(g, nLevels=3, tolPCG=1e-9, maxIterPCG=50)
     s2.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
     warm = s2.iterations
     psi0 = torch_gpu.zeros_like(psi)
@@ -187,7 +187,7 @@ def test_projection_step_divergence_free(torch_gpu):
     F.computeSource(g, dux, duy, duz, dq)
     dbeta = torch_gpu.zeros_like(drho)
     F.computeBeta(g, adt, drho, dbeta)
-    s = T.PCG(g, nLevels=4, tolPCG=1e-12, maxIterPCG=200, coarse_cycles=6)
+    s = T.PCG(g, nLevels=4, tolPCG=1e-12, maxIterPCG=200)
     psi = torch_gpu.zeros_like(drho)
     s.solvePCG(psi, dbeta, dq)
     F.makeVelocityDivFree(g, adt, drho, psi, dux, duy, duz)
