@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Poisson solves/s at 256x192x256 to 1e-8 (mixed vs FP64 GPU), p50/p99 ms.
+
+Workload (one "step" = one pressure solve from psi = 0 on device-resident data):
+  256x192x256 channel, Lx=2pi, Ly=2 (walls), Lz=pi, tanh-stretched y (gamma 1.5);
+  72 non-overlapping bubbles of diameter 0.3 (seed 1002, nref=50 sub-sampling,
+  two smoothing passes), rho_g/rho_l = 1/20, beta = alpha dt / rho with
+  alpha dt = 1e-3; q = div(u*) of a seeded N(0,1) staggered velocity.
+  Variable-density PCG + 4-level multigrid (2+2 red/black sweeps) to
+  ||r||/||q|| <= 1e-8.  The timed path is the mixed-precision form (FP32
+  multigrid inside an FP64 flexible PCG); the FP64 form is timed beside it.
+
+Output: one JSON line on stdout; diagnostics on stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Poisson solves/s at 256×192×256 to 1e-8 (mixed vs FP64 GPU); p50/p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def to_dev(a, torch):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)).cuda()
+
+
+def timed_solves(torch, solver, psi, beta, q, steps, warmup, profile=None):
+    for _ in range(warmup):
+        psi.zero_()
+        solver.solvePCG(psi, beta, q)
+    torch.cuda.synchronize()
+    if profile:
+        solver.profile_begin(profile)
+    times, iters = [], []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        psi.zero_()
+        solver.solvePCG(psi, beta, q)   # returns after the stop test; the stream is drained
+        times.append(time.perf_counter() - t0)
+        iters.append(solver.iterations)
+        if not solver.converged:
+            raise RuntimeError("solve did not converge (res %g after %d iterations)" % (solver.res_, solver.iterations))
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t_start
+    prof = solver.profile_end() if profile else None
+    return total, np.array(times) * 1e3, iters, prof
+
+
+def cpu_baseline(g_dims, rho, ux, uy, uz, q_host, beta_host):
+    """The oracle (C, OpenMP, FP64, the reference algorithm with an exact
+    coarsest solve) on the host cores: one full solve of the same workload."""
+    from oracle import pyoracle as O
+
+    go = O.desc(g_dims, stretch=(0, 1, 0), gamma=1.5)
+    prm = O.params(n_levels=4, n_pre=2, n_post=2, max_iter=200, tol=1e-8, flexible=0, coarse_mode=0)
+    t0 = time.perf_counter()
+    psi, it, res, rc = O.pcg_solve(go, prm, beta_host, q_host)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    return dt, it, res, cores, psi
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
+    ap.add_argument("--nref", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the pressure solve of one channel instance runs on a single "
+              "MI355X (instances shard one per GPU); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import tbfsolver_amd as T
+    from tbfsolver_amd import fields as F
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU")
+    dims = (256, 192, 256)
+    g = T.channel_grid(dims, stretch_y=True, gamma=1.5)
+    t0 = time.perf_counter()
+    rho, _ = F.make_bubble_density(g, 72, 0.3, 1002, nref=args.nref, rho_l=1.0, rho_g=1.0 / 20.0)
+    ux, uy, uz = F.make_velocity(g, 1002)
+    log("inputs generated in %.1f s" % (time.perf_counter() - t0))
+    adt = 1e-3
+    drho = to_dev(rho, torch)
+    dux, duy, duz = (to_dev(a, torch) for a in (ux, uy, uz))
+    q = torch.zeros_like(drho)
+    F.computeSource(g, dux, duy, duz, q)
+    beta = torch.zeros_like(drho)
+    F.computeBeta(g, adt, drho, beta)
+    torch.cuda.synchronize()
+
+    mixed = T.PCG(g, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=200, precision="mixed")
+    psi = torch.zeros_like(drho)
+    total, ms, iters, prof = timed_solves(torch, mixed, psi, beta, q, args.steps, args.warmup, profile="rbgs_fine")
+    psi_mixed = psi.clone()
+    value = args.steps / total
+    log("mixed: %.2f solves/s, median %.3f ms, iterations %s" % (value, np.median(ms), sorted(set(iters))))
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 4),
+        "p50_ms": round(float(np.percentile(ms, 50)), 4),
+        "p99_ms": round(float(np.percentile(ms, 99)), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 multigrid in fp64 flexible PCG",
+        "data": "synthetic (seeded bubbles and velocity, generated on the host)",
+        "config": {"workload": "256x192x256 tanh-y (gamma 1.5) channel, 72 bubbles rho_g/rho_l=1/20, "
+                               "PCG + 4-level RBGS multigrid to 1e-8, solve from psi=0",
+                   "grid": list(dims), "levels": 4, "sweeps": [2, 2], "tol": 1e-8, "precision": "mixed"},
+        "iterations": int(np.median(iters)),
+    }
+
+    # FP64 GPU path (the reference algorithm: Fletcher-Reeves PCG, FP64 multigrid)
+    if not args.no_fp64:
+        fp64 = T.PCG(g, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=200, precision="fp64")
+        psi64 = torch.zeros_like(drho)
+        k64 = max(3, args.steps // 2)
+        total64, ms64, it64, _ = timed_solves(torch, fp64, psi64, beta, q, k64, 1)
+        v64 = k64 / total64
+        tight = T.PCG(g, nLevels=4, tolPCG=1e-12, maxIterPCG=400, precision="fp64", flexible=1)
+        psi_ref = torch.zeros_like(drho)
+        tight.solvePCG(psi_ref, beta, q)
+        a = psi_mixed - psi_mixed.mean()
+        b = psi_ref - psi_ref.mean()
+        err_l2 = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+        err_max = float((a - b).abs().max() / b.abs().max())
+        c = psi64 - psi64.mean()
+        err64 = float(torch.linalg.norm(c - b) / torch.linalg.norm(b))
+        out["fp64"] = {"solves_per_s": round(v64, 3), "p50_ms": round(float(np.percentile(ms64, 50)), 4),
+                       "p99_ms": round(float(np.percentile(ms64, 99)), 4), "iterations": int(np.median(it64)),
+                       "pcg": "Fletcher-Reeves"}
+        out["speedup_mixed_vs_fp64"] = round(value / v64, 3)
+        out["error_vs_fp64_1e-12"] = {"mixed_rel_l2": err_l2, "mixed_rel_max": err_max, "fp64_rel_l2": err64}
+        log("fp64: %.2f solves/s, iterations %s; mixed err %.2e (L2) %.2e (max)" % (v64, sorted(set(it64)),
+                                                                                 err_l2, err_max))
+
+    # roofline of the dominant kernel (finest-level red/black half-sweep)
+    launches, kms, bytes_per = prof
+    if launches > 0:
+        avg_s = kms / launches * 1e-3
+        achieved = bytes_per / avg_s / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "rbgs_fine_traffic.json")
+        if os.path.exists(tf):
+            try:
+                traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+            except Exception:
+                traffic = None
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "kernel": "k_rbgs<float,double> finest-level half-sweep",
+                           "avg_us": round(avg_s * 1e6, 2), "launches": launches,
+                           "algorithmic_bytes_per_launch": bytes_per}
+        log("rbgs fine: %d launches, avg %.1f us, %.0f GB/s algorithmic" % (launches, avg_s * 1e6, achieved))
+
+    if not args.no_cpu:
+        q_host = np.asfortranarray(q.cpu().numpy().T)
+        b_host = np.asfortranarray(beta.cpu().numpy().T)
+        dt, it, res, cores, psi_cpu = cpu_baseline(dims, rho, ux, uy, uz, q_host, b_host)
+        out["cpu_baseline"] = {"value": round(1.0 / dt, 4), "unit": "solves/s", "cores": cores, "kind": "port",
+                               "sample": "one full FP64 solve of the same workload (the oracle: C/OpenMP, "
+                                         "Fletcher-Reeves PCG, exact coarsest solve), %d iterations, %.1f s"
+                                         % (it, dt)}
+        log("cpu oracle: %.2f s, %d iterations, %d threads" % (dt, it, cores))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -142,6 +142,25 @@ int tbfg_pcg_precondition_dev(tbfg_pcg* h, const double* beta_dev, const double*
 /* Device bytes held by the handle (for admission by HBM footprint). */
 int64_t tbfg_pcg_footprint(const tbfg_pcg* h);
 
+/* Kernel timing with HIP events recorded on the solve's stream around every
+ * launch of one kernel family (no counterpart in the reference, whose only
+ * timing is the wall clock of poissonEqn.f90:463-489):
+ *   0 red/black half-sweep on the finest level    1 fused residual+restriction, finest
+ *   2 operator apply + d.Ad (PCG)                 3 psi/r update + ||r|| (PCG)
+ * tbfg_pcg_profile_end returns the launches seen since _begin, their summed
+ * duration and the algorithmic bytes one launch must move (each input value
+ * read once, each output written once). */
+#define TBFG_KERNEL_RBGS_FINE 0
+#define TBFG_KERNEL_RESTRICT_FINE 1
+#define TBFG_KERNEL_SPMV 2
+#define TBFG_KERNEL_UPDATE 3
+int tbfg_pcg_profile_begin(tbfg_pcg* h, int32_t kernel);
+
+/* Diagnostics of the handle's last solve: coarsest-level solves made, their
+ * inner iterations (sum and maximum) and the worst final relative residual. */
+int tbfg_pcg_coarse_stats(const tbfg_pcg* h, int32_t* calls, int32_t* iters, int32_t* iters_max, double* res_max);
+int tbfg_pcg_profile_end(tbfg_pcg* h, int64_t* launches, double* total_ms, double* bytes_per_launch);
+
 int tbfg_pcg_destroy(tbfg_pcg* h);
 
 /* ------------------------------------------------------- stencils and fields */

@@ -79,6 +79,10 @@ SIGNATURES = {
     "tbfg_pcg_precondition_dev": (C.c_int, [_P, _P, _P, _P, _P]),
     "tbfg_pcg_footprint": (C.c_int64, [_P]),
     "tbfg_pcg_destroy": (C.c_int, [_P]),
+    "tbfg_pcg_profile_begin": (C.c_int, [_P, C.c_int32]),
+    "tbfg_pcg_coarse_stats": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_double)]),
+    "tbfg_pcg_profile_end": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tbfg_apply_operator_dev": (C.c_int, [C.POINTER(GridDesc), _P, _P, _P, _P]),
     "tbfg_divergence_dev": (C.c_int, [C.POINTER(GridDesc), _P, _P, _P, _P, _P]),
     "tbfg_beta_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, _P, _P, _P]),

@@ -256,6 +256,26 @@ int tbfg_pcg_solve(tbfg_pcg* h, double* psi, const int64_t psi_lb[3], const int6
 
 int64_t tbfg_pcg_footprint(const tbfg_pcg* h) { return h ? h->footprint : 0; }
 
+int tbfg_pcg_profile_begin(tbfg_pcg* h, int32_t kernel) { ABI_GUARD(return pcg_profile_begin(h, kernel);) }
+
+int tbfg_pcg_coarse_stats(const tbfg_pcg* h, int32_t* calls, int32_t* iters, int32_t* iters_max, double* res_max)
+{
+    if (!h || !h->hst) {
+        set_error("tbfg_pcg_coarse_stats: no solve yet");
+        return TBFG_EINVAL;
+    }
+    if (calls) *calls = h->hst->coarse_calls;
+    if (iters) *iters = h->hst->coarse_iters;
+    if (iters_max) *iters_max = h->hst->coarse_iters_max;
+    if (res_max) *res_max = h->hst->coarse_res_max;
+    return TBFG_OK;
+}
+
+int tbfg_pcg_profile_end(tbfg_pcg* h, int64_t* launches, double* total_ms, double* bytes_per_launch)
+{
+    ABI_GUARD(return pcg_profile_end(h, launches, total_ms, bytes_per_launch);)
+}
+
 int tbfg_pcg_destroy(tbfg_pcg* h)
 {
     ABI_GUARD({

@@ -534,8 +534,14 @@ __global__ void k_fft_residual(OpTables o, const double* __restrict__ s, const d
         r[c] = s[c] - lp;
         v[0] = pc * o.vol_y[j];
     }
+    block_partial<1>(v, red);
+}
+
+__global__ void k_fin_mean(unsigned nblocks, double inv_vg, double* __restrict__ out_mean, Reducer red)
+{
     double tot[1];
-    if (grid_sum<1>(v, red, tot) && threadIdx.x == 0) out_mean[0] = tot[0] * inv_vg;
+    final_sum<1>(red, nblocks, tot);
+    if (threadIdx.x == 0) out_mean[0] = tot[0] * inv_vg;
 }
 
 __global__ void k_fft_operator(OpTables o, const double* __restrict__ p, double* __restrict__ q)
@@ -768,11 +774,13 @@ int fps_solve_dev(tbfg_fps* h, const double* s, double* p, void* stream)
     hipStream_t st = (hipStream_t)stream;
     if (h->precision == TBFG_PREC_FP64) return spectral_solve<double, double>(h, s, p, 0, st);
     TBFG_TRY((spectral_solve<float, double>(h, s, p, 0, st)));
-    Reducer red{h->partials.as<double>(), h->counter.as<unsigned>()};
+    Reducer red{h->partials.as<double>()};
     for (int it = 0; it < h->refine; ++it) {
         const dim3 blk(row_threads(h->nx));
         hipLaunchKernelGGL(k_fft_residual, dim3(row_blocks(h->nx), h->ny, h->nz), blk, 0, st, h->ot, s, p,
                            h->r.as<double>(), h->mean.as<double>(), h->inv_vg, red);
+        hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(256), 0, st, row_blocks(h->nx) * h->ny * h->nz, h->inv_vg,
+                           h->mean.as<double>(), red);
         TBFG_TRY((spectral_solve<float, double>(h, h->r.as<double>(), p, 1, st)));
     }
     return TBFG_OK;

@@ -93,8 +93,15 @@ __global__ void k_op_dot(LevelDev<double> L, const double* __restrict__ beta, co
         qv[c] = out;
         v[0] = d[c] * out;
     }
+    block_partial<1>(v, red);
+}
+
+__global__ void k_fin_alpha(unsigned nblocks, PcgState* st, Reducer red)
+{
+    if (st->done) return;
     double tot[1];
-    if (grid_sum<1>(v, red, tot) && threadIdx.x == 0 && threadIdx.y == 0) {
+    final_sum<1>(red, nblocks, tot);
+    if (threadIdx.x == 0) {
         const double den = tot[0];
         if (!(den != 0.0) || !isfinite(den) || !isfinite(st->delta)) {
             st->done = 1;
@@ -124,8 +131,14 @@ __global__ void k_residual_init(LevelDev<double> L, const double* __restrict__ b
         v[0] = rc * rc;
         v[1] = qc * qc;
     }
+    block_partial<2>(v, red);
+}
+
+__global__ void k_fin_init(unsigned nblocks, PcgState* st, Reducer red)
+{
     double tot[2];
-    if (grid_sum<2>(v, red, tot) && threadIdx.x == 0 && threadIdx.y == 0) {
+    final_sum<2>(red, nblocks, tot);
+    if (threadIdx.x == 0) {
         st->rr = tot[0];
         st->qq = tot[1];
         st->iter = 0;
@@ -159,8 +172,15 @@ __global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __r
         r[t] = rn;
         v[0] = rn * rn;
     }
+    block_partial<1>(v, red);
+}
+
+__global__ void k_fin_update(unsigned nblocks, PcgState* st, Reducer red)
+{
+    if (st->done) return;
     double tot[1];
-    if (grid_sum<1>(v, red, tot) && threadIdx.x == 0) {
+    final_sum<1>(red, nblocks, tot);
+    if (threadIdx.x == 0) {
         st->rr = tot[0];
         st->iter += 1;
         const double res = sqrt(tot[0]) / sqrt(st->qq + 2.2250738585072014e-308);
@@ -200,8 +220,15 @@ __global__ void k_pcg_dots(size_t ncell, const double* __restrict__ r, const TZ*
             v[4] = qc;
         }
     }
+    block_partial<5>(v, red);
+}
+
+__global__ void k_fin_dots(unsigned nblocks, size_t ncell, int init, PcgState* st, Reducer red)
+{
+    if (st->done) return;
     double tot[5];
-    if (grid_sum<5>(v, red, tot) && threadIdx.x == 0) {
+    final_sum<5>(red, nblocks, tot);
+    if (threadIdx.x == 0) {
         const double m = tot[2] / (double)ncell;
         const double rz = tot[0] - m * tot[3];
         st->zmean = m;
@@ -548,7 +575,9 @@ __device__ void tail_coarse_pcg(const TailPlan<T>& P)
     wg_sum<1>(s3);
     double delta = s3[0];
     const double tol2 = (double)P.ctol * (double)P.ctol;
-    for (int it = 0; it < P.cmaxit; ++it) {
+    int it = 0;
+    double rlast = rr0;
+    for (it = 0; it < P.cmaxit; ++it) {
         double dq[1] = {0.0};
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const int i = t % L.nx, rr = t / L.nx, j = rr % L.ny, k = rr / L.ny;
@@ -571,7 +600,11 @@ __device__ void tail_coarse_pcg(const TailPlan<T>& P)
             rn[0] += (double)vol[t] * (double)rv * (double)rv;
         }
         wg_sum<1>(rn);
-        if (rn[0] <= tol2 * rr0) break;
+        rlast = rn[0];
+        if (rn[0] <= tol2 * rr0) {
+            ++it;
+            break;
+        }
         tail_inner_vcycle<T>(P);
         double dz[2] = {0.0, 0.0};
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
@@ -586,6 +619,13 @@ __device__ void tail_coarse_pcg(const TailPlan<T>& P)
         __syncthreads();
     }
     for (int t = threadIdx.x; t < n; t += blockDim.x) U.p[t] = x[t];
+    if (threadIdx.x == 0 && P.st) {
+        P.st->coarse_calls += 1;
+        P.st->coarse_iters += it;
+        if (it > P.st->coarse_iters_max) P.st->coarse_iters_max = it;
+        const double rel = sqrt(rlast / rr0);
+        if (rel > P.st->coarse_res_max) P.st->coarse_res_max = rel;
+    }
     __syncthreads();
 }
 
@@ -725,6 +765,7 @@ int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d
     dim3 blk = block2d(L.nx);
     dim3 grd(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), L.nz);
     hipLaunchKernelGGL(k_op_dot, grd, blk, 0, S(stream), L, beta, d, qv, st, red);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, S(stream), grd.x * grd.y * grd.z, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
@@ -735,6 +776,7 @@ int launch_residual_init(const LevelDev<double>& L, const double* beta, const do
     dim3 blk = block2d(L.nx);
     dim3 grd(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), L.nz);
     hipLaunchKernelGGL(k_residual_init, grd, blk, 0, S(stream), L, beta, q, psi, r, st, red);
+    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, S(stream), grd.x * grd.y * grd.z, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
@@ -743,6 +785,7 @@ int launch_pcg_update(size_t n, double* psi, double* r, const double* d, const d
                       const Reducer& red, void* stream)
 {
     hipLaunchKernelGGL(k_pcg_update, dim3(ceil_div(n, 256)), dim3(256), 0, S(stream), n, psi, r, d, qv, st, red);
+    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, S(stream), ceil_div(n, 256), st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
@@ -752,6 +795,7 @@ int launch_pcg_dots(size_t n, const double* r, const TZ* z, const double* qv, in
                     const Reducer& red, void* stream)
 {
     hipLaunchKernelGGL(k_pcg_dots<TZ>, dim3(ceil_div(n, 256)), dim3(256), 0, S(stream), n, r, z, qv, init, st, red);
+    hipLaunchKernelGGL(k_fin_dots, dim3(1), dim3(256), 0, S(stream), ceil_div(n, 256), n, init, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }

@@ -31,13 +31,18 @@ struct PcgState {
     int32_t status;    // TBFG_OK / TBFG_ENOTCONVERGED / TBFG_ENUMERIC
     int32_t flexible;  // 0 Fletcher-Reeves, 1 Polak-Ribiere
     int32_t pad;
+    // coarsest-level solves of this PCG solve (diagnostics)
+    int32_t coarse_calls;
+    int32_t coarse_iters;     // inner iterations summed over calls
+    int32_t coarse_iters_max;
+    int32_t pad2;
+    double coarse_res_max;    // worst final relative residual
 };
 
-// Deterministic grid reduction: every block writes NV partial sums, the last
-// block to finish adds the partials in block order and calls the finisher.
+// Partial sums of a grid reduction: NV doubles per block of the producing
+// kernel, added in block order by a one-block finalize kernel.
 struct Reducer {
     double* partials;    // >= blocks * NV
-    unsigned* counter;   // zero between uses (the last block resets it)
 };
 
 template <class T>
@@ -59,6 +64,7 @@ struct TailPlan {
     T *cx, *cr, *cd, *cad;  // inner PCG vectors (coarsest user level)
     const T* cvol;     // cell volumes of that level
     const double* G;   // bottom dense operator, row-major: x = G q
+    PcgState* st;      // receives the coarse-solve diagnostics (may be null)
     TailLevel<T> lv[kMaxTail];
 };
 

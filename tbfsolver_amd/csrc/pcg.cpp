@@ -40,11 +40,89 @@ constexpr int kMaxBottomCells = 512;
 
 tbfg_pcg::~tbfg_pcg()
 {
+    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     if (hst) (void)hipHostFree(hst);
     if (hstage) (void)hipHostFree(hstage);
 }
 
 namespace tbfg {
+
+// --------------------------------------------------------- kernel timing
+
+static constexpr int kProfPairs = 1024;
+
+// records an event pair around a launch of the profiled kernel family
+struct ProfScope {
+    tbfg_pcg* h;
+    hipStream_t s;
+    bool on;
+    ProfScope(tbfg_pcg* h_, int kernel, void* stream) : h(h_), s((hipStream_t)stream)
+    {
+        on = h->prof_kernel == kernel && h->prof_used < kProfPairs;
+        if (on) (void)hipEventRecord(h->prof_ev[2 * h->prof_used], s);
+    }
+    ~ProfScope()
+    {
+        if (on) {
+            (void)hipEventRecord(h->prof_ev[2 * h->prof_used + 1], s);
+            h->prof_used++;
+        }
+    }
+};
+
+// adds up the recorded pairs; the stream must have been synchronised
+static void prof_harvest(tbfg_pcg* h)
+{
+    for (int i = 0; i < h->prof_used; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->prof_ev[2 * i], h->prof_ev[2 * i + 1]) == hipSuccess) {
+            h->prof_ms += ms;
+            h->prof_launches++;
+        }
+    }
+    h->prof_used = 0;
+}
+
+int pcg_profile_begin(tbfg_pcg* h, int kernel)
+{
+    if (!h || kernel < 0 || kernel > 3) {
+        set_error("tbfg_pcg_profile_begin: bad handle or kernel id %d", kernel);
+        return TBFG_EINVAL;
+    }
+    if (h->prof_ev.empty()) {
+        h->prof_ev.resize(2 * kProfPairs);
+        for (auto& e : h->prof_ev) TBFG_HIP(hipEventCreate(&e));
+    }
+    h->prof_kernel = kernel;
+    h->prof_used = 0;
+    h->prof_launches = 0;
+    h->prof_ms = 0.0;
+    return TBFG_OK;
+}
+
+int pcg_profile_end(tbfg_pcg* h, int64_t* launches, double* total_ms, double* bytes_per_launch)
+{
+    if (!h || h->prof_kernel < 0) {
+        set_error("tbfg_pcg_profile_end: profiling not started");
+        return TBFG_EINVAL;
+    }
+    TBFG_HIP(hipDeviceSynchronize());
+    prof_harvest(h);
+    const double n = (double)h->n0;
+    const double bp = h->mixed ? 4.0 : 8.0;  // MG storage (p, beta on the finest level)
+    double bytes = 0.0;
+    switch (h->prof_kernel) {
+        case 0: bytes = bp * n + bp * n + 8.0 * n / 2; break;         // p (other colour in, own out), beta, q own colour
+        case 1: bytes = bp * n + bp * n + 8.0 * n + bp * n / 8; break; // p, beta, r in; coarse q out
+        case 2: bytes = 24.0 * n; break;                               // d, beta in; qv out
+        case 3: bytes = 48.0 * n; break;                               // psi, r, d, qv in; psi, r out
+    }
+    if (launches) *launches = h->prof_launches;
+    if (total_ms) *total_ms = h->prof_ms;
+    if (bytes_per_launch) *bytes_per_launch = bytes;
+    h->prof_kernel = -1;
+    return TBFG_OK;
+}
 
 template <class T>
 static const LevelDev<T>& geo(tbfg_pcg* h, int l);
@@ -71,6 +149,7 @@ static int build_plan(tbfg_pcg* h)
     P.cad = h->cad.as<T>();
     P.cvol = h->cvol.as<T>();
     P.G = h->G.as<double>();
+    P.st = h->st.as<PcgState>();
     for (int m = 0; m < P.nlev; ++m) {
         const int l = h->t0 + m;
         P.lv[m].geo = geo<T>(h, l);
@@ -284,17 +363,24 @@ static int vcycle(tbfg_pcg* h, const double* r, T* z, const T* beta0, const int*
             for (int s = 0; s < h->prm.n_pre; ++s) {
                 const bool f0 = first && s == 0;
                 if (l == 0) {
-                    TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, RED, f0, done, stream));
-                    TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, BLACK, f0, done, stream));
+                    {
+                        ProfScope ps(h, 0, stream);
+                        TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, RED, f0, done, stream));
+                    }
+                    {
+                        ProfScope ps(h, 0, stream);
+                        TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, BLACK, f0, done, stream));
+                    }
                 } else {
                     const T* q = h->q[l].as<T>();
                     TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, RED, f0, done, stream));
                     TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, BLACK, f0, done, stream));
                 }
             }
-            if (l == 0)
+            if (l == 0) {
+                ProfScope ps(h, 1, stream);
                 TBFG_TRY(launch_resid_restrict<T, double>(geo<T>(h, 0), b, r, p, h->q[1].as<T>(), done, stream));
-            else
+            } else
                 TBFG_TRY(launch_resid_restrict<T, T>(geo<T>(h, l), b, h->q[l].as<T>(), p, h->q[l + 1].as<T>(), done,
                                                      stream));
         }
@@ -305,8 +391,14 @@ static int vcycle(tbfg_pcg* h, const double* r, T* z, const T* beta0, const int*
             TBFG_TRY(launch_prolong_add<T>(geo<T>(h, l), h->p[l + 1].as<T>(), p, done, stream));
             for (int s = 0; s < h->prm.n_post; ++s) {
                 if (l == 0) {
-                    TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, RED, false, done, stream));
-                    TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, BLACK, false, done, stream));
+                    {
+                        ProfScope ps(h, 0, stream);
+                        TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, RED, false, done, stream));
+                    }
+                    {
+                        ProfScope ps(h, 0, stream);
+                        TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, BLACK, false, done, stream));
+                    }
                 } else {
                     const T* q = h->q[l].as<T>();
                     TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, RED, false, done, stream));
@@ -325,7 +417,7 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     hipStream_t s = (hipStream_t)stream;
     PcgState* st = h->st.as<PcgState>();
     const int* done = &st->done;
-    Reducer red{h->partials.as<double>(), h->counter.as<unsigned>()};
+    Reducer red{h->partials.as<double>()};
     std::memset(h->hst, 0, sizeof(PcgState));
     h->hst->tol = h->prm.tol;
     h->hst->max_iter = h->prm.max_iter;
@@ -350,13 +442,20 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     TBFG_HIP(hipStreamSynchronize(s));
     while (!h->hst->done) {
-        TBFG_TRY(launch_op_dot(L0, beta, d, qv, st, red, stream));
-        TBFG_TRY(launch_pcg_update(n, psi, r, d, qv, st, red, stream));
+        {
+            ProfScope ps(h, 2, stream);
+            TBFG_TRY(launch_op_dot(L0, beta, d, qv, st, red, stream));
+        }
+        {
+            ProfScope ps(h, 3, stream);
+            TBFG_TRY(launch_pcg_update(n, psi, r, d, qv, st, red, stream));
+        }
         TBFG_TRY(vcycle<T>(h, r, z, beta0, done, stream));
         TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 0, st, red, stream));
         TBFG_TRY(launch_pcg_dir<T>(n, z, d, 0, st, stream));
         TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         TBFG_HIP(hipStreamSynchronize(s));
+        if (h->prof_kernel >= 0) prof_harvest(h);
     }
     if (iters) *iters = h->hst->iter;
     if (res) *res = h->hst->res;
@@ -387,7 +486,7 @@ static int precond_t(tbfg_pcg* h, const double* beta, const double* r, double* z
 {
     hipStream_t s = (hipStream_t)stream;
     PcgState* st = h->st.as<PcgState>();
-    Reducer red{h->partials.as<double>(), h->counter.as<unsigned>()};
+    Reducer red{h->partials.as<double>()};
     std::memset(h->hst, 0, sizeof(PcgState));
     TBFG_HIP(hipMemcpyAsync(st, h->hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
     TBFG_TRY(setup_levels<T>(h, beta, stream));

@@ -31,6 +31,13 @@ struct tbfg_pcg {
     double* hstage = nullptr;           // pinned staging (3 * n0 doubles)
     int64_t footprint = 0;
 
+    // kernel timing (tbfg_pcg_profile_begin/end)
+    int prof_kernel = -1;
+    std::vector<hipEvent_t> prof_ev;  // pairs
+    int prof_used = 0;
+    int64_t prof_launches = 0;
+    double prof_ms = 0.0;
+
     ~tbfg_pcg();
 };
 
@@ -39,4 +46,6 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm, tbfg_pcg** o
 int pcg_solve_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t* iters, double* res,
                   void* stream);
 int pcg_precondition_dev(tbfg_pcg* h, const double* beta, const double* r, double* z, void* stream);
+int pcg_profile_begin(tbfg_pcg* h, int kernel);
+int pcg_profile_end(tbfg_pcg* h, int64_t* launches, double* total_ms, double* bytes_per_launch);
 }  // namespace tbfg

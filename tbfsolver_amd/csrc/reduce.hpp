@@ -26,39 +26,34 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= 32*
     }
 }
 
-// Writes the block's partials; returns true in every thread of the last block,
-// whose thread 0 then holds the grand totals in `tot`.
+// Deterministic grid reduction in two launches: every block of the producing
+// kernel writes its NV partial sums (block_partial); a one-block finalize
+// kernel adds them in block order (final_sum).  No device-scope fence is
+// needed, which matters on a multi-XCD part where such a fence writes back L2.
 template <int NV>
-__device__ bool grid_sum(double (&v)[NV], const Reducer& red, double (&tot)[NV])
+__device__ __forceinline__ void block_partial(double (&v)[NV], const Reducer& red)
 {
     __shared__ double lds[32 * 8];
-    __shared__ int am_last;
     const int lane = threadIdx.x + blockDim.x * threadIdx.y;
-    const int nthreads = blockDim.x * blockDim.y;
-    const unsigned nblocks = gridDim.x * gridDim.y * gridDim.z;
     const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     block_sum<NV>(v, lds);
-    if (lane == 0) {
+    if (lane == 0)
         for (int q = 0; q < NV; ++q) red.partials[(size_t)bid * NV + q] = v[q];
-        __threadfence();
-        unsigned prev = atomicAdd(red.counter, 1u);
-        am_last = (prev == nblocks - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
+}
+
+// one block; thread 0 receives the totals
+template <int NV>
+__device__ __forceinline__ void final_sum(const Reducer& red, unsigned nblocks, double (&tot)[NV])
+{
+    __shared__ double lds[32 * 8];
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (unsigned b = lane; b < nblocks; b += nthreads)
+    for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x)
         for (int q = 0; q < NV; ++q) acc[q] += red.partials[(size_t)b * NV + q];
-    __syncthreads();
     block_sum<NV>(acc, lds);
-    if (lane == 0) {
+    if (threadIdx.x == 0)
         for (int q = 0; q < NV; ++q) tot[q] = acc[q];
-        *red.counter = 0u;
-    }
-    return true;
 }
 
 }  // namespace tbfg

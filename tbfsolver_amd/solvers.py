@@ -187,6 +187,23 @@ class PCG:
     def footprint(self):
         return int(lib().tbfg_pcg_footprint(self._h))
 
+    KERNELS = {"rbgs_fine": 0, "restrict_fine": 1, "spmv": 2, "update": 3}
+
+    def coarse_stats(self):
+        """Coarsest-level solves of the last solve: dict(calls, iters, iters_max, res_max)."""
+        a, b, c, d = C.c_int32(0), C.c_int32(0), C.c_int32(0), C.c_double(0.0)
+        check(lib().tbfg_pcg_coarse_stats(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return {"calls": a.value, "iters": b.value, "iters_max": c.value, "res_max": d.value}
+
+    def profile_begin(self, kernel):
+        check(lib().tbfg_pcg_profile_begin(self._h, self.KERNELS.get(kernel, kernel)))
+
+    def profile_end(self):
+        """(launches, total_ms, algorithmic bytes per launch) since profile_begin."""
+        n, ms, b = C.c_int64(0), C.c_double(0.0), C.c_double(0.0)
+        check(lib().tbfg_pcg_profile_end(self._h, C.byref(n), C.byref(ms), C.byref(b)))
+        return int(n.value), float(ms.value), float(b.value)
+
     def close(self):
         if getattr(self, "_h", None):
             lib().tbfg_pcg_destroy(self._h)
