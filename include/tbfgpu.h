@@ -106,7 +106,9 @@ typedef struct tbfg_pcg_params {
     double tol;            /* tolPCG on ||r||_2 / ||q||_2                             */
     int32_t mg_cycles;     /* V-cycles per preconditioner application (default 1)    */
     int32_t precision;     /* TBFG_PREC_FP64 (reference algorithm) or TBFG_PREC_MIXED */
-    int32_t coarse_max_iter; /* inner CG iterations of the coarsest-level solve      */
+    int32_t coarse_max_iter; /* coarsest reference level: > 0 inner CG iterations to
+                                coarse_tol, 0 the V-cycle continues below it (one
+                                cycle), -1 by precision (FP64 40, mixed 0)           */
     int32_t flexible;        /* 0 Fletcher-Reeves, 1 Polak-Ribiere, -1 by precision  */
     double coarse_tol;       /* relative residual of the coarsest-level solve
                                 (0: 1e-10 in FP64, 1e-5 in mixed precision)          */

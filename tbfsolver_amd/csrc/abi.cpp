@@ -192,7 +192,7 @@ void tbfg_pcg_default_params(tbfg_pcg_params* p)
     p->tol = 1e-8;
     p->mg_cycles = 1;
     p->precision = TBFG_PREC_FP64;
-    p->coarse_max_iter = 40;
+    p->coarse_max_iter = -1;
     p->flexible = -1;
     p->coarse_tol = 0.0;
 }

@@ -83,15 +83,18 @@ __global__ void k_op_dot(LevelDev<double> L, const double* __restrict__ beta, co
     if (st->done) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     double v[1] = {0.0};
     if (i < L.nx && j < L.ny) {
-        size_t c, e, w, n, s, f, b;
-        neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
-        Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
-        const double out = apply_cell<double>(a, d, c, e, w, n, s, f, b);
-        qv[c] = out;
-        v[0] = d[c] * out;
+        const int kc = (L.nz + gridDim.z - 1) / gridDim.z;  // contiguous planes: neighbours stay cached
+        const int k1 = min(L.nz, (int)(blockIdx.z + 1) * kc);
+        for (int k = blockIdx.z * kc; k < k1; ++k) {
+            size_t c, e, w, n, s, f, b;
+            neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
+            Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
+            const double out = apply_cell<double>(a, d, c, e, w, n, s, f, b);
+            qv[c] = out;
+            v[0] += d[c] * out;
+        }
     }
     block_partial<1>(v, red);
 }
@@ -119,17 +122,20 @@ __global__ void k_residual_init(LevelDev<double> L, const double* __restrict__ b
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     double v[2] = {0.0, 0.0};
     if (i < L.nx && j < L.ny) {
-        size_t c, e, w, n, s, f, b;
-        neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
-        Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
-        const double qc = q[c];
-        const double rc = qc - apply_cell<double>(a, psi, c, e, w, n, s, f, b);
-        r[c] = rc;
-        v[0] = rc * rc;
-        v[1] = qc * qc;
+        const int kc = (L.nz + gridDim.z - 1) / gridDim.z;
+        const int k1 = min(L.nz, (int)(blockIdx.z + 1) * kc);
+        for (int k = blockIdx.z * kc; k < k1; ++k) {
+            size_t c, e, w, n, s, f, b;
+            neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
+            Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
+            const double qc = q[c];
+            const double rc = qc - apply_cell<double>(a, psi, c, e, w, n, s, f, b);
+            r[c] = rc;
+            v[0] += rc * rc;
+            v[1] += qc * qc;
+        }
     }
     block_partial<2>(v, red);
 }
@@ -165,12 +171,12 @@ __global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __r
     if (st->done) return;
     const double alpha = st->alpha;
     double v[1] = {0.0};
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ncell) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += stride) {
         psi[t] = psi[t] + alpha * d[t];
         const double rn = r[t] - alpha * qv[t];
         r[t] = rn;
-        v[0] = rn * rn;
+        v[0] += rn * rn;
     }
     block_partial<1>(v, red);
 }
@@ -207,17 +213,17 @@ __global__ void k_pcg_dots(size_t ncell, const double* __restrict__ r, const TZ*
 {
     if (st->done) return;
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ncell) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += stride) {
         const double zc = (double)z[t];
         const double rc = r[t];
-        v[0] = rc * zc;
-        v[2] = zc;
-        v[3] = rc;
+        v[0] += rc * zc;
+        v[2] += zc;
+        v[3] += rc;
         if (!init) {
             const double qc = qv[t];
-            v[1] = qc * zc;
-            v[4] = qc;
+            v[1] += qc * zc;
+            v[4] += qc;
         }
     }
     block_partial<5>(v, red);
@@ -753,17 +759,36 @@ int launch_apply_op(const LevelDev<double>& L, const double* beta, const double*
     return TBFG_OK;
 }
 
+// reduction kernels: a few thousand blocks, each reducing once (per-block
+// reduction latency times tens of thousands of blocks would dominate)
+constexpr unsigned kRedBlocks = 2048;
+
+static dim3 red_grid3(const LevelDev<double>& L, dim3 blk)
+{
+    const unsigned gx = ceil_div(L.nx, blk.x), gy = ceil_div(L.ny, blk.y);
+    unsigned gz = kRedBlocks / (gx * gy);
+    if (gz < 1) gz = 1;
+    if (gz > (unsigned)L.nz) gz = (unsigned)L.nz;
+    return dim3(gx, gy, gz);
+}
+
+static unsigned red_blocks1(size_t n)
+{
+    const unsigned b = ceil_div(n, 256);
+    return b < kRedBlocks ? b : kRedBlocks;
+}
+
 unsigned cell_blocks(const LevelDev<double>& L)
 {
-    dim3 blk = block2d(L.nx);
-    return ceil_div(L.nx, blk.x) * ceil_div(L.ny, blk.y) * (unsigned)L.nz;
+    dim3 g = red_grid3(L, block2d(L.nx));
+    return g.x * g.y * g.z;
 }
 
 int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d, double* qv, PcgState* st,
                   const Reducer& red, void* stream)
 {
     dim3 blk = block2d(L.nx);
-    dim3 grd(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), L.nz);
+    dim3 grd = red_grid3(L, blk);
     hipLaunchKernelGGL(k_op_dot, grd, blk, 0, S(stream), L, beta, d, qv, st, red);
     hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, S(stream), grd.x * grd.y * grd.z, st, red);
     TBFG_HIP(hipGetLastError());
@@ -774,7 +799,7 @@ int launch_residual_init(const LevelDev<double>& L, const double* beta, const do
                          double* r, PcgState* st, const Reducer& red, void* stream)
 {
     dim3 blk = block2d(L.nx);
-    dim3 grd(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), L.nz);
+    dim3 grd = red_grid3(L, blk);
     hipLaunchKernelGGL(k_residual_init, grd, blk, 0, S(stream), L, beta, q, psi, r, st, red);
     hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, S(stream), grd.x * grd.y * grd.z, st, red);
     TBFG_HIP(hipGetLastError());
@@ -784,8 +809,9 @@ int launch_residual_init(const LevelDev<double>& L, const double* beta, const do
 int launch_pcg_update(size_t n, double* psi, double* r, const double* d, const double* qv, PcgState* st,
                       const Reducer& red, void* stream)
 {
-    hipLaunchKernelGGL(k_pcg_update, dim3(ceil_div(n, 256)), dim3(256), 0, S(stream), n, psi, r, d, qv, st, red);
-    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, S(stream), ceil_div(n, 256), st, red);
+    const unsigned nb = red_blocks1(n);
+    hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, S(stream), n, psi, r, d, qv, st, red);
+    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, S(stream), nb, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
@@ -794,8 +820,9 @@ template <class TZ>
 int launch_pcg_dots(size_t n, const double* r, const TZ* z, const double* qv, int init, PcgState* st,
                     const Reducer& red, void* stream)
 {
-    hipLaunchKernelGGL(k_pcg_dots<TZ>, dim3(ceil_div(n, 256)), dim3(256), 0, S(stream), n, r, z, qv, init, st, red);
-    hipLaunchKernelGGL(k_fin_dots, dim3(1), dim3(256), 0, S(stream), ceil_div(n, 256), n, init, st, red);
+    const unsigned nb = red_blocks1(n);
+    hipLaunchKernelGGL(k_pcg_dots<TZ>, dim3(nb), dim3(256), 0, S(stream), n, r, z, qv, init, st, red);
+    hipLaunchKernelGGL(k_fin_dots, dim3(1), dim3(256), 0, S(stream), nb, n, init, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }

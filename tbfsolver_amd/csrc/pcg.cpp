@@ -31,7 +31,7 @@ size_t tail_cells_limit()
         long v = std::atol(e);
         if (v > 0) return (size_t)v;
     }
-    return 32768;
+    return 4096;
 }
 
 constexpr int kMaxBottomCells = 512;
@@ -137,7 +137,8 @@ static int build_plan(tbfg_pcg* h)
     TailPlan<T> P;
     std::memset(&P, 0, sizeof P);
     P.nlev = h->ntot - h->t0;
-    P.user_last = (h->nuser - 1) - h->t0;
+    // with coarse_max_iter == 0 the V-cycle runs straight to the dense bottom
+    P.user_last = (h->prm.coarse_max_iter > 0 ? h->nuser - 1 : h->ntot - 1) - h->t0;
     P.bottom = (h->ntot - 1) - h->t0;
     P.n_pre = h->prm.n_pre;
     P.n_post = h->prm.n_post;
@@ -174,7 +175,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         return TBFG_EINVAL;
     }
     if (prm.n_pre < 0 || prm.n_post < 0 || prm.n_pre + prm.n_post < 1 || prm.mg_cycles < 1 || prm.max_iter < 0 ||
-        !(prm.tol >= 0.0) || prm.coarse_max_iter < 0 || !(prm.coarse_tol >= 0.0)) {
+        !(prm.tol >= 0.0) || prm.coarse_max_iter < -1 || !(prm.coarse_tol >= 0.0)) {
         set_error("invalid PCG/MG parameters (pre %d post %d cycles %d max_iter %d tol %g coarse %d/%g)", prm.n_pre,
                   prm.n_post, prm.mg_cycles, prm.max_iter, prm.tol, prm.coarse_max_iter, prm.coarse_tol);
         return TBFG_EINVAL;
@@ -197,10 +198,15 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     h->mixed = prm.precision == TBFG_PREC_MIXED;
     if (h->prm.flexible < 0) h->prm.flexible = h->mixed ? 1 : 0;
     if (!(h->prm.coarse_tol > 0.0)) h->prm.coarse_tol = h->mixed ? 1e-5 : 1e-10;
+    if (h->prm.coarse_max_iter < 0) h->prm.coarse_max_iter = h->mixed ? 0 : 40;
+    prm.coarse_max_iter = h->prm.coarse_max_iter;
     h->nuser = prm.n_levels;
     // sub-hierarchy under the coarsest level: halve while every axis is even
+    // The sub-hierarchy serves the coarsest-level solve: with coarse_max_iter > 0
+    // as the preconditioner of an inner CG, with coarse_max_iter == 0 as the
+    // continuation of the V-cycle itself (one cycle from zero, dense bottom).
     int ntot = h->nuser;
-    if (prm.coarse_max_iter > 0) {
+    {
         for (;;) {
             bool ok = true;
             size_t cells = 1;
@@ -218,15 +224,19 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         size_t cb = 1;
         for (int a = 0; a < 3; ++a) cb *= (size_t)(g->n[a] >> (ntot - 1));
         if (cb > (size_t)kMaxBottomCells) {
-            set_error("the coarsest grid has %zu cells; the dense coarsest solve needs <= %d (use more levels "
-                      "or coarse_max_iter > 0 on an axis-even grid)",
+            set_error("the coarsest grid has %zu cells after halving every axis while all are even; the dense "
+                      "bottom solve needs <= %d",
                       cb, kMaxBottomCells);
             return TBFG_EINVAL;
         }
     }
     const size_t tail_lim = tail_cells_limit();
-    int t0 = h->nuser - 1;
-    for (int l = 1; l < h->nuser; ++l) {
+    // first level of the single-workgroup tail: the first small level, and at
+    // the latest the reference's coarsest level when that one is solved by the
+    // inner CG (which runs inside the tail)
+    const int t_cap = prm.coarse_max_iter > 0 ? h->nuser - 1 : ntot - 1;
+    int t0 = t_cap;
+    for (int l = 1; l < t_cap; ++l) {
         size_t cells = 1;
         for (int a = 0; a < 3; ++a) cells *= (size_t)(g->n[a] >> l);
         if (cells <= tail_lim) {
@@ -287,7 +297,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         TBFG_TRY(upload(h->vol, vol.data(), nbc * 8));
     }
     // inner CG vectors and cell volumes of the coarsest user level
-    {
+    if (prm.coarse_max_iter > 0) {
         const HostLevel& U = h->hl[h->nuser - 1];
         const size_t nu = U.cells();
         TBFG_TRY(h->cx.alloc(nu * es));

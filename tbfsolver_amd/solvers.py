@@ -124,7 +124,7 @@ class PCG:
     `iterations` the number of direction updates."""
 
     def __init__(self, grid, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=100,
-                 precision="fp64", mg_cycles=1, coarse_max_iter=40, coarse_tol=0.0, flexible=None):
+                 precision="fp64", mg_cycles=1, coarse_max_iter=-1, coarse_tol=0.0, flexible=None):
         # pcgCTOR + multiGridCTOR
         self.grid = grid
         p = default_params()

@@ -117,6 +117,24 @@ def test_pcg_fp64_matches_oracle(torch_gpu, n, stretch, nlev, nb):
         assert true_res < 2e-12
 
 
+@pytest.mark.parametrize("n,stretch,nlev,nb", [((64, 64, 64), 0, 4, 18), ((128, 96, 128), 1, 4, 30)])
+def test_pcg_vcycle_coarse_mode_matches_oracle(torch_gpu, n, stretch, nlev, nb):
+    """coarse_max_iter=0: the V-cycle continues below the coarsest reference level
+    (one cycle, dense bottom); the oracle's cascade mode with one cycle is the
+    same preconditioner."""
+    print("case pcg vcycle-coarse", n, stretch, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, nb, stretch, 23)
+    s = T.PCG(g, nLevels=nlev, tolPCG=1e-12, maxIterPCG=200, flexible=1, coarse_max_iter=0)
+    psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+    s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+    ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=nlev, tol=1e-12, max_iter=200, coarse_mode=1,
+                                                coarse_cycles=1, flexible=1), beta, q)
+    print("  gpu iterations", s.iterations, "oracle", it, flush=True)
+    assert rc == 0 and s.converged
+    assert abs(s.iterations - it) <= 1, (s.iterations, it)
+    assert rel(mean_free(to_host(psi)), mean_free(ref)) < 1e-10
+
+
 @pytest.mark.parametrize("n,stretch,nb", [((64, 64, 64), 0, 18), ((64, 48, 64), 1, 20)])
 def test_pcg_mixed_vs_fp64_oracle(torch_gpu, n, stretch, nb):
     print("case pcg mixed", n, stretch, flush=True)
