@@ -51,7 +51,31 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_TEST = os.path.join(FORTRAN_DIR, "tbfgpu_test")
+
+
+def build_fortran(verbose=False):
+    """Compiles the iso_c_binding module and its test driver (needs flang)."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("flang not found: Fortran binding not built")
+        return None
+    os.makedirs(OBJ, exist_ok=True)
+    cmd = [FLANG, "-O2", "-J", OBJ, os.path.join(FORTRAN_DIR, "tbfgpu_mod.f90"),
+           os.path.join(FORTRAN_DIR, "tbfgpu_test.f90"), "-o", FORTRAN_TEST, "-L" + HERE, "-ltbfgpu",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib/llvm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    if verbose:
+        print("built", FORTRAN_TEST)
+    return FORTRAN_TEST
 
 
 def clean():
