@@ -7,6 +7,8 @@
 #include "mg_launch.hpp"
 #include "reduce.hpp"
 
+#include <algorithm>
+
 namespace tbfg {
 
 // ------------------------------------------------------------ device helpers
@@ -76,7 +78,11 @@ __global__ void k_apply_op(LevelDev<double> L, const double* __restrict__ beta, 
     q[c] = apply_cell<double>(a, p, c, e, w, n, s, f, b);
 }
 
-// qv = A d and alpha = delta / (d . qv) (pcg.f90:144-224).
+// qv = A d and alpha = delta / (d . qv) (pcg.f90:144-224).  One cell of each
+// of kOpPlanes consecutive planes per thread: the loads of all of them are in
+// flight together (a longer per-thread loop over planes is latency bound).
+constexpr int kOpPlanes = 2;
+
 __global__ void k_op_dot(LevelDev<double> L, const double* __restrict__ beta, const double* __restrict__ d,
                          double* __restrict__ qv, PcgState* st, Reducer red)
 {
@@ -85,15 +91,17 @@ __global__ void k_op_dot(LevelDev<double> L, const double* __restrict__ beta, co
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     double v[1] = {0.0};
     if (i < L.nx && j < L.ny) {
-        const int kc = (L.nz + gridDim.z - 1) / gridDim.z;  // contiguous planes: neighbours stay cached
-        const int k1 = min(L.nz, (int)(blockIdx.z + 1) * kc);
-        for (int k = blockIdx.z * kc; k < k1; ++k) {
-            size_t c, e, w, n, s, f, b;
-            neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
-            Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
-            const double out = apply_cell<double>(a, d, c, e, w, n, s, f, b);
-            qv[c] = out;
-            v[0] += d[c] * out;
+#pragma unroll
+        for (int u = 0; u < kOpPlanes; ++u) {
+            const int k = blockIdx.z * kOpPlanes + u;
+            if (k < L.nz) {
+                size_t c, e, w, n, s, f, b;
+                neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
+                Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
+                const double out = apply_cell<double>(a, d, c, e, w, n, s, f, b);
+                qv[c] = out;
+                v[0] += d[c] * out;
+            }
         }
     }
     block_partial<1>(v, red);
@@ -117,8 +125,10 @@ __global__ void k_fin_alpha(unsigned nblocks, PcgState* st, Reducer red)
 }
 
 // r = q - A psi with ||r||^2, ||q||^2 (computeResiduals, rbgs.f90:276-333, used at pcg.f90:116).
+// r32 (optional): an FP32 copy of r, the right-hand side of the FP32 multigrid.
 __global__ void k_residual_init(LevelDev<double> L, const double* __restrict__ beta, const double* __restrict__ q,
-                                const double* __restrict__ psi, double* __restrict__ r, PcgState* st, Reducer red)
+                                const double* __restrict__ psi, double* __restrict__ r, float* __restrict__ r32,
+                                PcgState* st, Reducer red)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -133,6 +143,7 @@ __global__ void k_residual_init(LevelDev<double> L, const double* __restrict__ b
             const double qc = q[c];
             const double rc = qc - apply_cell<double>(a, psi, c, e, w, n, s, f, b);
             r[c] = rc;
+            if (r32) r32[c] = (float)rc;
             v[0] += rc * rc;
             v[1] += qc * qc;
         }
@@ -165,7 +176,7 @@ __global__ void k_fin_init(unsigned nblocks, PcgState* st, Reducer red)
 }
 
 // psi += alpha d; r -= alpha qv; ||r|| and the stop test (pcg.f90:228-262,331-355,359-429).
-__global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __restrict__ r,
+__global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __restrict__ r, float* __restrict__ r32,
                              const double* __restrict__ d, const double* __restrict__ qv, PcgState* st, Reducer red)
 {
     if (st->done) return;
@@ -176,6 +187,7 @@ __global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __r
         psi[t] = psi[t] + alpha * d[t];
         const double rn = r[t] - alpha * qv[t];
         r[t] = rn;
+        if (r32) r32[t] = (float)rn;
         v[0] += rn * rn;
     }
     block_partial<1>(v, red);
@@ -778,39 +790,48 @@ static unsigned red_blocks1(size_t n)
     return b < kRedBlocks ? b : kRedBlocks;
 }
 
-unsigned cell_blocks(const LevelDev<double>& L)
+static dim3 op_grid(const LevelDev<double>& L, dim3 blk)
 {
-    dim3 g = red_grid3(L, block2d(L.nx));
-    return g.x * g.y * g.z;
+    return dim3(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), ceil_div(L.nz, kOpPlanes));
+}
+
+size_t reduction_slots(const LevelDev<double>& L)
+{
+    const dim3 g3 = red_grid3(L, block2d(L.nx));
+    const dim3 go = op_grid(L, block2d(L.nx));
+    const size_t chunked = (size_t)g3.x * g3.y * g3.z * 5;  // dots: 5 sums per block
+    const size_t flat = (size_t)kRedBlocks * 5;
+    const size_t op = (size_t)go.x * go.y * go.z;
+    return std::max(std::max(chunked, flat), op);
 }
 
 int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d, double* qv, PcgState* st,
                   const Reducer& red, void* stream)
 {
     dim3 blk = block2d(L.nx);
-    dim3 grd = red_grid3(L, blk);
+    dim3 grd = op_grid(L, blk);
     hipLaunchKernelGGL(k_op_dot, grd, blk, 0, S(stream), L, beta, d, qv, st, red);
-    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, S(stream), grd.x * grd.y * grd.z, st, red);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, S(stream), grd.x * grd.y * grd.z, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
 
 int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
-                         double* r, PcgState* st, const Reducer& red, void* stream)
+                         double* r, float* r32, PcgState* st, const Reducer& red, void* stream)
 {
     dim3 blk = block2d(L.nx);
     dim3 grd = red_grid3(L, blk);
-    hipLaunchKernelGGL(k_residual_init, grd, blk, 0, S(stream), L, beta, q, psi, r, st, red);
+    hipLaunchKernelGGL(k_residual_init, grd, blk, 0, S(stream), L, beta, q, psi, r, r32, st, red);
     hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, S(stream), grd.x * grd.y * grd.z, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
 
-int launch_pcg_update(size_t n, double* psi, double* r, const double* d, const double* qv, PcgState* st,
-                      const Reducer& red, void* stream)
+int launch_pcg_update(size_t n, double* psi, double* r, float* r32, const double* d, const double* qv,
+                      PcgState* st, const Reducer& red, void* stream)
 {
     const unsigned nb = red_blocks1(n);
-    hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, S(stream), n, psi, r, d, qv, st, red);
+    hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, S(stream), n, psi, r, r32, d, qv, st, red);
     hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, S(stream), nb, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;

@@ -4,15 +4,16 @@
 
 namespace tbfg {
 
-unsigned cell_blocks(const LevelDev<double>& L);
+// doubles of reduction scratch the CG kernels of a level need
+size_t reduction_slots(const LevelDev<double>& L);
 
 int launch_apply_op(const LevelDev<double>& L, const double* beta, const double* p, double* q, void* stream);
 int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d, double* qv, PcgState* st,
                   const Reducer& red, void* stream);
 int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
-                         double* r, PcgState* st, const Reducer& red, void* stream);
-int launch_pcg_update(size_t n, double* psi, double* r, const double* d, const double* qv, PcgState* st,
-                      const Reducer& red, void* stream);
+                         double* r, float* r32, PcgState* st, const Reducer& red, void* stream);
+int launch_pcg_update(size_t n, double* psi, double* r, float* r32, const double* d, const double* qv,
+                      PcgState* st, const Reducer& red, void* stream);
 template <class TZ>
 int launch_pcg_dots(size_t n, const double* r, const TZ* z, const double* qv, int init, PcgState* st,
                     const Reducer& red, void* stream);
@@ -22,6 +23,12 @@ int launch_pcg_dir(size_t n, const TZ* z, double* d, int init, const PcgState* s
 template <class T, class TQ>
 int launch_rbgs(const LevelDev<T>& L, const T* beta, const TQ* q, T* p, int colour, bool first, const int* done,
                 void* stream);
+// one full red/black sweep, out of place (pin -> pout); levels with nx % 32 == 0
+template <class T>
+bool sweep_supported(const LevelDev<T>& L);
+template <class T, class TQ>
+int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
+                 const int* done, void* stream);
 template <class T, class TQ>
 int launch_resid_restrict(const LevelDev<T>& L, const T* beta, const TQ* q, const T* p, T* qc, const int* done,
                           void* stream);

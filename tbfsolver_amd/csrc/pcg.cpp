@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
 
 #include "mg_launch.hpp"
 
@@ -35,6 +36,7 @@ size_t tail_cells_limit()
 }
 
 constexpr int kMaxBottomCells = 512;
+constexpr int kMaxLevels = 32;
 
 }  // namespace
 
@@ -58,7 +60,7 @@ struct ProfScope {
     bool on;
     ProfScope(tbfg_pcg* h_, int kernel, void* stream) : h(h_), s((hipStream_t)stream)
     {
-        on = h->prof_kernel == kernel && h->prof_used < kProfPairs;
+        on = kernel >= 0 && h->prof_kernel == kernel && h->prof_used < kProfPairs;
         if (on) (void)hipEventRecord(h->prof_ev[2 * h->prof_used], s);
     }
     ~ProfScope()
@@ -109,13 +111,13 @@ int pcg_profile_end(tbfg_pcg* h, int64_t* launches, double* total_ms, double* by
     TBFG_HIP(hipDeviceSynchronize());
     prof_harvest(h);
     const double n = (double)h->n0;
-    const double bp = h->mixed ? 4.0 : 8.0;  // MG storage (p, beta on the finest level)
+    const double bp = h->mixed ? 4.0 : 8.0;  // MG storage (p, beta, q of the finest level)
     double bytes = 0.0;
     switch (h->prof_kernel) {
-        case 0: bytes = bp * n + bp * n + 8.0 * n / 2; break;         // p (other colour in, own out), beta, q own colour
-        case 1: bytes = bp * n + bp * n + 8.0 * n + bp * n / 8; break; // p, beta, r in; coarse q out
-        case 2: bytes = 24.0 * n; break;                               // d, beta in; qv out
-        case 3: bytes = 48.0 * n; break;                               // psi, r, d, qv in; psi, r out
+        case 0: bytes = 4.0 * bp * n; break;                               // full sweep: p, beta, q in; p out
+        case 1: bytes = 3.0 * bp * n + bp * n / 8; break;                  // p, beta, q in; coarse q out
+        case 2: bytes = 24.0 * n; break;                                   // d, beta in; qv out
+        case 3: bytes = 48.0 * n + (h->mixed ? 4.0 * n : 0.0); break;      // psi, r, d, qv in; psi, r (+ FP32 r) out
     }
     if (launches) *launches = h->prof_launches;
     if (total_ms) *total_ms = h->prof_ms;
@@ -266,20 +268,27 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     }
     const size_t es = h->mixed ? sizeof(float) : sizeof(double);
     h->p.resize(ntot);
+    h->palt.resize(ntot);
     h->q.resize(ntot);
     h->beta.resize(ntot);
+    h->fused.assign(ntot, 0);
     for (int l = 0; l < ntot; ++l) {
         const size_t n = h->hl[l].cells();
         TBFG_TRY(h->p[l].alloc(n * es));
         if (l > 0) TBFG_TRY(h->q[l].alloc(n * es));
         if (l > 0 || h->mixed) TBFG_TRY(h->beta[l].alloc(n * es));
+        if (l < t0) {
+            h->fused[l] = h->mixed ? sweep_supported<float>(h->tabf[l]->view) : sweep_supported<double>(h->tabd[l]->view);
+            if (std::getenv("TBFG_HALF_SWEEPS")) h->fused[l] = 0;
+            if (h->fused[l]) TBFG_TRY(h->palt[l].alloc(n * es));
+        }
     }
+    if (h->mixed) TBFG_TRY(h->r32.alloc(h->n0 * 4));
     TBFG_TRY(h->r.alloc(h->n0 * 8));
     TBFG_TRY(h->d.alloc(h->n0 * 8));
     TBFG_TRY(h->qv.alloc(h->n0 * 8));
     TBFG_TRY(h->st.alloc(sizeof(PcgState)));
-    const unsigned nb = std::max(cell_blocks(h->tabd[0]->view), ceil_div(h->n0, 256));
-    TBFG_TRY(h->partials.alloc((size_t)nb * 8 * sizeof(double)));
+    TBFG_TRY(h->partials.alloc(reduction_slots(h->tabd[0]->view) * sizeof(double)));
     TBFG_TRY(h->counter.alloc(256));
     TBFG_HIP(hipMemset(h->counter.p, 0, 256));
     // dense bottom operator
@@ -326,11 +335,12 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     // footprint: every device buffer of the handle
     int64_t fp = 0;
     for (auto& b : h->p) fp += (int64_t)b.bytes;
+    for (auto& b : h->palt) fp += (int64_t)b.bytes;
     for (auto& b : h->q) fp += (int64_t)b.bytes;
     for (auto& b : h->beta) fp += (int64_t)b.bytes;
     for (auto& t : h->tabd) fp += (int64_t)t->buf.bytes;
     for (auto& t : h->tabf) fp += (int64_t)t->buf.bytes;
-    fp += (int64_t)(h->r.bytes + h->d.bytes + h->qv.bytes + h->st.bytes + h->partials.bytes + h->counter.bytes +
+    fp += (int64_t)(h->r.bytes + h->r32.bytes + h->d.bytes + h->qv.bytes + h->st.bytes + h->partials.bytes + h->counter.bytes +
                     h->plan.bytes + h->G.bytes + h->dwork.bytes + h->vol.bytes + h->cx.bytes + h->cr.bytes +
                     h->cd.bytes + h->cad.bytes + h->cvol.bytes);
     h->footprint = fp;
@@ -359,64 +369,65 @@ static int setup_levels(tbfg_pcg* h, const double* beta0, void* stream)
     return TBFG_OK;
 }
 
-// One preconditioner application z = M r (multiGrid.f90:165-232, fixed cycles).
+// nsweeps red/black sweeps on level l (rbgs.f90:205-259); `first`: the level
+// starts from p = 0.  Levels with the full-sweep kernel alternate between
+// their two buffers, so p (and alt) may be swapped on return.
 template <class T>
-static int vcycle(tbfg_pcg* h, const double* r, T* z, const T* beta0, const int* done, void* stream)
+static int smooth(tbfg_pcg* h, int l, const T* b, const T* q, T*& p, T*& alt, int nsweeps, bool first,
+                  const int* done, void* stream)
+{
+    const LevelDev<T>& L = geo<T>(h, l);
+    for (int s = 0; s < nsweeps; ++s) {
+        const bool f0 = first && s == 0;
+        if (h->fused[l]) {
+            ProfScope ps(h, (l == 0 && !f0) ? 0 : -1, stream);
+            TBFG_TRY(launch_sweep<T, T>(L, b, q, p, alt, f0, done, stream));
+            std::swap(p, alt);
+        } else {
+            ProfScope ps(h, l == 0 ? 0 : -1, stream);
+            TBFG_TRY(launch_rbgs<T, T>(L, b, q, p, RED, f0, done, stream));
+            TBFG_TRY(launch_rbgs<T, T>(L, b, q, p, BLACK, f0, done, stream));
+        }
+    }
+    return TBFG_OK;
+}
+
+// One preconditioner application z = M r (multiGrid.f90:165-232, fixed cycles).
+// r0 is the finest right-hand side in the multigrid precision; on return *z
+// points at the level-0 buffer that holds the result.
+template <class T>
+static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* done, void* stream)
 {
     const int t0 = h->t0;
+    T* cur[kMaxLevels];
+    T* alt[kMaxLevels];
+    for (int l = 0; l < t0; ++l) {
+        cur[l] = h->p[l].as<T>();
+        alt[l] = h->palt[l].as<T>();
+    }
     for (int cyc = 0; cyc < h->prm.mg_cycles; ++cyc) {
         for (int l = 0; l < t0; ++l) {
             const bool first = (l > 0) || (cyc == 0);
-            T* p = (l == 0) ? z : h->p[l].as<T>();
             const T* b = (l == 0) ? beta0 : h->beta[l].as<T>();
-            if (h->prm.n_pre == 0 && first) TBFG_HIP(hipMemsetAsync(p, 0, h->hl[l].cells() * sizeof(T), (hipStream_t)stream));
-            for (int s = 0; s < h->prm.n_pre; ++s) {
-                const bool f0 = first && s == 0;
-                if (l == 0) {
-                    {
-                        ProfScope ps(h, 0, stream);
-                        TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, RED, f0, done, stream));
-                    }
-                    {
-                        ProfScope ps(h, 0, stream);
-                        TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, BLACK, f0, done, stream));
-                    }
-                } else {
-                    const T* q = h->q[l].as<T>();
-                    TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, RED, f0, done, stream));
-                    TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, BLACK, f0, done, stream));
-                }
+            const T* q = (l == 0) ? r0 : h->q[l].as<T>();
+            if (h->prm.n_pre == 0 && first)
+                TBFG_HIP(hipMemsetAsync(cur[l], 0, h->hl[l].cells() * sizeof(T), (hipStream_t)stream));
+            TBFG_TRY(smooth<T>(h, l, b, q, cur[l], alt[l], h->prm.n_pre, first, done, stream));
+            {
+                ProfScope ps(h, l == 0 ? 1 : -1, stream);
+                TBFG_TRY(launch_resid_restrict<T, T>(geo<T>(h, l), b, q, cur[l], h->q[l + 1].as<T>(), done, stream));
             }
-            if (l == 0) {
-                ProfScope ps(h, 1, stream);
-                TBFG_TRY(launch_resid_restrict<T, double>(geo<T>(h, 0), b, r, p, h->q[1].as<T>(), done, stream));
-            } else
-                TBFG_TRY(launch_resid_restrict<T, T>(geo<T>(h, l), b, h->q[l].as<T>(), p, h->q[l + 1].as<T>(), done,
-                                                     stream));
         }
         TBFG_TRY(launch_tail<T>(h->plan.as<TailPlan<T>>(), done, stream));
         for (int l = t0 - 1; l >= 0; --l) {
-            T* p = (l == 0) ? z : h->p[l].as<T>();
             const T* b = (l == 0) ? beta0 : h->beta[l].as<T>();
-            TBFG_TRY(launch_prolong_add<T>(geo<T>(h, l), h->p[l + 1].as<T>(), p, done, stream));
-            for (int s = 0; s < h->prm.n_post; ++s) {
-                if (l == 0) {
-                    {
-                        ProfScope ps(h, 0, stream);
-                        TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, RED, false, done, stream));
-                    }
-                    {
-                        ProfScope ps(h, 0, stream);
-                        TBFG_TRY(launch_rbgs<T, double>(geo<T>(h, 0), b, r, p, BLACK, false, done, stream));
-                    }
-                } else {
-                    const T* q = h->q[l].as<T>();
-                    TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, RED, false, done, stream));
-                    TBFG_TRY(launch_rbgs<T, T>(geo<T>(h, l), b, q, p, BLACK, false, done, stream));
-                }
-            }
+            const T* q = (l == 0) ? r0 : h->q[l].as<T>();
+            const T* pc = (l + 1 < t0) ? cur[l + 1] : h->p[l + 1].as<T>();
+            TBFG_TRY(launch_prolong_add<T>(geo<T>(h, l), pc, cur[l], done, stream));
+            TBFG_TRY(smooth<T>(h, l, b, q, cur[l], alt[l], h->prm.n_post, false, done, stream));
         }
     }
+    *z = cur[0];
     return TBFG_OK;
 }
 
@@ -439,30 +450,36 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
         beta0 = h->beta[0].as<T>();
     else
         beta0 = beta;
-    T* z = h->p[0].as<T>();
+    T* z = nullptr;
     const LevelDev<double>& L0 = h->tabd[0]->view;
     const size_t n = h->n0;
     double* r = h->r.as<double>();
     double* d = h->d.as<double>();
     double* qv = h->qv.as<double>();
-    TBFG_TRY(launch_residual_init(L0, beta, q, psi, r, st, red, stream));
-    TBFG_TRY(vcycle<T>(h, r, z, beta0, done, stream));
+    float* r32 = h->mixed ? h->r32.as<float>() : nullptr;
+    const T* r0;
+    if constexpr (sizeof(T) == 4)
+        r0 = r32;
+    else
+        r0 = r;
+    TBFG_TRY(launch_residual_init(L0, beta, q, psi, r, r32, st, red, stream));
+    TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream));
     TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 1, st, red, stream));
-    TBFG_TRY(launch_pcg_dir<T>(n, z, d, 1, st, stream));
     TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     TBFG_HIP(hipStreamSynchronize(s));
-    while (!h->hst->done) {
+    for (int init = 1; !h->hst->done; init = 0) {
+        // d = (z - m) + gamma d, then qv = A d and alpha (pcg.f90:266-290, 144-224)
+        TBFG_TRY(launch_pcg_dir<T>(n, z, d, init, st, stream));
         {
             ProfScope ps(h, 2, stream);
             TBFG_TRY(launch_op_dot(L0, beta, d, qv, st, red, stream));
         }
         {
             ProfScope ps(h, 3, stream);
-            TBFG_TRY(launch_pcg_update(n, psi, r, d, qv, st, red, stream));
+            TBFG_TRY(launch_pcg_update(n, psi, r, r32, d, qv, st, red, stream));
         }
-        TBFG_TRY(vcycle<T>(h, r, z, beta0, done, stream));
+        TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream));
         TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 0, st, red, stream));
-        TBFG_TRY(launch_pcg_dir<T>(n, z, d, 0, st, stream));
         TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         TBFG_HIP(hipStreamSynchronize(s));
         if (h->prof_kernel >= 0) prof_harvest(h);
@@ -505,8 +522,15 @@ static int precond_t(tbfg_pcg* h, const double* beta, const double* r, double* z
         beta0 = h->beta[0].as<T>();
     else
         beta0 = beta;
-    T* zt = h->p[0].as<T>();
-    TBFG_TRY(vcycle<T>(h, r, zt, beta0, nullptr, stream));
+    const T* r0;
+    if constexpr (sizeof(T) == 4) {
+        TBFG_TRY(launch_convert<double, float>(h->n0, r, h->r32.as<float>(), stream));
+        r0 = h->r32.as<float>();
+    } else {
+        r0 = r;
+    }
+    T* zt = nullptr;
+    TBFG_TRY(vcycle<T>(h, r0, &zt, beta0, nullptr, stream));
     // z with its arithmetic mean removed, as the reference's sweeps leave it
     TBFG_TRY(launch_pcg_dots<T>(h->n0, r, zt, nullptr, 1, st, red, stream));
     TBFG_TRY(launch_pcg_dir<T>(h->n0, zt, z, 1, st, stream));

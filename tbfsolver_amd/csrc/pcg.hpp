@@ -21,7 +21,10 @@ struct tbfg_pcg {
     // per-level MG arrays (element type float in mixed mode, double otherwise);
     // level 0: p = z, q = r (double), beta = caller's (FP64) or a float copy (mixed)
     std::vector<tbfg::DevBuf> p, q, beta;
+    std::vector<tbfg::DevBuf> palt;     // second p buffer of the levels smoothed out of place
+    std::vector<char> fused;            // level smoothed by the out-of-place full-sweep kernel
     tbfg::DevBuf r, d, qv;              // CG vectors (FP64)
+    tbfg::DevBuf r32;                   // FP32 copy of r (mixed precision: the multigrid's right-hand side)
     tbfg::DevBuf st, partials, counter; // PcgState + reduction scratch
     tbfg::DevBuf plan;                  // TailPlan<T>
     tbfg::DevBuf G, dwork, vol;         // dense bottom operator and its set-up scratch

@@ -1,0 +1,453 @@
+// sweep.hip -- one full red/black Gauss-Seidel sweep per launch (rbgs.f90:205-259:
+// the red half-sweep, then the black one), out of place: pin holds the level's
+// values before the sweep and is only read, pout receives the values after it.
+//
+// A workgroup owns a TX x TY tile of cells in the (x, y) plane and a chunk of
+// planes in z, and marches through the chunk one plane at a time.  At plane
+// step k it computes the red cells of plane k over the tile plus a one-cell
+// ring and then the black cells of plane k-1 over the tile, which need the new
+// red values of planes k-2, k-1, k.  The ring and the two extra red planes at
+// the chunk ends are recomputed by every workgroup that needs them, from the
+// old values in pin, so no workgroup waits on another and the result equals
+// the two half-sweep launches exactly.
+//
+// Every input plane is staged once in LDS (p and beta with a two-cell ring, q
+// with a one-cell ring), loaded into registers one step ahead so that the HBM
+// latency of plane k+2 overlaps the arithmetic of plane k.  HBM traffic per
+// cell: p in, beta, q, p out -- once per sweep, where two half-sweep launches
+// stream every array twice.
+#include "mg_kernels.hpp"
+#include "mg_launch.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+
+namespace tbfg {
+
+namespace {
+
+constexpr int kSweepMaxChunk = 60;  // planes per workgroup (z face factors staged in LDS)
+
+template <class T, int TX, int TY, int NT>
+struct SweepShape {
+    static constexpr int PX = TX + 4, PY = TY + 4, PS = PX * PY;  // p, beta planes: tile + 2-cell ring
+    static constexpr int QX = TX + 2, QY = TY + 2, QS = QX * QY;  // q, new-red planes: tile + 1-cell ring
+    static constexpr int NP = (PS + NT - 1) / NT;
+    static constexpr int NQ = (QS + NT - 1) / NT;
+    static constexpr int NBLACK = TY * (TX / 2);                    // black cells of a tile plane
+    // ring depths: a slot is refilled two steps after its last reader, so two
+    // barriers per plane step suffice
+    static constexpr int SP = 4, SB = 5, SQ = 3, SR = 3;
+};
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+__device__ __forceinline__ int clampi(int j, int n) { return j < 0 ? 0 : (j >= n ? n - 1 : j); }
+
+template <int NT, class T, int N>
+__device__ __forceinline__ void stash_plane(const T (&reg)[N], T* __restrict__ dst, int S)
+{
+#pragma unroll
+    for (int m = 0; m < N; ++m) dst[min((int)threadIdx.x + m * NT, S - 1)] = reg[m];
+}
+
+template <class T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+    using type = float2;
+};
+template <>
+struct Vec2<double> {
+    using type = double2;
+};
+
+// Per-thread column history: the values a thread used at its tile cell two
+// plane steps earlier (same colour parity, same column) that the next step of
+// the same parity needs again.
+template <class T>
+struct ColHist {
+    T bf;  // beta(k-1)
+    T bc;  // beta(k-2)
+    T pf;  // p_old(k-1)
+    T r;   // new red value at k-2
+};
+
+// The tile height TY divides ny, so every thread stores on every step: no
+// branch around the stores, which lets the compiler wait for the loads of a
+// plane without waiting for the stores issued after them.
+//
+// Thread t owns the pair of columns (tx0 + 2h, tx0 + 2h + 1) of tile row
+// t / (TX/2).  At plane step k its red cell of plane k and its black cell of
+// plane k-1 share one column (both have gx + gy + k odd), which alternates
+// between the two columns of the pair from step to step; the beta, p_old and
+// red values of that column from two steps earlier stay in registers.  The
+// red cells of the ring around the tile are a second, smaller round.
+template <class T, class TQ, int TX, int TY, int NT, bool FIRST>
+__global__ void __launch_bounds__(NT)
+    k_sweep(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q, const T* __restrict__ pin,
+            T* __restrict__ pout, int kc, int tiles_x, int ntiles, const int* done)
+{
+    using Sh = SweepShape<T, TX, TY, NT>;
+    static_assert(Sh::NBLACK == NT, "one tile cell pair per thread");
+    constexpr int PX = Sh::PX, PS = Sh::PS, QX = Sh::QX, QS = Sh::QS, QY = Sh::QY;
+    constexpr int SP = FIRST ? 1 : Sh::SP;
+    constexpr int NRING = QX + TY;  // red cells of the one-cell ring
+    __shared__ T sP[SP * PS];
+    __shared__ T sB[Sh::SB * PS];
+    __shared__ T sQ[Sh::SQ * QS];
+    __shared__ T sR[Sh::SR * QS];
+    __shared__ T sTx[2 * QX];        // x face factors of the tile columns -1..TX
+    __shared__ T sTy[2 * QY];        // y face factors of the tile rows -1..TY
+    __shared__ T sTz[2 * (kSweepMaxChunk + 2)];  // z face factors of the planes k0-1..k1
+
+    if (done && *done) return;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    // consecutive tiles of a chunk on one XCD (blocks go round-robin to the 8 XCDs)
+    unsigned id = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) id = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const int tile = (int)(id % (unsigned)ntiles);
+    const int chunk = (int)(id / (unsigned)ntiles);
+    const int k0 = chunk * kc;
+    if (k0 >= nz) return;
+    const int k1 = min(nz, k0 + kc);
+    const int tx0 = (tile % tiles_x) * TX;
+    const int ty0 = (tile / tiles_x) * TY;
+
+    auto slotP = [&](int k) { return FIRST ? 0 : (k - k0 + 4) % Sh::SP; };
+    auto slotB = [&](int k) { return (k - k0 + 5) % Sh::SB; };
+    auto slotQ = [&](int k) { return (k - k0 + 3) % Sh::SQ; };
+    auto slotR = [&](int k) { return (k - k0 + 3) % Sh::SR; };
+    auto modz = [&](int k) { return ((k % nz) + nz) % nz; };
+
+    for (int t = threadIdx.x; t < QX; t += NT) {
+        const int gx = wrapi(tx0 + t - 1, nx);
+        sTx[t] = L.a[0].ip[gx];
+        sTx[QX + t] = L.a[0].im[gx];
+    }
+    for (int t = threadIdx.x; t < QY; t += NT) {
+        const int gy = clampi(ty0 + t - 1, ny);
+        sTy[t] = L.a[1].ip[gy];
+        sTy[QY + t] = L.a[1].im[gy];
+    }
+    // (vector loads of these inside the plane loop would wait behind the
+    // prefetch of the next planes)
+    for (int t = threadIdx.x; t < k1 - k0 + 2; t += NT) {
+        const int kw = wrapi(k0 - 1 + t, nz);
+        sTz[t] = L.a[2].ip[kw];
+        sTz[kSweepMaxChunk + 2 + t] = L.a[2].im[kw];
+    }
+
+    // in-plane offsets of the cells this thread loads (fixed for the chunk)
+    int offp[Sh::NP], offq[Sh::NQ];
+#pragma unroll
+    for (int m = 0; m < Sh::NP; ++m) {
+        const int t = min((int)threadIdx.x + m * NT, PS - 1);
+        const int yy = t / PX, xx = t - yy * PX;
+        offp[m] = nx * clampi(ty0 + yy - 2, ny) + wrapi(tx0 + xx - 2, nx);
+    }
+#pragma unroll
+    for (int m = 0; m < Sh::NQ; ++m) {
+        const int t = min((int)threadIdx.x + m * NT, QS - 1);
+        const int yy = t / QX, xx = t - yy * QX;
+        offq[m] = nx * clampi(ty0 + yy - 1, ny) + wrapi(tx0 + xx - 1, nx);
+    }
+    const size_t plane_sz = (size_t)nx * ny;
+    // planes of p and beta at kp, of q at kp - 1, into registers
+    auto fetch_pb = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], int kp) {
+        const T* pp = pin + plane_sz * modz(kp);
+        const T* bp = beta + plane_sz * modz(kp);
+#pragma unroll
+        for (int m = 0; m < Sh::NP; ++m) {
+            if (!FIRST) rp[m] = pp[offp[m]];
+            rb[m] = bp[offp[m]];
+        }
+    };
+    auto fetch = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], int kp) {
+        fetch_pb(rp, rb, kp);
+        const TQ* qp = q + plane_sz * modz(kp - 1);
+#pragma unroll
+        for (int m = 0; m < Sh::NQ; ++m) rq[m] = (T)qp[offq[m]];
+    };
+    auto stash = [&](const T(&rp)[Sh::NP], const T(&rb)[Sh::NP], int k) {
+        if (!FIRST) stash_plane<NT, T, Sh::NP>(rp, sP + slotP(k) * PS, PS);
+        stash_plane<NT, T, Sh::NP>(rb, sB + slotB(k) * PS, PS);
+    };
+
+    // this thread's tile cell pair and the column used at steps of each parity
+    const int row = threadIdx.x / (TX / 2), h = threadIdx.x - row * (TX / 2);
+    const int gy = ty0 + row;
+    const T ipy = sTy[row + 1], imy = sTy[QY + row + 1];
+    const bool wall_s = gy == 0, wall_n = gy == ny - 1;
+
+    // prologue: planes k0-2 and k0-1 of p and beta into LDS; the next two
+    // planes into the two register sets (loads run two plane steps ahead)
+    {
+        T rp[Sh::NP], rb[Sh::NP];
+        fetch_pb(rp, rb, k0 - 2);
+        stash(rp, rb, k0 - 2);
+        fetch_pb(rp, rb, k0 - 1);
+        stash(rp, rb, k0 - 1);
+    }
+    T rpA[Sh::NP], rbA[Sh::NP], rqA[Sh::NQ];
+    T rpB[Sh::NP], rbB[Sh::NP], rqB[Sh::NQ];
+    fetch(rpA, rbA, rqA, k0);
+    // keep the first set's loads ahead of the second's: the wait before each
+    // plane step then covers only the loads of the set it stashes
+    __builtin_amdgcn_sched_barrier(0);
+    fetch(rpB, rbB, rqB, k0 + 1);
+    __syncthreads();
+
+    const T half = (T)0.5;
+    // column of the steps with the parity of k: red (gx + gy + k) odd
+    auto colx = [&](int k) { return 2 * h + ((tx0 + gy + modz(k) + 1) & 1); };
+    ColHist<T> hA, hB;  // steps k0-1, k0+1, ... and k0, k0+2, ...
+    {
+        const int xa = colx(k0 - 1), xb = colx(k0);
+        hA.bf = sB[slotB(k0 - 2) * PS + (row + 2) * PX + xa + 2];
+        hB.bf = sB[slotB(k0 - 1) * PS + (row + 2) * PX + xb + 2];
+        hA.pf = FIRST ? (T)0 : sP[slotP(k0 - 2) * PS + (row + 2) * PX + xa + 2];
+        hB.pf = FIRST ? (T)0 : sP[slotP(k0 - 1) * PS + (row + 2) * PX + xb + 2];
+        hA.bc = hB.bc = hA.r = hB.r = (T)0;  // first read two steps later
+    }
+
+    auto step = [&](int k, T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], ColHist<T>& H) {
+        // planes k+1 (p, beta) and k (q) into LDS, then start loading planes
+        // k+3 and k+2 into the same registers (past the chunk end: unused)
+        stash(rp, rb, k + 1);
+        stash_plane<NT, T, Sh::NQ>(rq, sQ + slotQ(k) * QS, QS);
+        fetch(rp, rb, rq, k + 3);
+        __syncthreads();
+
+        const int kw = modz(k);
+        const T* B0 = sB + slotB(k) * PS;
+        const T* Bp = sB + slotB(k + 1) * PS;
+        const T* Bm = sB + slotB(k - 1) * PS;
+        const T* P0 = sP + slotP(k) * PS;
+        const T* Pp = sP + slotP(k + 1) * PS;
+        const T* Pm = sP + slotP(k - 1) * PS;
+        const T* Q0 = sQ + slotQ(k) * QS;
+        T* R0 = sR + slotR(k) * QS;
+
+        // ---- red cell of plane k at this thread's column
+        const int xr = colx(k);
+        const int ipc = (row + 2) * PX + xr + 2;
+        const int iqc = (row + 1) * QX + xr + 1;
+        const T ipx = sTx[xr + 1], imx = sTx[QX + xr + 1];
+        const T p_self = H.pf;  // p_old(k-1) at this column
+        T red_c, b_c, b_f;
+        {
+            const T azp = sTz[k - k0 + 1], azm = sTz[kSweepMaxChunk + 2 + k - k0 + 1];
+            const T bc = B0[ipc];
+            const T bf = Bp[ipc];
+            const T aR = half * (B0[ipc + 1] + bc) * ipx;
+            const T aL = half * (bc + B0[ipc - 1]) * imx;
+            const T aT = half * (B0[ipc + PX] + bc) * ipy;
+            const T aB = half * (bc + B0[ipc - PX]) * imy;
+            const T aF = half * (bf + bc) * azp;
+            const T aK = half * (bc + H.bf) * azm;
+            const T den = aR + aL + aT + aB + aF + aK;
+            const T qc = Q0[iqc];
+            T num;
+            T pf = (T)0;
+            if (FIRST) {
+                num = -qc;
+            } else {
+                // rows outside the walls hold the wall row itself: the
+                // Neumann ghost of the reference equals the cell's own value
+                pf = Pp[ipc];
+                num = -qc + (aR * P0[ipc + 1] + aL * P0[ipc - 1]) + (aT * P0[ipc + PX] + aB * P0[ipc - PX]) +
+                      (aF * pf + aK * H.pf);
+            }
+            red_c = num / den;
+            R0[iqc] = red_c;
+            b_c = bc;
+            b_f = bf;
+            H.pf = pf;
+        }
+        // ---- red cells of the ring around the tile
+        for (int t = threadIdx.x; t < NRING; t += NT) {
+            int qrow, xx;
+            if (t < QX) {  // rows -1 and TY, every second cell
+                qrow = t < QX / 2 ? 0 : QY - 1;
+                const int hh = t < QX / 2 ? t : t - QX / 2;
+                xx = 2 * hh + ((tx0 + ty0 + qrow - 1 + kw) & 1);
+            } else if (t < QX + TY / 2) {  // column -1, every second row
+                xx = 0;
+                qrow = 1 + 2 * (t - QX) + ((tx0 + ty0 + kw) & 1);
+            } else {  // column TX
+                xx = QX - 1;
+                qrow = 1 + 2 * (t - QX - TY / 2) + ((tx0 + TX + ty0 + kw + 1) & 1);
+            }
+            const int gyr = ty0 + qrow - 1;
+            if (gyr < 0 || gyr >= ny) continue;
+            const int ip = (qrow + 1) * PX + xx + 1;
+            const T bc = B0[ip];
+            const T aR = half * (B0[ip + 1] + bc) * sTx[xx];
+            const T aL = half * (bc + B0[ip - 1]) * sTx[QX + xx];
+            const T aT = half * (B0[ip + PX] + bc) * sTy[qrow];
+            const T aB = half * (bc + B0[ip - PX]) * sTy[QY + qrow];
+            const T aF = half * (Bp[ip] + bc) * sTz[k - k0 + 1];
+            const T aK = half * (bc + Bm[ip]) * sTz[kSweepMaxChunk + 2 + k - k0 + 1];
+            const T den = aR + aL + aT + aB + aF + aK;
+            const T qc = Q0[qrow * QX + xx];
+            T num;
+            if (FIRST) {
+                num = -qc;
+            } else {
+                num = -qc + (aR * P0[ip + 1] + aL * P0[ip - 1]) + (aT * P0[ip + PX] + aB * P0[ip - PX]) +
+                      (aF * Pp[ip] + aK * Pm[ip]);
+            }
+            R0[qrow * QX + xx] = num / den;
+        }
+        __syncthreads();
+
+        // ---- black cell of plane k-1 at the same column; write the pair
+        if (k > k0 && k <= k1) {
+            const int kb = k - 1;
+            const T azp = sTz[kb - k0 + 1], azm = sTz[kSweepMaxChunk + 2 + kb - k0 + 1];
+            const T* B1 = sB + slotB(kb) * PS;
+            const T* Q1 = sQ + slotQ(kb) * QS;
+            const T* R1 = sR + slotR(kb) * QS;
+            const T bc = H.bf;  // beta(k-1) at this column
+            const T aR = half * (B1[ipc + 1] + bc) * ipx;
+            const T aL = half * (bc + B1[ipc - 1]) * imx;
+            const T aT = half * (B1[ipc + PX] + bc) * ipy;
+            const T aB = half * (bc + B1[ipc - PX]) * imy;
+            const T aF = half * (b_c + bc) * azp;  // beta(k)
+            const T aK = half * (bc + H.bc) * azm; // beta(k-2)
+            const T den = aR + aL + aT + aB + aF + aK;
+            const T self = FIRST ? (T)0 : p_self;
+            const T pn = wall_n ? self : R1[iqc + QX];
+            const T ps = wall_s ? self : R1[iqc - QX];
+            const T num = -Q1[iqc] + (aR * R1[iqc + 1] + aL * R1[iqc - 1]) + (aT * pn + aB * ps) +
+                          (aF * red_c + aK * H.r);
+            const T vb = num / den;
+            const int par = xr - 2 * h;
+            const T vr = R1[iqc + 1 - 2 * par];  // the red cell sharing the aligned pair
+            typename Vec2<T>::type v;
+            v.x = par ? vr : vb;
+            v.y = par ? vb : vr;
+            *reinterpret_cast<typename Vec2<T>::type*>(pout + plane_sz * modz(kb) + (size_t)nx * gy + tx0 + 2 * h) = v;
+        }
+        H.bc = b_c;
+        H.bf = b_f;
+        H.r = red_c;
+    };
+
+    // steps k0-1 .. k1 in pairs (a final step k1+1 computes red cells only)
+    for (int k = k0 - 1; k <= k1; k += 2) {
+        step(k, rpA, rbA, rqA, hA);
+        step(k + 1, rpB, rbB, rqB, hB);
+    }
+}
+
+}  // namespace
+
+struct SweepCfg {
+    int tx, ty, nt;
+};
+
+// tile shapes with an instantiated kernel, in order of preference
+constexpr SweepCfg kShapesF[] = {{64, 8, 256}, {64, 16, 512}, {32, 16, 256}, {32, 8, 128}};
+constexpr SweepCfg kShapesD[] = {{64, 4, 128}, {64, 8, 256}, {32, 8, 128}};
+
+template <class T>
+static bool pick_shape(const LevelDev<T>& L, SweepCfg& out)
+{
+    if (L.nz < 2 || L.nz % 2 != 0 || L.ny % 2 != 0) return false;
+    const SweepCfg* tab = sizeof(T) == 4 ? kShapesF : kShapesD;
+    const int n = sizeof(T) == 4 ? (int)(sizeof kShapesF / sizeof kShapesF[0]) : (int)(sizeof kShapesD / sizeof kShapesD[0]);
+    int want_tx = 0, want_ty = 0, want_nt = 0;
+    if (const char* e = std::getenv("TBFG_SWEEP_SHAPE")) std::sscanf(e, "%dx%dx%d", &want_tx, &want_ty, &want_nt);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < n; ++i) {
+            const SweepCfg c = tab[i];
+            if (pass == 0 && !(c.tx == want_tx && c.ty == want_ty && c.nt == want_nt)) continue;
+            if (L.nx % c.tx == 0 && L.ny % c.ty == 0) {
+                out = c;
+                return true;
+            }
+        }
+    return false;
+}
+
+template <class T>
+bool sweep_supported(const LevelDev<T>& L)
+{
+    SweepCfg c;
+    return pick_shape(L, c);
+}
+
+template <class T, class TQ, int TX, int TY, int NT, bool FIRST>
+static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout,
+                              const int* done, void* stream)
+{
+    static int per_cu = -1, ncu = 0;
+    if (per_cu < 0) {
+        int dev = 0;
+        TBFG_HIP(hipGetDevice(&dev));
+        TBFG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        int nb = 0;
+        TBFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, TQ, TX, TY, NT, FIRST>, NT, 0));
+        per_cu = nb < 1 ? 1 : nb;
+    }
+    const int tiles_x = L.nx / TX;
+    const int tiles_y = L.ny / TY;
+    const int ntiles = tiles_x * tiles_y;
+    // one resident wave of workgroups, chunks of at least 4 planes
+    const int target = per_cu * (ncu > 0 ? ncu : 256);
+    int chunks = (target + ntiles - 1) / ntiles;
+    int kc = (L.nz + chunks - 1) / chunks;
+    if (kc < 4) kc = 4;
+    if (kc > kSweepMaxChunk) kc = kSweepMaxChunk;
+    chunks = (L.nz + kc - 1) / kc;
+    const unsigned nblocks = (unsigned)ntiles * (unsigned)chunks;
+    hipLaunchKernelGGL((k_sweep<T, TQ, TX, TY, NT, FIRST>), dim3(nblocks), dim3(NT), 0, (hipStream_t)stream, L, beta,
+                       q, pin, pout, kc, tiles_x, ntiles, done);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class T, class TQ, int TX, int TY, int NT>
+static int launch_sweep_first(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
+                              const int* done, void* stream)
+{
+    return first ? launch_sweep_shape<T, TQ, TX, TY, NT, true>(L, beta, q, pin, pout, done, stream)
+                 : launch_sweep_shape<T, TQ, TX, TY, NT, false>(L, beta, q, pin, pout, done, stream);
+}
+
+template <class T, class TQ>
+int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
+                 const int* done, void* stream)
+{
+    SweepCfg c;
+    if (!pick_shape(L, c)) {
+        set_error("launch_sweep: level %dx%dx%d not supported", L.nx, L.ny, L.nz);
+        return TBFG_EINVAL;
+    }
+#define TBFG_SWEEP_CASE(X, Y, N)                                                             \
+    if (c.tx == X && c.ty == Y && c.nt == N)                                                \
+        return launch_sweep_first<T, TQ, X, Y, N>(L, beta, q, pin, pout, first, done, stream);
+    if constexpr (sizeof(T) == 4) {
+        TBFG_SWEEP_CASE(64, 8, 256)
+        TBFG_SWEEP_CASE(64, 16, 512)
+        TBFG_SWEEP_CASE(32, 16, 256)
+        TBFG_SWEEP_CASE(32, 8, 128)
+    } else {
+        TBFG_SWEEP_CASE(64, 4, 128)
+        TBFG_SWEEP_CASE(64, 8, 256)
+        TBFG_SWEEP_CASE(32, 8, 128)
+    }
+#undef TBFG_SWEEP_CASE
+    set_error("launch_sweep: no kernel for shape %dx%dx%d", c.tx, c.ty, c.nt);
+    return TBFG_EINVAL;
+}
+
+template bool sweep_supported<float>(const LevelDev<float>&);
+template bool sweep_supported<double>(const LevelDev<double>&);
+template int launch_sweep<float, float>(const LevelDev<float>&, const float*, const float*, const float*, float*,
+                                        bool, const int*, void*);
+template int launch_sweep<double, double>(const LevelDev<double>&, const double*, const double*, const double*,
+                                          double*, bool, const int*, void*);
+
+}  // namespace tbfg

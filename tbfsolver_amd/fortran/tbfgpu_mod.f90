@@ -151,14 +151,19 @@ contains
         ierr = tbfg_fps_solve(h, s, slb, sub, p, plb, pub)
     end subroutine
 
-    ! pcgCTOR: parameters with the keys of specs/pcg_solver
-    subroutine tbfg_pcgCTOR(h, grid, nLevels, nPreSweep, nPostSweep, tolPCG, maxIterPCG, mixed, ierr)
+    ! pcgCTOR: parameters with the keys of specs/pcg_solver.  The optional
+    ! flexible selects the direction update: .false. Fletcher-Reeves as
+    ! pcg.f90:241-255 (the FP64 default), .true. Polak-Ribiere, which keeps
+    ! converging where the V-cycle preconditioner (red-black order in both pre-
+    ! and post-smoothing) is not symmetric enough for Fletcher-Reeves.
+    subroutine tbfg_pcgCTOR(h, grid, nLevels, nPreSweep, nPostSweep, tolPCG, maxIterPCG, mixed, ierr, flexible)
         type(c_ptr), intent(out) :: h
         type(tbfg_grid_desc), intent(in) :: grid
         integer, intent(in) :: nLevels, nPreSweep, nPostSweep, maxIterPCG
         real(c_double), intent(in) :: tolPCG
         logical, intent(in) :: mixed
         integer, intent(out) :: ierr
+        logical, intent(in), optional :: flexible
         type(tbfg_pcg_params) :: prm
         call tbfg_pcg_default_params(prm)
         prm%n_levels = nLevels
@@ -167,6 +172,10 @@ contains
         prm%tol = tolPCG
         prm%max_iter = maxIterPCG
         if (mixed) prm%precision = TBFG_PREC_MIXED
+        if (present(flexible)) then
+            prm%flexible = 0
+            if (flexible) prm%flexible = 1
+        end if
         ierr = tbfg_pcg_create(grid, prm, h)
     end subroutine
 

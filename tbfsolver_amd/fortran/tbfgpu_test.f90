@@ -5,6 +5,10 @@
 ! Right-hand side: s = cos(2 pi x / Lx) (uniform in y and z).  The discrete
 ! solution of both operators (FFT operator; 7-point operator with beta = 1) is
 ! s / lambda with lambda = 2 (cos(2 pi / nx) - 1) / dx^2, which is mean free.
+! On this single-mode right-hand side Fletcher-Reeves PCG stagnates (as the
+! reference's does: its V-cycle smooths red-black before and after the coarse
+! correction), so the PCG case uses the Polak-Ribiere update, and the
+! Fletcher-Reeves call checks that stagnation is reported as not converged.
 program tbfgpu_test
     use tbfgpu_mod
     implicit none
@@ -41,7 +45,7 @@ program tbfgpu_test
 
     beta = 1.d0
     psi = 0.d0
-    call tbfg_pcgCTOR(pcg, g, 4, 2, 2, 1.d-12, 100, .false., ierr)
+    call tbfg_pcgCTOR(pcg, g, 4, 2, 2, 1.d-12, 100, .false., ierr, flexible = .true.)
     if (ierr /= TBFG_OK) call fail('pcgCTOR: ' // tbfg_last_error())
     call tbfg_solvePCG(pcg, psi, beta, s, iter_, res_, ierr)
     if (ierr /= TBFG_OK) call fail('solvePCG: ' // tbfg_last_error())
@@ -51,6 +55,14 @@ program tbfgpu_test
     print '(a, i4, a, es10.3, a, es10.3)', 'solvePCG iter_ ', iter_, ' res_ ', res_, ' max error ', err
 
     ierr = tbfg_fps_destroy(fps)
+    ierr = tbfg_pcg_destroy(pcg)
+
+    psi = 0.d0
+    call tbfg_pcgCTOR(pcg, g, 4, 2, 2, 1.d-12, 100, .false., ierr)
+    if (ierr /= TBFG_OK) call fail('pcgCTOR (Fletcher-Reeves): ' // tbfg_last_error())
+    call tbfg_solvePCG(pcg, psi, beta, s, iter_, res_, ierr)
+    if (ierr /= TBFG_ENOTCONVERGED .or. iter_ /= 100) call fail('expected TBFG_ENOTCONVERGED after 100')
+    print '(a, i4, a, es10.3)', 'solvePCG (Fletcher-Reeves) not converged: iter_ ', iter_, ' res_ ', res_
     ierr = tbfg_pcg_destroy(pcg)
 
     ! error path: non-divisible hierarchy is rejected with a status code

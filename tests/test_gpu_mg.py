@@ -97,6 +97,26 @@ def test_preconditioner_matches_oracle(torch_gpu, n, stretch, nlev):
     assert rel(mean_free(out), mean_free(zr)) < 1e-8
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+@pytest.mark.parametrize("n,nlev", [((128, 96, 64), 4), ((96, 32, 40), 3)])
+def test_full_sweep_kernel_matches_half_sweeps(torch_gpu, monkeypatch, precision, tol, n, nlev):
+    """The out-of-place full-sweep kernel (levels with nx % 32 == 0) gives the
+    V-cycle of the two in-place half-sweep launches (TBFG_HALF_SWEEPS forces
+    those at handle creation)."""
+    print("case full sweep vs half sweeps", precision, n, nlev, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 8, 1, 5)
+    zs = []
+    for half in (False, True):
+        if half:
+            monkeypatch.setenv("TBFG_HALF_SWEEPS", "1")
+        s = T.PCG(g, nLevels=nlev, precision=precision)
+        z = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.precondition(to_dev(beta, torch_gpu), to_dev(q, torch_gpu), z)
+        zs.append(to_host(z))
+    assert np.all(np.isfinite(zs[0]))
+    assert rel(zs[0], zs[1]) < tol
+
+
 @pytest.mark.parametrize("n,stretch,nlev,nb", [((64, 64, 64), 0, 4, 18), ((64, 48, 64), 1, 4, 20)])
 def test_pcg_fp64_matches_oracle(torch_gpu, n, stretch, nlev, nb):
     print("case pcg fp64", n, stretch, flush=True)
