@@ -62,6 +62,20 @@ def timed_solves(torch, solver, psi, beta, q, steps, warmup, profile=None):
     return total, np.array(times) * 1e3, iters, prof
 
 
+def copy_peak(torch, nbytes=1 << 30, reps=5):
+    """Device-to-device copy rate (read + write bytes per second) of a plain copy kernel."""
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    return 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+
 def cpu_baseline(g_dims, rho, ux, uy, uz, q_host, beta_host):
     """The oracle (C, OpenMP, FP64, the reference algorithm with an exact
     coarsest solve) on the host cores: one full solve of the same workload."""
@@ -79,8 +93,8 @@ def cpu_baseline(g_dims, rho, ux, uy, uz, q_host, beta_host):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     ap.add_argument("--nref", type=int, default=50)
@@ -115,7 +129,7 @@ def main():
     mixed = T.PCG(g, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=200, precision="mixed",
                   flexible=1, coarse_max_iter=0)
     psi = torch.zeros_like(drho)
-    total, ms, iters, prof = timed_solves(torch, mixed, psi, beta, q, args.steps, args.warmup, profile="rbgs_fine")
+    total, ms, iters, prof = timed_solves(torch, mixed, psi, beta, q, args.steps, args.warmup, profile="sweep_fine")
     psi_mixed = psi.clone()
     value = args.steps / total
     log("mixed: %.2f solves/s, median %.3f ms, iterations %s" % (value, np.median(ms), sorted(set(iters))))
@@ -172,24 +186,35 @@ def main():
         log("fp64: %.2f solves/s, iterations %s; mixed err %.2e (L2) %.2e (max); reference algorithm %.2f/s"
             % (v64, sorted(set(it64)), err_l2, err_max, 2 / totr))
 
-    # roofline of the dominant kernel (finest-level red/black half-sweep)
+    # roofline of the dominant kernel (finest-level red/black sweep)
     launches, kms, bytes_per = prof
     if launches > 0:
         avg_s = kms / launches * 1e-3
         achieved = bytes_per / avg_s / 1e9
         traffic = None
-        tf = os.path.join(ROOT, "profiles", "rbgs_fine_traffic.json")
+        tf = os.path.join(ROOT, "profiles", "sweep_fine_traffic.json")
         if os.path.exists(tf):
             try:
                 traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
             except Exception:
                 traffic = None
+        copy_gbs = copy_peak(torch)
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                           "kernel": "k_rbgs<float,double> finest-level half-sweep",
+                           "kernel": "k_sweep<float> finest-level red/black sweep (out of place)",
                            "avg_us": round(avg_s * 1e6, 2), "launches": launches,
-                           "algorithmic_bytes_per_launch": bytes_per}
-        log("rbgs fine: %d launches, avg %.1f us, %.0f GB/s algorithmic" % (launches, avg_s * 1e6, achieved))
+                           "algorithmic_bytes_per_launch": bytes_per,
+                           "copy_kernel_gbs": round(copy_gbs, 1)}
+        log("sweep fine: %d launches, avg %.1f us, %.0f GB/s algorithmic (device copy: %.0f GB/s)"
+            % (launches, avg_s * 1e6, achieved, copy_gbs))
+        # the whole solve against the bytes a PCG iteration must stream: CG vectors
+        # 128 N plus the multigrid, b N_l (4 (pre + post) + 7.25) summed over levels
+        n0 = float(np.prod(dims))
+        mg = 4.0 * sum(n0 / 8 ** l for l in range(12)) * (4 * (2 + 2) + 7.25)
+        per_it = 128.0 * n0 + mg
+        solve_gbs = out["iterations"] * per_it / (out["p50_ms"] * 1e-3) / 1e9
+        out["solve_roofline"] = {"bytes_per_iteration": per_it, "achieved": round(solve_gbs, 1), "unit": "GB/s",
+                                 "frac": round(solve_gbs / HBM_PEAK_GBS, 4)}
 
     if not args.no_cpu:
         q_host = np.asfortranarray(q.cpu().numpy().T)

@@ -147,12 +147,13 @@ int64_t tbfg_pcg_footprint(const tbfg_pcg* h);
 /* Kernel timing with HIP events recorded on the solve's stream around every
  * launch of one kernel family (no counterpart in the reference, whose only
  * timing is the wall clock of poissonEqn.f90:463-489):
- *   0 red/black half-sweep on the finest level    1 fused residual+restriction, finest
+ *   0 red/black sweep on the finest level         1 fused residual+restriction, finest
+ *     (one full sweep, or its two half-sweeps)
  *   2 operator apply + d.Ad (PCG)                 3 psi/r update + ||r|| (PCG)
  * tbfg_pcg_profile_end returns the launches seen since _begin, their summed
  * duration and the algorithmic bytes one launch must move (each input value
  * read once, each output written once). */
-#define TBFG_KERNEL_RBGS_FINE 0
+#define TBFG_KERNEL_SWEEP_FINE 0
 #define TBFG_KERNEL_RESTRICT_FINE 1
 #define TBFG_KERNEL_SPMV 2
 #define TBFG_KERNEL_UPDATE 3
@@ -184,6 +185,21 @@ int tbfg_beta_dev(const tbfg_grid_desc* grid, double alpha_dt, const double* rho
 /* MG makeVelocityDivFree -> addPressureGrad (momentumEqn.f90:795-869,980-993). */
 int tbfg_project_mg_dev(const tbfg_grid_desc* grid, double alpha_dt, const double* rho, const double* psi,
                         double* ux, double* uy, double* uz, void* stream);
+
+/* FAST_MODE computeSource + computeOldPressDiv (poissonEqn.f90:193-305), nl = 2,
+ * no surface tension: s = rho0/(alpha dt) div u* + div[(1 - rho0 (1/rho)_f) G_f]
+ * with G_f the face gradient of 2 psi1 - psi2 (psi1 = psi^n, psi2 = psi^(n-1);
+ * either may be NULL on the first steps, meaning zero) and (1/rho)_f the face
+ * mean of 1/rho.  rho0 = min(rho_l, rho_g) (poissonEqn.f90:127).  x, z uniform. */
+int tbfg_source_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rho0, const double* rho,
+                         const double* ux, const double* uy, const double* uz, const double* psi1,
+                         const double* psi2, double* s, void* stream);
+
+/* FAST_MODE makeVelocityDivFree + computeOldPressGrad (momentumEqn.f90:999-1127):
+ * u_f -= alpha dt [(1/rho0) d psi + ((1/rho)_f - 1/rho0) G_f], G_f as above. */
+int tbfg_project_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rho0, const double* rho,
+                          const double* psi, const double* psi1, const double* psi2, double* ux, double* uy,
+                          double* uz, void* stream);
 
 /* computeContinuityError (auxiliaryRoutines.f90:42-91): returns max|div u| (not
  * multiplied by dt) in *out.  Blocking. */

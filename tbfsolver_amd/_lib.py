@@ -88,6 +88,10 @@ SIGNATURES = {
     "tbfg_beta_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, _P, _P, _P]),
     "tbfg_project_mg_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, _P, _P, _P, _P, _P, _P]),
     "tbfg_max_divergence_dev": (C.c_int, [C.POINTER(GridDesc), _P, _P, _P, _D, _P]),
+    "tbfg_source_fast_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, _P, _P, _P, _P, _P, _P, _P,
+                                       _P]),
+    "tbfg_project_fast_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, _P, _P, _P, _P, _P, _P, _P,
+                                        _P]),
     "tbfg_make_velocity": (C.c_int, [C.POINTER(GridDesc), C.c_uint64, _D, _D, _D]),
     "tbfg_make_bubble_density": (C.c_int, [C.POINTER(GridDesc), C.c_int32, C.c_double, C.c_uint64,
                                            C.c_int32, C.c_double, C.c_double, _D, _D]),

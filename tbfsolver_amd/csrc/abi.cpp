@@ -322,6 +322,20 @@ int tbfg_project_mg_dev(const tbfg_grid_desc* grid, double alpha_dt, const doubl
     ABI_GUARD(return project_mg_dev(grid, alpha_dt, rho, psi, ux, uy, uz, stream);)
 }
 
+int tbfg_source_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rho0, const double* rho,
+                         const double* ux, const double* uy, const double* uz, const double* psi1,
+                         const double* psi2, double* s, void* stream)
+{
+    ABI_GUARD(return source_fast_dev(grid, alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2, s, stream);)
+}
+
+int tbfg_project_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rho0, const double* rho,
+                          const double* psi, const double* psi1, const double* psi2, double* ux, double* uy,
+                          double* uz, void* stream)
+{
+    ABI_GUARD(return project_fast_dev(grid, alpha_dt, rho0, rho, psi, psi1, psi2, ux, uy, uz, stream);)
+}
+
 int tbfg_max_divergence_dev(const tbfg_grid_desc* grid, const double* ux, const double* uy, const double* uz,
                             double* out, void* stream)
 {

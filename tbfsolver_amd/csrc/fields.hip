@@ -1,6 +1,8 @@
 // fields.hip -- device kernels around the pressure solve: divergence of the
 // staggered velocity (RHS of MG_MODE and the continuity error), the PCG
-// coefficient beta = alpha dt / rho, and the MG_MODE projection.
+// coefficient beta = alpha dt / rho, the MG_MODE projection, and the FAST_MODE
+// source and projection of the constant-coefficient splitting (reference
+// density rho0, extrapolated old pressure 2 psi^n - psi^(n-1)).
 //
 // Staggered layout: ux, uz hold nx*ny*nz faces (face i between cells i and i+1,
 // face 0 == face nx by periodicity); uy holds nx*(ny+1)*nz faces, j = 0..ny,
@@ -146,6 +148,87 @@ __global__ void k_project_mg(FieldGeo G, double alpha_dt, const double* __restri
     }
 }
 
+// old-pressure level, absent on the first steps (then zero)
+__device__ __forceinline__ double old_level(const double* __restrict__ a, size_t i) { return a ? a[i] : 0.0; }
+
+// extrapolated old-pressure difference across a face: (2 (psi1_b - psi1_a) - (psi2_b - psi2_a)) / dc
+__device__ __forceinline__ double old_grad(const double* __restrict__ psi1, const double* __restrict__ psi2, size_t a,
+                                           size_t b, double dc)
+{
+    return (2.0 * (old_level(psi1, b) - old_level(psi1, a)) - (old_level(psi2, b) - old_level(psi2, a))) / dc;
+}
+
+// FAST computeSource + computeOldPressDiv with two old levels (poissonEqn.f90:193-305),
+// without surface tension: s = rho0/(alpha dt) div u* + div[(1 - rho0 (1/rho)_f) G_f],
+// G_f the face gradient of 2 psi^n - psi^(n-1); (1/rho)_f the arithmetic face mean.
+__global__ void k_source_fast(FieldGeo G, double r_adt, double rho0, const double* __restrict__ rho,
+                              const double* __restrict__ ux, const double* __restrict__ uy,
+                              const double* __restrict__ uz, const double* __restrict__ psi1,
+                              const double* __restrict__ psi2, double* __restrict__ s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    if (i >= nx) return;
+    const size_t plane = (size_t)nx * ny;
+    const size_t c = i + (size_t)nx * j + plane * k;
+    const size_t e = (i + 1 < nx ? i + 1 : 0) + (size_t)nx * j + plane * k;
+    const size_t w = (i > 0 ? i - 1 : nx - 1) + (size_t)nx * j + plane * k;
+    const size_t n = i + (size_t)nx * (j + 1 < ny ? j + 1 : j) + plane * k;  // wall ghost = the cell
+    const size_t so = i + (size_t)nx * (j > 0 ? j - 1 : j) + plane * k;
+    const size_t f = i + (size_t)nx * j + plane * (k + 1 < nz ? k + 1 : 0);
+    const size_t b = i + (size_t)nx * j + plane * (k > 0 ? k - 1 : nz - 1);
+    const double dxf = G.df[0][0], dzf = G.df[2][0], dyf = G.df[1][j];
+    const double dxc = G.dcn[0][0], dzc = G.dcn[2][0];
+    const double dycp = G.dcn[1][j], dycm = G.dcn[1][j > 0 ? j - 1 : 0];
+    const double sv = div_cell(G, ux, uy, uz, i, j, k) * r_adt * rho0;
+    const double ic = 1.0 / rho[c];
+    const double apx = 1.0 - rho0 * 0.5 * (1.0 / rho[e] + ic);
+    const double amx = 1.0 - rho0 * 0.5 * (1.0 / rho[w] + ic);
+    const double apy = 1.0 - rho0 * 0.5 * (1.0 / rho[n] + ic);
+    const double amy = 1.0 - rho0 * 0.5 * (1.0 / rho[so] + ic);
+    const double apz = 1.0 - rho0 * 0.5 * (1.0 / rho[f] + ic);
+    const double amz = 1.0 - rho0 * 0.5 * (1.0 / rho[b] + ic);
+    const double dx = (apx * old_grad(psi1, psi2, c, e, dxc) - amx * old_grad(psi1, psi2, w, c, dxc)) / dxf;
+    const double dy = (apy * old_grad(psi1, psi2, c, n, dycp) - amy * old_grad(psi1, psi2, so, c, dycm)) / dyf;
+    const double dz = (apz * old_grad(psi1, psi2, c, f, dzc) - amz * old_grad(psi1, psi2, b, c, dzc)) / dzf;
+    s[c] = sv + dx + dy + dz;
+}
+
+// FAST makeVelocityDivFree + computeOldPressGrad (momentumEqn.f90:999-1127), without
+// surface tension: u_f -= alpha dt [(1/rho0) d psi + ((1/rho)_f - 1/rho0) G_f] on
+// the periodic x/z faces and the interior y faces.
+__global__ void k_project_fast(FieldGeo G, double alpha_dt, double rho0, const double* __restrict__ rho,
+                               const double* __restrict__ psi, const double* __restrict__ psi1,
+                               const double* __restrict__ psi2, double* __restrict__ ux, double* __restrict__ uy,
+                               double* __restrict__ uz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    if (i >= nx) return;
+    const size_t plane = (size_t)nx * ny;
+    const size_t c = i + (size_t)nx * j + plane * k;
+    const size_t e = (i + 1 < nx ? i + 1 : 0) + (size_t)nx * j + plane * k;
+    const size_t f = i + (size_t)nx * j + plane * (k + 1 < nz ? k + 1 : 0);
+    const double dxc = G.dcn[0][0], dzc = G.dcn[2][0];
+    const double inv0 = 1.0 / rho0;
+    const double ic = 1.0 / rho[c];
+    double inv = 0.5 * (1.0 / rho[e] + ic);
+    ux[c] = ux[c] - alpha_dt * (inv0 * ((psi[e] - psi[c]) / dxc) + (inv - inv0) * old_grad(psi1, psi2, c, e, dxc));
+    inv = 0.5 * (1.0 / rho[f] + ic);
+    uz[c] = uz[c] - alpha_dt * (inv0 * ((psi[f] - psi[c]) / dzc) + (inv - inv0) * old_grad(psi1, psi2, c, f, dzc));
+    if (j + 1 < ny) {
+        const size_t n = c + nx;
+        const size_t yf = i + (size_t)nx * ((j + 1) + (size_t)(ny + 1) * k);
+        const double dyc = G.dcn[1][j];
+        inv = 0.5 * (1.0 / rho[n] + ic);
+        uy[yf] = uy[yf] - alpha_dt * (inv0 * ((psi[n] - psi[c]) / dyc) + (inv - inv0) * old_grad(psi1, psi2, c, n, dyc));
+    }
+}
+
 static dim3 row_block(int nx) { return dim3(nx >= 256 ? 256 : (unsigned)next_pow2(nx < 64 ? 64 : nx)); }
 
 int divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, const double* uz, double* div,
@@ -200,6 +283,55 @@ int beta_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, double
     }
     const size_t n = (size_t)g->n[0] * g->n[1] * g->n[2];
     hipLaunchKernelGGL(k_beta, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, n, alpha_dt, rho, beta);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+int source_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, const double* rho, const double* ux,
+                    const double* uy, const double* uz, const double* psi1, const double* psi2, double* s,
+                    void* stream)
+{
+    if (!rho || !ux || !uy || !uz || !s) {
+        set_error("tbfg_source_fast_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    if (!(alpha_dt > 0.0) || !(rho0 > 0.0)) {
+        set_error("tbfg_source_fast_dev: alpha_dt = %g, rho0 = %g (both must be > 0)", alpha_dt, rho0);
+        return TBFG_EINVAL;
+    }
+    FieldGeo G;
+    TBFG_TRY(get_geo(g, G));
+    if (g->stretch[0] || g->stretch[2]) {
+        set_error("tbfg_source_fast_dev: x and z must be uniform (FAST_MODE)");
+        return TBFG_EINVAL;
+    }
+    dim3 blk = row_block(G.nx);
+    hipLaunchKernelGGL(k_source_fast, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G,
+                       1.0 / alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2, s);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+int project_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, const double* rho, const double* psi,
+                     const double* psi1, const double* psi2, double* ux, double* uy, double* uz, void* stream)
+{
+    if (!rho || !psi || !ux || !uy || !uz) {
+        set_error("tbfg_project_fast_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    if (!(rho0 > 0.0)) {
+        set_error("tbfg_project_fast_dev: rho0 = %g (must be > 0)", rho0);
+        return TBFG_EINVAL;
+    }
+    FieldGeo G;
+    TBFG_TRY(get_geo(g, G));
+    if (g->stretch[0] || g->stretch[2]) {
+        set_error("tbfg_project_fast_dev: x and z must be uniform (FAST_MODE)");
+        return TBFG_EINVAL;
+    }
+    dim3 blk = row_block(G.nx);
+    hipLaunchKernelGGL(k_project_fast, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G,
+                       alpha_dt, rho0, rho, psi, psi1, psi2, ux, uy, uz);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }

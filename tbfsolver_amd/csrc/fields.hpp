@@ -10,6 +10,11 @@ int divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, 
 int max_divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, const double* uz, double* out,
                        void* stream);
 int beta_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, double* beta, void* stream);
+int source_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, const double* rho, const double* ux,
+                    const double* uy, const double* uz, const double* psi1, const double* psi2, double* s,
+                    void* stream);
+int project_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, const double* rho, const double* psi,
+                     const double* psi1, const double* psi2, double* ux, double* uy, double* uz, void* stream);
 int project_mg_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, const double* psi, double* ux,
                    double* uy, double* uz, void* stream);
 }  // namespace tbfg

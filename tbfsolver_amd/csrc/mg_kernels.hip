@@ -31,6 +31,27 @@ __device__ __forceinline__ void neighbours(int i, int j, int k, int nx, int ny, 
     b = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * wrap_dn(k, nz));
 }
 
+// Block coordinates of a 3-D block grid launched as a 1-D grid: blocks go
+// round-robin to the 8 XCDs, each with its own L2, so consecutive blocks are
+// renumbered to fall on one XCD -- neighbouring tiles (whose halos overlap)
+// then share that XCD's L2 instead of each fetching the halo from HBM.
+struct Blk3 {
+    int x, y, z;
+};
+
+__device__ __forceinline__ Blk3 xcd_block3(int gx, int gy)
+{
+    const unsigned nb = gridDim.x;
+    unsigned id = blockIdx.x;
+    if ((nb & 7u) == 0) id = (id & 7u) * (nb >> 3) + (id >> 3);
+    Blk3 b;
+    b.x = (int)(id % (unsigned)gx);
+    const unsigned r = id / (unsigned)gx;
+    b.y = (int)(r % (unsigned)gy);
+    b.z = (int)(r / (unsigned)gy);
+    return b;
+}
+
 // face coefficients a_f = 0.5*(beta_c + beta_nb) / (dc*df) (rbgs.f90:220-226)
 template <class T>
 struct Coef {
@@ -78,29 +99,59 @@ __global__ void k_apply_op(LevelDev<double> L, const double* __restrict__ beta, 
     q[c] = apply_cell<double>(a, p, c, e, w, n, s, f, b);
 }
 
-// qv = A d and alpha = delta / (d . qv) (pcg.f90:144-224).  One cell of each
-// of kOpPlanes consecutive planes per thread: the loads of all of them are in
-// flight together (a longer per-thread loop over planes is latency bound).
-constexpr int kOpPlanes = 2;
+// qv = A d and alpha = delta / (d . qv) (pcg.f90:144-224).  A thread owns a
+// column segment of kOpPlanes cells: the column's d and beta are loaded once
+// for the segment and its two neighbours in z (instead of three times), the
+// face factors of its x and y once, so a cell costs the four in-plane
+// neighbours of d and beta plus its share of the column.
+constexpr int kOpPlanes = 8;
 
-__global__ void k_op_dot(LevelDev<double> L, const double* __restrict__ beta, const double* __restrict__ d,
-                         double* __restrict__ qv, PcgState* st, Reducer red)
+__global__ void __launch_bounds__(256) k_op_dot(LevelDev<double> L, const double* __restrict__ beta,
+                                                const double* __restrict__ d, double* __restrict__ qv, PcgState* st,
+                                                Reducer red)
 {
     if (st->done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const Blk3 bk = xcd_block3((nx + blockDim.x - 1) / blockDim.x, (ny + blockDim.y - 1) / blockDim.y);
+    const int i = bk.x * blockDim.x + threadIdx.x;
+    const int j = bk.y * blockDim.y + threadIdx.y;
+    const int k0 = bk.z * kOpPlanes;
     double v[1] = {0.0};
-    if (i < L.nx && j < L.ny) {
+    if (i < nx && j < ny) {
+        const double ipx = L.a[0].ip[i], imx = L.a[0].im[i];
+        const double ipy = L.a[1].ip[j], imy = L.a[1].im[j];
+        const size_t plane = (size_t)nx * ny;
+        const size_t col = (size_t)nx * j + i;
+        // in-plane neighbours (x periodic, y walls: the ghost is the cell itself)
+        const size_t oe = (size_t)nx * j + (i + 1 < nx ? i + 1 : 0);
+        const size_t ow = (size_t)nx * j + (i > 0 ? i - 1 : nx - 1);
+        const size_t on = (size_t)nx * (j + 1 < ny ? j + 1 : j) + i;
+        const size_t os = (size_t)nx * (j > 0 ? j - 1 : j) + i;
+        double dcol[kOpPlanes + 2], bcol[kOpPlanes + 2];
 #pragma unroll
-        for (int u = 0; u < kOpPlanes; ++u) {
-            const int k = blockIdx.z * kOpPlanes + u;
-            if (k < L.nz) {
-                size_t c, e, w, n, s, f, b;
-                neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
-                Coef<double> a = coefs<double>(L, beta, i, j, k, c, e, w, n, s, f, b);
-                const double out = apply_cell<double>(a, d, c, e, w, n, s, f, b);
-                qv[c] = out;
-                v[0] += d[c] * out;
+        for (int u = 0; u < kOpPlanes + 2; ++u) {
+            int k = k0 - 1 + u;
+            k = k < 0 ? k + nz : (k >= nz ? k - nz : k);
+            dcol[u] = d[plane * k + col];
+            bcol[u] = beta[plane * k + col];
+        }
+#pragma unroll
+        for (int u = 1; u <= kOpPlanes; ++u) {
+            const int k = k0 + u - 1;
+            if (k < nz) {
+                const size_t pk = plane * k;
+                const double bc = bcol[u];
+                const double aR = 0.5 * (beta[pk + oe] + bc) * ipx;
+                const double aL = 0.5 * (bc + beta[pk + ow]) * imx;
+                const double aT = 0.5 * (beta[pk + on] + bc) * ipy;
+                const double aB = 0.5 * (bc + beta[pk + os]) * imy;
+                const double aF = 0.5 * (bcol[u + 1] + bc) * L.a[2].ip[k];
+                const double aK = 0.5 * (bc + bcol[u - 1]) * L.a[2].im[k];
+                const double dc = dcol[u];
+                const double out = aR * (d[pk + oe] - dc) - aL * (dc - d[pk + ow]) + aT * (d[pk + on] - dc) -
+                                   aB * (dc - d[pk + os]) + aF * (dcol[u + 1] - dc) - aK * (dc - dcol[u - 1]);
+                qv[pk + col] = out;
+                v[0] += dc * out;
             }
         }
     }
@@ -311,9 +362,10 @@ __global__ void k_rbgs(LevelDev<T> L, const T* __restrict__ beta, const TQ* __re
                        const int* done)
 {
     if (done && *done) return;
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk3 bk = xcd_block3(((L.nx >> 1) + blockDim.x - 1) / blockDim.x, (L.ny + blockDim.y - 1) / blockDim.y);
+    const int h = bk.x * blockDim.x + threadIdx.x;
+    const int j = bk.y * blockDim.y + threadIdx.y;
+    const int k = bk.z;
     if (h >= (L.nx >> 1) || j >= L.ny) return;
     rbgs_cell<T, TQ, COLOUR, FIRST>(L, beta, q, p, h, j, k);
 }
@@ -349,17 +401,100 @@ __device__ __forceinline__ T resid_restrict_cell(const LevelDev<T>& L, const T* 
     return acc;
 }
 
+// The same for a column segment of kRrPlanes coarse cells per thread: the 2x2
+// fine columns under the coarse column are loaded once for the segment (and
+// the fine planes either side), the eight in-plane neighbours of the 2x2 block
+// once per fine plane -- about 7 loads per fine cell instead of 20.
+constexpr int kRrPlanes = 2;
+
+template <class T>
+__device__ __forceinline__ T cell_residual(T qc, T bc, T be, T bw, T bn, T bs, T bf, T bb, T pc, T pe, T pw, T pn,
+                                           T ps, T pf, T pb, T ipx, T imx, T ipy, T imy, T ipz, T imz)
+{
+    // coefficients as coefs(), (A p)_c as apply_cell()
+    const T aR = (T)0.5 * (be + bc) * ipx;
+    const T aL = (T)0.5 * (bc + bw) * imx;
+    const T aT = (T)0.5 * (bn + bc) * ipy;
+    const T aB = (T)0.5 * (bc + bs) * imy;
+    const T aF = (T)0.5 * (bf + bc) * ipz;
+    const T aK = (T)0.5 * (bc + bb) * imz;
+    return qc - (aR * (pe - pc) - aL * (pc - pw) + aT * (pn - pc) - aB * (pc - ps) + aF * (pf - pc) - aK * (pc - pb));
+}
+
 template <class T, class TQ>
 __global__ void k_resid_restrict(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q,
                                  const T* __restrict__ p, T* __restrict__ qc, const int* done)
 {
     if (done && *done) return;
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = blockIdx.y * blockDim.y + threadIdx.y;
-    const int K = blockIdx.z;
-    const int ncx = L.nx >> 1, ncy = L.ny >> 1;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int ncx = nx >> 1, ncy = ny >> 1, ncz = nz >> 1;
+    const Blk3 bk = xcd_block3((ncx + blockDim.x - 1) / blockDim.x, (ncy + blockDim.y - 1) / blockDim.y);
+    const int I = bk.x * blockDim.x + threadIdx.x;
+    const int J = bk.y * blockDim.y + threadIdx.y;
+    const int K0 = bk.z * kRrPlanes;
     if (I >= ncx || J >= ncy) return;
-    qc[I + (size_t)ncx * (J + (size_t)ncy * K)] = resid_restrict_cell<T, TQ>(L, beta, q, p, I, J, K);
+    const int i0 = 2 * I, i1 = i0 + 1, j0 = 2 * J, j1 = j0 + 1;
+    const int iw = i0 > 0 ? i0 - 1 : nx - 1, ie = i1 + 1 < nx ? i1 + 1 : 0;
+    const int js = j0 > 0 ? j0 - 1 : j0, jn = j1 + 1 < ny ? j1 + 1 : j1;  // wall ghost = the cell itself
+    const size_t plane = (size_t)nx * ny;
+    // the 2x2 block (c_ij) and its eight in-plane neighbours
+    const size_t c00 = (size_t)nx * j0 + i0, c10 = c00 + 1, c01 = c00 + nx, c11 = c01 + 1;
+    const size_t w0 = (size_t)nx * j0 + iw, w1 = (size_t)nx * j1 + iw;
+    const size_t e0 = (size_t)nx * j0 + ie, e1 = (size_t)nx * j1 + ie;
+    const size_t s0 = (size_t)nx * js + i0, s1 = (size_t)nx * js + i1;
+    const size_t n0 = (size_t)nx * jn + i0, n1 = (size_t)nx * jn + i1;
+    const T ipx0 = L.a[0].ip[i0], imx0 = L.a[0].im[i0], ipx1 = L.a[0].ip[i1], imx1 = L.a[0].im[i1];
+    const T ipy0 = L.a[1].ip[j0], imy0 = L.a[1].im[j0], ipy1 = L.a[1].ip[j1], imy1 = L.a[1].im[j1];
+    const T wx0 = L.a[0].rlo[I], wx1 = L.a[0].rhi[I], wy0 = L.a[1].rlo[J], wy1 = L.a[1].rhi[J];
+    constexpr int NF = 2 * kRrPlanes + 2;  // fine planes 2K0-1 .. 2K0+2kRrPlanes
+    T P[NF][4], B[NF][4];
+#pragma unroll
+    for (int u = 0; u < NF; ++u) {
+        int k = 2 * K0 - 1 + u;
+        k = k < 0 ? k + nz : (k >= nz ? k - nz : k);
+        const size_t pk = plane * k;
+        P[u][0] = p[pk + c00], P[u][1] = p[pk + c10], P[u][2] = p[pk + c01], P[u][3] = p[pk + c11];
+        B[u][0] = beta[pk + c00], B[u][1] = beta[pk + c10], B[u][2] = beta[pk + c01], B[u][3] = beta[pk + c11];
+    }
+#pragma unroll
+    for (int kk = 0; kk < kRrPlanes; ++kk) {
+        const int K = K0 + kk;
+        if (K >= ncz) break;
+        T acc = 0;
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk) {
+            const int k = 2 * K + dk;
+            const int u = 2 * kk + dk + 1;
+            const size_t pk = plane * k;
+            const T wz = dk ? L.a[2].rhi[K] : L.a[2].rlo[K];
+            const T ipz = L.a[2].ip[k], imz = L.a[2].im[k];
+            const T pw0 = p[pk + w0], pw1 = p[pk + w1], pe0 = p[pk + e0], pe1 = p[pk + e1];
+            const T ps0 = p[pk + s0], ps1 = p[pk + s1], pn0 = p[pk + n0], pn1 = p[pk + n1];
+            const T bw0 = beta[pk + w0], bw1 = beta[pk + w1], be0 = beta[pk + e0], be1 = beta[pk + e1];
+            const T bs0 = beta[pk + s0], bs1 = beta[pk + s1], bn0 = beta[pk + n0], bn1 = beta[pk + n1];
+            const T q00 = (T)q[pk + c00], q10 = (T)q[pk + c10], q01 = (T)q[pk + c01], q11 = (T)q[pk + c11];
+            const T* Pc = P[u];
+            const T* Bc = B[u];
+            const T* Pf = P[u + 1];
+            const T* Bf = B[u + 1];
+            const T* Pb = P[u - 1];
+            const T* Bb = B[u - 1];
+            // children in the reference's order: dj outer, di inner
+            const T r00 = cell_residual<T>(q00, Bc[0], Bc[1], bw0, Bc[2], bs0, Bf[0], Bb[0], Pc[0], Pc[1], pw0, Pc[2],
+                                           ps0, Pf[0], Pb[0], ipx0, imx0, ipy0, imy0, ipz, imz);
+            const T r10 = cell_residual<T>(q10, Bc[1], be0, Bc[0], Bc[3], bs1, Bf[1], Bb[1], Pc[1], pe0, Pc[0], Pc[3],
+                                           ps1, Pf[1], Pb[1], ipx1, imx1, ipy0, imy0, ipz, imz);
+            const T r01 = cell_residual<T>(q01, Bc[2], Bc[3], bw1, bn0, Bc[0], Bf[2], Bb[2], Pc[2], Pc[3], pw1, pn0,
+                                           Pc[0], Pf[2], Pb[2], ipx0, imx0, ipy1, imy1, ipz, imz);
+            const T r11 = cell_residual<T>(q11, Bc[3], be1, Bc[2], bn1, Bc[1], Bf[3], Bb[3], Pc[3], pe1, Pc[2], pn1,
+                                           Pc[1], Pf[3], Pb[3], ipx1, imx1, ipy1, imy1, ipz, imz);
+            acc += (wx0 * wy0 * wz) * r00;
+            acc += (wx1 * wy0 * wz) * r10;
+            acc += (wx0 * wy1 * wz) * r01;
+            acc += (wx1 * wy1 * wz) * r11;
+        }
+        qc[I + (size_t)ncx * (J + (size_t)ncy * K)] = acc;
+    }
 }
 
 // plain restriction of a field (beta, restrictField at multiGrid.f90:210)
@@ -412,9 +547,10 @@ template <class T>
 __global__ void k_prolong_add(LevelDev<T> L, const T* __restrict__ pc, T* __restrict__ p, const int* done)
 {
     if (done && *done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const Blk3 bk = xcd_block3((L.nx + blockDim.x - 1) / blockDim.x, (L.ny + blockDim.y - 1) / blockDim.y);
+    const int i = bk.x * blockDim.x + threadIdx.x;
+    const int j = bk.y * blockDim.y + threadIdx.y;
+    const int k = bk.z;
     if (i >= L.nx || j >= L.ny) return;
     prolong_cell<T>(L, L.nx >> 1, L.ny >> 1, L.nz >> 1, pc, p, i, j, k);
 }
@@ -810,8 +946,9 @@ int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d
 {
     dim3 blk = block2d(L.nx);
     dim3 grd = op_grid(L, blk);
-    hipLaunchKernelGGL(k_op_dot, grd, blk, 0, S(stream), L, beta, d, qv, st, red);
-    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, S(stream), grd.x * grd.y * grd.z, st, red);
+    const unsigned nb = grd.x * grd.y * grd.z;
+    hipLaunchKernelGGL(k_op_dot, dim3(nb), blk, 0, S(stream), L, beta, d, qv, st, red);
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, S(stream), nb, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
@@ -861,7 +998,7 @@ int launch_rbgs(const LevelDev<T>& L, const T* beta, const TQ* q, T* p, int colo
                 void* stream)
 {
     dim3 blk = block2d(L.nx >> 1);
-    dim3 grd(ceil_div(L.nx >> 1, blk.x), ceil_div(L.ny, blk.y), L.nz);
+    const dim3 grd(ceil_div(L.nx >> 1, blk.x) * ceil_div(L.ny, blk.y) * L.nz);
     if (colour == RED) {
         if (first)
             hipLaunchKernelGGL((k_rbgs<T, TQ, RED, true>), grd, blk, 0, S(stream), L, beta, q, p, done);
@@ -882,7 +1019,7 @@ int launch_resid_restrict(const LevelDev<T>& L, const T* beta, const TQ* q, cons
                           void* stream)
 {
     dim3 blk = block2d(L.nx >> 1);
-    dim3 grd(ceil_div(L.nx >> 1, blk.x), ceil_div(L.ny >> 1, blk.y), L.nz >> 1);
+    const dim3 grd(ceil_div(L.nx >> 1, blk.x) * ceil_div(L.ny >> 1, blk.y) * ceil_div(L.nz >> 1, kRrPlanes));
     hipLaunchKernelGGL((k_resid_restrict<T, TQ>), grd, blk, 0, S(stream), L, beta, q, p, qc, done);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
@@ -902,7 +1039,7 @@ template <class T>
 int launch_prolong_add(const LevelDev<T>& L, const T* pc, T* p, const int* done, void* stream)
 {
     dim3 blk = block2d(L.nx);
-    dim3 grd(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), L.nz);
+    const dim3 grd(ceil_div(L.nx, blk.x) * ceil_div(L.ny, blk.y) * L.nz);
     hipLaunchKernelGGL(k_prolong_add<T>, grd, blk, 0, S(stream), L, pc, p, done);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;

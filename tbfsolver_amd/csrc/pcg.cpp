@@ -44,6 +44,9 @@ tbfg_pcg::~tbfg_pcg()
 {
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     if (hst) (void)hipHostFree(hst);
+    if (hring) (void)hipHostFree(hring);
+    for (hipEvent_t e : hev)
+        if (e) (void)hipEventDestroy(e);
     if (hstage) (void)hipHostFree(hstage);
 }
 
@@ -332,6 +335,8 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         TBFG_TRY(build_plan<double>(h.get()));
     TBFG_HIP(hipHostMalloc((void**)&h->hst, sizeof(PcgState), hipHostMallocDefault));
     std::memset(h->hst, 0, sizeof(PcgState));
+    TBFG_HIP(hipHostMalloc((void**)&h->hring, 2 * sizeof(PcgState), hipHostMallocDefault));
+    for (auto& e : h->hev) TBFG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // footprint: every device buffer of the handle
     int64_t fp = 0;
     for (auto& b : h->p) fp += (int64_t)b.bytes;
@@ -465,10 +470,9 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     TBFG_TRY(launch_residual_init(L0, beta, q, psi, r, r32, st, red, stream));
     TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream));
     TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 1, st, red, stream));
-    TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    TBFG_HIP(hipStreamSynchronize(s));
-    for (int init = 1; !h->hst->done; init = 0) {
-        // d = (z - m) + gamma d, then qv = A d and alpha (pcg.f90:266-290, 144-224)
+    // one iteration (pcg.f90:142-290): d = (z - m) + gamma d, qv = A d, alpha,
+    // psi += alpha d, r -= alpha qv, the stop test, z = M r, the dot products
+    auto iteration = [&](int init) -> int {
         TBFG_TRY(launch_pcg_dir<T>(n, z, d, init, st, stream));
         {
             ProfScope ps(h, 2, stream);
@@ -480,10 +484,29 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
         }
         TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream));
         TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 0, st, red, stream));
-        TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-        TBFG_HIP(hipStreamSynchronize(s));
-        if (h->prof_kernel >= 0) prof_harvest(h);
+        return TBFG_OK;
+    };
+    // copy of the device state after the work enqueued so far, and its event
+    auto snapshot = [&](int slot) -> int {
+        TBFG_HIP(hipMemcpyAsync(&h->hring[slot], st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        TBFG_HIP(hipEventRecord(h->hev[slot], s));
+        return TBFG_OK;
+    };
+    // The host examines the state after iteration i while iteration i+1 is
+    // already queued, so the device does not idle through the stop test; the
+    // kernels of an iteration queued after convergence return at once.
+    TBFG_TRY(snapshot(0));
+    TBFG_TRY(iteration(1));
+    TBFG_TRY(snapshot(1));
+    for (int slot = 0;; slot ^= 1) {
+        TBFG_HIP(hipEventSynchronize(h->hev[slot]));
+        if (h->hring[slot].done) break;
+        TBFG_TRY(iteration(0));
+        TBFG_TRY(snapshot(slot));
     }
+    TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    TBFG_HIP(hipStreamSynchronize(s));
+    if (h->prof_kernel >= 0) prof_harvest(h);
     if (iters) *iters = h->hst->iter;
     if (res) *res = h->hst->res;
     if (h->hst->status == TBFG_ENUMERIC) {

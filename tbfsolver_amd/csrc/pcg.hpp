@@ -30,7 +30,9 @@ struct tbfg_pcg {
     tbfg::DevBuf G, dwork, vol;         // dense bottom operator and its set-up scratch
     tbfg::DevBuf cx, cr, cd, cad, cvol; // inner CG of the coarsest-level solve
     tbfg::DevBuf hpsi, hbeta, hq;       // device copies for the host-array entry point
-    tbfg::PcgState* hst = nullptr;      // pinned mirror of the device state
+    tbfg::PcgState* hst = nullptr;      // pinned mirror of the device state (final, after a solve)
+    tbfg::PcgState* hring = nullptr;    // pinned copies of the state after consecutive iterations
+    hipEvent_t hev[2] = {nullptr, nullptr};
     double* hstage = nullptr;           // pinned staging (3 * n0 doubles)
     int64_t footprint = 0;
 

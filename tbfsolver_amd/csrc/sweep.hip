@@ -27,6 +27,7 @@ namespace tbfg {
 namespace {
 
 constexpr int kSweepMaxChunk = 60;  // planes per workgroup (z face factors staged in LDS)
+constexpr int kZT = kSweepMaxChunk + 3;
 
 template <class T, int TX, int TY, int NT>
 struct SweepShape {
@@ -98,7 +99,7 @@ __global__ void __launch_bounds__(NT)
     __shared__ T sR[Sh::SR * QS];
     __shared__ T sTx[2 * QX];        // x face factors of the tile columns -1..TX
     __shared__ T sTy[2 * QY];        // y face factors of the tile rows -1..TY
-    __shared__ T sTz[2 * (kSweepMaxChunk + 2)];  // z face factors of the planes k0-1..k1
+    __shared__ T sTz[2 * kZT];      // z face factors of the planes k0-1..k1+1
 
     if (done && *done) return;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -131,10 +132,10 @@ __global__ void __launch_bounds__(NT)
     }
     // (vector loads of these inside the plane loop would wait behind the
     // prefetch of the next planes)
-    for (int t = threadIdx.x; t < k1 - k0 + 2; t += NT) {
+    for (int t = threadIdx.x; t < k1 - k0 + 3; t += NT) {
         const int kw = wrapi(k0 - 1 + t, nz);
         sTz[t] = L.a[2].ip[kw];
-        sTz[kSweepMaxChunk + 2 + t] = L.a[2].im[kw];
+        sTz[kZT + t] = L.a[2].im[kw];
     }
 
     // in-plane offsets of the cells this thread loads (fixed for the chunk)
@@ -176,7 +177,6 @@ __global__ void __launch_bounds__(NT)
     // this thread's tile cell pair and the column used at steps of each parity
     const int row = threadIdx.x / (TX / 2), h = threadIdx.x - row * (TX / 2);
     const int gy = ty0 + row;
-    const T ipy = sTy[row + 1], imy = sTy[QY + row + 1];
     const bool wall_s = gy == 0, wall_n = gy == ny - 1;
 
     // prologue: planes k0-2 and k0-1 of p and beta into LDS; the next two
@@ -195,7 +195,9 @@ __global__ void __launch_bounds__(NT)
     // plane step then covers only the loads of the set it stashes
     __builtin_amdgcn_sched_barrier(0);
     fetch(rpB, rbB, rqB, k0 + 1);
-    __syncthreads();
+    __syncthreads();  // tables and planes k0-2, k0-1 in LDS
+
+    const T ipy = sTy[row + 1], imy = sTy[QY + row + 1];
 
     const T half = (T)0.5;
     // column of the steps with the parity of k: red (gx + gy + k) odd
@@ -236,7 +238,7 @@ __global__ void __launch_bounds__(NT)
         const T p_self = H.pf;  // p_old(k-1) at this column
         T red_c, b_c, b_f;
         {
-            const T azp = sTz[k - k0 + 1], azm = sTz[kSweepMaxChunk + 2 + k - k0 + 1];
+            const T azp = sTz[k - k0 + 1], azm = sTz[kZT + k - k0 + 1];
             const T bc = B0[ipc];
             const T bf = Bp[ipc];
             const T aR = half * (B0[ipc + 1] + bc) * ipx;
@@ -287,7 +289,7 @@ __global__ void __launch_bounds__(NT)
             const T aT = half * (B0[ip + PX] + bc) * sTy[qrow];
             const T aB = half * (bc + B0[ip - PX]) * sTy[QY + qrow];
             const T aF = half * (Bp[ip] + bc) * sTz[k - k0 + 1];
-            const T aK = half * (bc + Bm[ip]) * sTz[kSweepMaxChunk + 2 + k - k0 + 1];
+            const T aK = half * (bc + Bm[ip]) * sTz[kZT + k - k0 + 1];
             const T den = aR + aL + aT + aB + aF + aK;
             const T qc = Q0[qrow * QX + xx];
             T num;
@@ -304,7 +306,7 @@ __global__ void __launch_bounds__(NT)
         // ---- black cell of plane k-1 at the same column; write the pair
         if (k > k0 && k <= k1) {
             const int kb = k - 1;
-            const T azp = sTz[kb - k0 + 1], azm = sTz[kSweepMaxChunk + 2 + kb - k0 + 1];
+            const T azp = sTz[kb - k0 + 1], azm = sTz[kZT + kb - k0 + 1];
             const T* B1 = sB + slotB(kb) * PS;
             const T* Q1 = sQ + slotQ(kb) * QS;
             const T* R1 = sR + slotR(kb) * QS;

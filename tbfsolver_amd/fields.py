@@ -3,6 +3,8 @@
   computeSource (MG)      poissonEqn.f90:155-190      div(u*) of the staggered velocity
   computeBeta             poissonEqn.f90:311-342      beta = alpha dt / rho
   makeVelocityDivFree     momentumEqn.f90:795-869,980-993
+  computeSourceFast       poissonEqn.f90:193-305      FAST_MODE source (rho0 splitting, old pressure)
+  makeVelocityDivFreeFast momentumEqn.f90:999-1127
   computeContinuityError  auxiliaryRoutines.f90:42-91 max|div u| * dt
 
 and the host-side input mirrors used by the tests and benchmarks (seeded
@@ -32,6 +34,23 @@ def computeBeta(grid, alpha_dt, rho, beta, stream=None):
 def makeVelocityDivFree(grid, alpha_dt, rho, psi, ux, uy, uz, stream=None):
     check(lib().tbfg_project_mg_dev(C.byref(grid), float(alpha_dt), ptr(rho), ptr(psi), ptr(ux), ptr(uy),
                                     ptr(uz), stream_ptr(stream)))
+
+
+def computeSourceFast(grid, alpha_dt, rho0, rho, ux, uy, uz, s, psi1=None, psi2=None, stream=None):
+    """FAST_MODE source with the old-pressure splitting (poissonEqn.f90:193-305);
+    psi1 = psi^n, psi2 = psi^(n-1), None before they exist."""
+    check(lib().tbfg_source_fast_dev(C.byref(grid), float(alpha_dt), float(rho0), ptr(rho), ptr(ux), ptr(uy),
+                                     ptr(uz), ptr(psi1) if psi1 is not None else None,
+                                     ptr(psi2) if psi2 is not None else None, ptr(s), stream_ptr(stream)))
+    return s
+
+
+def makeVelocityDivFreeFast(grid, alpha_dt, rho0, rho, psi, ux, uy, uz, psi1=None, psi2=None, stream=None):
+    """FAST_MODE projection (momentumEqn.f90:999-1127)."""
+    check(lib().tbfg_project_fast_dev(C.byref(grid), float(alpha_dt), float(rho0), ptr(rho), ptr(psi),
+                                      ptr(psi1) if psi1 is not None else None,
+                                      ptr(psi2) if psi2 is not None else None, ptr(ux), ptr(uy), ptr(uz),
+                                      stream_ptr(stream)))
 
 
 def max_divergence(grid, ux, uy, uz, stream=None):

@@ -187,7 +187,7 @@ class PCG:
     def footprint(self):
         return int(lib().tbfg_pcg_footprint(self._h))
 
-    KERNELS = {"rbgs_fine": 0, "restrict_fine": 1, "spmv": 2, "update": 3}
+    KERNELS = {"sweep_fine": 0, "restrict_fine": 1, "spmv": 2, "update": 3}
 
     def coarse_stats(self):
         """Coarsest-level solves of the last solve: dict(calls, iters, iters_max, res_max)."""
