@@ -985,6 +985,13 @@ int launch_pcg_dots(size_t n, const double* r, const TZ* z, const double* qv, in
     return TBFG_OK;
 }
 
+int launch_fin_dots(unsigned nblocks, size_t ncell, int init, PcgState* st, const Reducer& red, void* stream)
+{
+    hipLaunchKernelGGL(k_fin_dots, dim3(1), dim3(1024), 0, S(stream), nblocks, ncell, init, st, red);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
 template <class TZ>
 int launch_pcg_dir(size_t n, const TZ* z, double* d, int init, const PcgState* st, void* stream)
 {

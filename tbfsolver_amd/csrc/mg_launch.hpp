@@ -23,12 +23,22 @@ int launch_pcg_dir(size_t n, const TZ* z, double* d, int init, const PcgState* s
 template <class T, class TQ>
 int launch_rbgs(const LevelDev<T>& L, const T* beta, const TQ* q, T* p, int colour, bool first, const int* done,
                 void* stream);
-// one full red/black sweep, out of place (pin -> pout); levels with nx % 32 == 0
+// one full red/black sweep, out of place (pin -> pout); levels with a tile
+// shape dividing (nx, ny).  With dots: also the partial sums of k_pcg_dots for
+// z = pout (finish with launch_fin_dots over *nblocks_out partials).
+struct SweepDots {
+    const double* r = nullptr;
+    const double* qv = nullptr;
+    int init = 0;
+    Reducer red{nullptr};
+    size_t capacity = 0;  // doubles of red.partials
+};
 template <class T>
 bool sweep_supported(const LevelDev<T>& L);
 template <class T, class TQ>
 int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
-                 const int* done, void* stream);
+                 const int* done, void* stream, const SweepDots* dots = nullptr, unsigned* nblocks_out = nullptr);
+int launch_fin_dots(unsigned nblocks, size_t ncell, int init, PcgState* st, const Reducer& red, void* stream);
 template <class T, class TQ>
 int launch_resid_restrict(const LevelDev<T>& L, const T* beta, const TQ* q, const T* p, T* qc, const int* done,
                           void* stream);

@@ -287,6 +287,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         }
     }
     if (h->mixed) TBFG_TRY(h->r32.alloc(h->n0 * 4));
+    h->fuse_dots = !std::getenv("TBFG_UNFUSED_DOTS");
     TBFG_TRY(h->r.alloc(h->n0 * 8));
     TBFG_TRY(h->d.alloc(h->n0 * 8));
     TBFG_TRY(h->qv.alloc(h->n0 * 8));
@@ -377,16 +378,20 @@ static int setup_levels(tbfg_pcg* h, const double* beta0, void* stream)
 // nsweeps red/black sweeps on level l (rbgs.f90:205-259); `first`: the level
 // starts from p = 0.  Levels with the full-sweep kernel alternate between
 // their two buffers, so p (and alt) may be swapped on return.
+// last_dots: the dot products of the PCG are formed by the last sweep (see
+// launch_sweep); *nb_out receives its partial count.
 template <class T>
 static int smooth(tbfg_pcg* h, int l, const T* b, const T* q, T*& p, T*& alt, int nsweeps, bool first,
-                  const int* done, void* stream)
+                  const int* done, void* stream, const SweepDots* last_dots = nullptr, unsigned* nb_out = nullptr)
 {
     const LevelDev<T>& L = geo<T>(h, l);
     for (int s = 0; s < nsweeps; ++s) {
         const bool f0 = first && s == 0;
         if (h->fused[l]) {
             ProfScope ps(h, (l == 0 && !f0) ? 0 : -1, stream);
-            TBFG_TRY(launch_sweep<T, T>(L, b, q, p, alt, f0, done, stream));
+            const bool last = s == nsweeps - 1;
+            TBFG_TRY(launch_sweep<T, T>(L, b, q, p, alt, f0, done, stream, last ? last_dots : nullptr,
+                                        last ? nb_out : nullptr));
             std::swap(p, alt);
         } else {
             ProfScope ps(h, l == 0 ? 0 : -1, stream);
@@ -399,10 +404,23 @@ static int smooth(tbfg_pcg* h, int l, const T* b, const T* q, T*& p, T*& alt, in
 
 // One preconditioner application z = M r (multiGrid.f90:165-232, fixed cycles).
 // r0 is the finest right-hand side in the multigrid precision; on return *z
-// points at the level-0 buffer that holds the result.
+// points at the level-0 buffer that holds the result.  dots_init >= 0: the
+// finest level's last sweep also forms the PCG dot products of z (k_pcg_dots
+// with that init flag, finalized here); *dots_done tells whether it could.
 template <class T>
-static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* done, void* stream)
+static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* done, void* stream,
+                  int dots_init = -1, bool* dots_done = nullptr)
 {
+    if (dots_done) *dots_done = false;
+    const bool want_dots = dots_init >= 0 && h->fused[0] && h->prm.n_post > 0 && h->t0 > 0;
+    SweepDots sd;
+    if (want_dots) {
+        sd.r = h->r.as<double>();
+        sd.qv = h->qv.as<double>();
+        sd.init = dots_init;
+        sd.red = Reducer{h->partials.as<double>()};
+        sd.capacity = h->partials.bytes / sizeof(double);
+    }
     const int t0 = h->t0;
     T* cur[kMaxLevels];
     T* alt[kMaxLevels];
@@ -429,7 +447,14 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
             const T* q = (l == 0) ? r0 : h->q[l].as<T>();
             const T* pc = (l + 1 < t0) ? cur[l + 1] : h->p[l + 1].as<T>();
             TBFG_TRY(launch_prolong_add<T>(geo<T>(h, l), pc, cur[l], done, stream));
-            TBFG_TRY(smooth<T>(h, l, b, q, cur[l], alt[l], h->prm.n_post, false, done, stream));
+            const bool fuse = want_dots && l == 0 && cyc == h->prm.mg_cycles - 1;
+            unsigned nb = 0;
+            TBFG_TRY(smooth<T>(h, l, b, q, cur[l], alt[l], h->prm.n_post, false, done, stream, fuse ? &sd : nullptr,
+                               fuse ? &nb : nullptr));
+            if (fuse) {
+                TBFG_TRY(launch_fin_dots(nb, h->n0, dots_init, h->st.as<PcgState>(), sd.red, stream));
+                if (dots_done) *dots_done = true;
+            }
         }
     }
     *z = cur[0];
@@ -468,8 +493,9 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     else
         r0 = r;
     TBFG_TRY(launch_residual_init(L0, beta, q, psi, r, r32, st, red, stream));
-    TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream));
-    TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 1, st, red, stream));
+    bool dots_done = false;
+    TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream, h->fuse_dots ? 1 : -1, &dots_done));
+    if (!dots_done) TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 1, st, red, stream));
     // one iteration (pcg.f90:142-290): d = (z - m) + gamma d, qv = A d, alpha,
     // psi += alpha d, r -= alpha qv, the stop test, z = M r, the dot products
     auto iteration = [&](int init) -> int {
@@ -482,8 +508,8 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
             ProfScope ps(h, 3, stream);
             TBFG_TRY(launch_pcg_update(n, psi, r, r32, d, qv, st, red, stream));
         }
-        TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream));
-        TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 0, st, red, stream));
+        TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream, h->fuse_dots ? 0 : -1, &dots_done));
+        if (!dots_done) TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 0, st, red, stream));
         return TBFG_OK;
     };
     // copy of the device state after the work enqueued so far, and its event

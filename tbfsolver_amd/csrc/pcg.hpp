@@ -23,6 +23,7 @@ struct tbfg_pcg {
     std::vector<tbfg::DevBuf> p, q, beta;
     std::vector<tbfg::DevBuf> palt;     // second p buffer of the levels smoothed out of place
     std::vector<char> fused;            // level smoothed by the out-of-place full-sweep kernel
+    bool fuse_dots = true;              // PCG dot products formed by the finest level's last sweep
     tbfg::DevBuf r, d, qv;              // CG vectors (FP64)
     tbfg::DevBuf r32;                   // FP32 copy of r (mixed precision: the multigrid's right-hand side)
     tbfg::DevBuf st, partials, counter; // PcgState + reduction scratch

@@ -18,6 +18,7 @@
 // stream every array twice.
 #include "mg_kernels.hpp"
 #include "mg_launch.hpp"
+#include "reduce.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -83,10 +84,15 @@ struct ColHist {
 // between the two columns of the pair from step to step; the beta, p_old and
 // red values of that column from two steps earlier stay in registers.  The
 // red cells of the ring around the tile are a second, smaller round.
-template <class T, class TQ, int TX, int TY, int NT, bool FIRST>
+// DOTS (the finest level's last sweep of a preconditioner application): the
+// sums of k_pcg_dots -- r.z, qv.z, sum z, sum r, sum qv of the new values z --
+// accumulate while the plane is written, one partial per workgroup, instead of
+// a separate pass over r, z and qv.  r and qv of a plane are loaded with the
+// plane registers, two steps ahead.
+template <class T, class TQ, int TX, int TY, int NT, bool FIRST, bool DOTS>
 __global__ void __launch_bounds__(NT)
     k_sweep(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q, const T* __restrict__ pin,
-            T* __restrict__ pout, int kc, int tiles_x, int ntiles, const int* done)
+            T* __restrict__ pout, int kc, int tiles_x, int ntiles, const int* done, SweepDots dots)
 {
     using Sh = SweepShape<T, TX, TY, NT>;
     static_assert(Sh::NBLACK == NT, "one tile cell pair per thread");
@@ -109,7 +115,11 @@ __global__ void __launch_bounds__(NT)
     const int tile = (int)(id % (unsigned)ntiles);
     const int chunk = (int)(id / (unsigned)ntiles);
     const int k0 = chunk * kc;
-    if (k0 >= nz) return;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // DOTS
+    if (k0 >= nz) {  // (the launch covers nz exactly; an empty chunk adds nothing)
+        if (DOTS) block_partial<5>(acc, dots.red);
+        return;
+    }
     const int k1 = min(nz, k0 + kc);
     const int tx0 = (tile % tiles_x) * TX;
     const int ty0 = (tile / tiles_x) * TY;
@@ -153,6 +163,12 @@ __global__ void __launch_bounds__(NT)
         offq[m] = nx * clampi(ty0 + yy - 1, ny) + wrapi(tx0 + xx - 1, nx);
     }
     const size_t plane_sz = (size_t)nx * ny;
+    // this thread's tile cell pair and the column used at steps of each parity
+    const int row = threadIdx.x / (TX / 2), h = threadIdx.x - row * (TX / 2);
+    const int gy = ty0 + row;
+    const bool wall_s = gy == 0, wall_n = gy == ny - 1;
+    const size_t pair_off = (size_t)nx * gy + tx0 + 2 * h;
+
     // planes of p and beta at kp, of q at kp - 1, into registers
     auto fetch_pb = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], int kp) {
         const T* pp = pin + plane_sz * modz(kp);
@@ -163,21 +179,23 @@ __global__ void __launch_bounds__(NT)
             rb[m] = bp[offp[m]];
         }
     };
-    auto fetch = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], int kp) {
+    // ... and, for DOTS, r and qv of this thread's pair on plane kp - 2 (the
+    // black plane of the step that stashes these registers)
+    auto fetch = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], double2& dr, double2& dq, int kp) {
         fetch_pb(rp, rb, kp);
         const TQ* qp = q + plane_sz * modz(kp - 1);
 #pragma unroll
         for (int m = 0; m < Sh::NQ; ++m) rq[m] = (T)qp[offq[m]];
+        if (DOTS) {
+            const size_t o = plane_sz * modz(kp - 2) + pair_off;
+            dr = *reinterpret_cast<const double2*>(dots.r + o);
+            if (!dots.init) dq = *reinterpret_cast<const double2*>(dots.qv + o);
+        }
     };
     auto stash = [&](const T(&rp)[Sh::NP], const T(&rb)[Sh::NP], int k) {
         if (!FIRST) stash_plane<NT, T, Sh::NP>(rp, sP + slotP(k) * PS, PS);
         stash_plane<NT, T, Sh::NP>(rb, sB + slotB(k) * PS, PS);
     };
-
-    // this thread's tile cell pair and the column used at steps of each parity
-    const int row = threadIdx.x / (TX / 2), h = threadIdx.x - row * (TX / 2);
-    const int gy = ty0 + row;
-    const bool wall_s = gy == 0, wall_n = gy == ny - 1;
 
     // prologue: planes k0-2 and k0-1 of p and beta into LDS; the next two
     // planes into the two register sets (loads run two plane steps ahead)
@@ -190,11 +208,12 @@ __global__ void __launch_bounds__(NT)
     }
     T rpA[Sh::NP], rbA[Sh::NP], rqA[Sh::NQ];
     T rpB[Sh::NP], rbB[Sh::NP], rqB[Sh::NQ];
-    fetch(rpA, rbA, rqA, k0);
+    double2 drA = {0.0, 0.0}, dqA = {0.0, 0.0}, drB = {0.0, 0.0}, dqB = {0.0, 0.0};
+    fetch(rpA, rbA, rqA, drA, dqA, k0);
     // keep the first set's loads ahead of the second's: the wait before each
     // plane step then covers only the loads of the set it stashes
     __builtin_amdgcn_sched_barrier(0);
-    fetch(rpB, rbB, rqB, k0 + 1);
+    fetch(rpB, rbB, rqB, drB, dqB, k0 + 1);
     __syncthreads();  // tables and planes k0-2, k0-1 in LDS
 
     const T ipy = sTy[row + 1], imy = sTy[QY + row + 1];
@@ -212,12 +231,14 @@ __global__ void __launch_bounds__(NT)
         hA.bc = hB.bc = hA.r = hB.r = (T)0;  // first read two steps later
     }
 
-    auto step = [&](int k, T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], ColHist<T>& H) {
+    auto step = [&](int k, T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], double2& dr, double2& dq,
+                    ColHist<T>& H) {
         // planes k+1 (p, beta) and k (q) into LDS, then start loading planes
         // k+3 and k+2 into the same registers (past the chunk end: unused)
         stash(rp, rb, k + 1);
         stash_plane<NT, T, Sh::NQ>(rq, sQ + slotQ(k) * QS, QS);
-        fetch(rp, rb, rq, k + 3);
+        const double2 r_pair = dr, qv_pair = dq;  // plane k-1
+        fetch(rp, rb, rq, dr, dq, k + 3);
         __syncthreads();
 
         const int kw = modz(k);
@@ -329,7 +350,24 @@ __global__ void __launch_bounds__(NT)
             typename Vec2<T>::type v;
             v.x = par ? vr : vb;
             v.y = par ? vb : vr;
-            *reinterpret_cast<typename Vec2<T>::type*>(pout + plane_sz * modz(kb) + (size_t)nx * gy + tx0 + 2 * h) = v;
+            *reinterpret_cast<typename Vec2<T>::type*>(pout + plane_sz * modz(kb) + pair_off) = v;
+            if (DOTS) {  // the order of k_pcg_dots, cell by cell
+                const double z0 = (double)v.x, z1 = (double)v.y;
+                acc[0] += r_pair.x * z0;
+                acc[2] += z0;
+                acc[3] += r_pair.x;
+                if (!dots.init) {
+                    acc[1] += qv_pair.x * z0;
+                    acc[4] += qv_pair.x;
+                }
+                acc[0] += r_pair.y * z1;
+                acc[2] += z1;
+                acc[3] += r_pair.y;
+                if (!dots.init) {
+                    acc[1] += qv_pair.y * z1;
+                    acc[4] += qv_pair.y;
+                }
+            }
         }
         H.bc = b_c;
         H.bf = b_f;
@@ -338,9 +376,10 @@ __global__ void __launch_bounds__(NT)
 
     // steps k0-1 .. k1 in pairs (a final step k1+1 computes red cells only)
     for (int k = k0 - 1; k <= k1; k += 2) {
-        step(k, rpA, rbA, rqA, hA);
-        step(k + 1, rpB, rbB, rqB, hB);
+        step(k, rpA, rbA, rqA, drA, dqA, hA);
+        step(k + 1, rpB, rbB, rqB, drB, dqB, hB);
     }
+    if (DOTS) block_partial<5>(acc, dots.red);
 }
 
 }  // namespace
@@ -380,9 +419,9 @@ bool sweep_supported(const LevelDev<T>& L)
     return pick_shape(L, c);
 }
 
-template <class T, class TQ, int TX, int TY, int NT, bool FIRST>
+template <class T, class TQ, int TX, int TY, int NT, bool FIRST, bool DOTS>
 static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout,
-                              const int* done, void* stream)
+                              const int* done, const SweepDots& dots, unsigned* nblocks_out, void* stream)
 {
     static int per_cu = -1, ncu = 0;
     if (per_cu < 0) {
@@ -390,7 +429,7 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
         TBFG_HIP(hipGetDevice(&dev));
         TBFG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         int nb = 0;
-        TBFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, TQ, TX, TY, NT, FIRST>, NT, 0));
+        TBFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS>, NT, 0));
         per_cu = nb < 1 ? 1 : nb;
     }
     const int tiles_x = L.nx / TX;
@@ -404,24 +443,39 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
     if (kc > kSweepMaxChunk) kc = kSweepMaxChunk;
     chunks = (L.nz + kc - 1) / kc;
     const unsigned nblocks = (unsigned)ntiles * (unsigned)chunks;
-    hipLaunchKernelGGL((k_sweep<T, TQ, TX, TY, NT, FIRST>), dim3(nblocks), dim3(NT), 0, (hipStream_t)stream, L, beta,
-                       q, pin, pout, kc, tiles_x, ntiles, done);
+    if (DOTS && (size_t)nblocks * 5 > dots.capacity) {
+        set_error("launch_sweep: %u workgroups exceed the reduction scratch", nblocks);
+        return TBFG_EINVAL;
+    }
+    hipLaunchKernelGGL((k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS>), dim3(nblocks), dim3(NT), 0, (hipStream_t)stream, L,
+                       beta, q, pin, pout, kc, tiles_x, ntiles, done, dots);
     TBFG_HIP(hipGetLastError());
+    if (nblocks_out) *nblocks_out = nblocks;
     return TBFG_OK;
 }
 
 template <class T, class TQ, int TX, int TY, int NT>
 static int launch_sweep_first(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
-                              const int* done, void* stream)
+                              const int* done, const SweepDots* dots, unsigned* nblocks_out, void* stream)
 {
-    return first ? launch_sweep_shape<T, TQ, TX, TY, NT, true>(L, beta, q, pin, pout, done, stream)
-                 : launch_sweep_shape<T, TQ, TX, TY, NT, false>(L, beta, q, pin, pout, done, stream);
+    const SweepDots none{};
+    if (dots)  // the dot products follow a post-smoothing sweep, never the first
+        return launch_sweep_shape<T, TQ, TX, TY, NT, false, true>(L, beta, q, pin, pout, done, *dots, nblocks_out,
+                                                                  stream);
+    return first ? launch_sweep_shape<T, TQ, TX, TY, NT, true, false>(L, beta, q, pin, pout, done, none,
+                                                                      nblocks_out, stream)
+                 : launch_sweep_shape<T, TQ, TX, TY, NT, false, false>(L, beta, q, pin, pout, done, none,
+                                                                       nblocks_out, stream);
 }
 
 template <class T, class TQ>
 int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
-                 const int* done, void* stream)
+                 const int* done, void* stream, const SweepDots* dots, unsigned* nblocks_out)
 {
+    if (dots && first) {
+        set_error("launch_sweep: the fused dot products need a sweep from a non-zero start");
+        return TBFG_EINVAL;
+    }
     SweepCfg c;
     if (!pick_shape(L, c)) {
         set_error("launch_sweep: level %dx%dx%d not supported", L.nx, L.ny, L.nz);
@@ -429,7 +483,7 @@ int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin,
     }
 #define TBFG_SWEEP_CASE(X, Y, N)                                                             \
     if (c.tx == X && c.ty == Y && c.nt == N)                                                \
-        return launch_sweep_first<T, TQ, X, Y, N>(L, beta, q, pin, pout, first, done, stream);
+        return launch_sweep_first<T, TQ, X, Y, N>(L, beta, q, pin, pout, first, done, dots, nblocks_out, stream);
     if constexpr (sizeof(T) == 4) {
         TBFG_SWEEP_CASE(64, 8, 256)
         TBFG_SWEEP_CASE(64, 16, 512)
@@ -448,8 +502,8 @@ int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin,
 template bool sweep_supported<float>(const LevelDev<float>&);
 template bool sweep_supported<double>(const LevelDev<double>&);
 template int launch_sweep<float, float>(const LevelDev<float>&, const float*, const float*, const float*, float*,
-                                        bool, const int*, void*);
+                                        bool, const int*, void*, const SweepDots*, unsigned*);
 template int launch_sweep<double, double>(const LevelDev<double>&, const double*, const double*, const double*,
-                                          double*, bool, const int*, void*);
+                                          double*, bool, const int*, void*, const SweepDots*, unsigned*);
 
 }  // namespace tbfg

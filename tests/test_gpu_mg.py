@@ -117,6 +117,26 @@ def test_full_sweep_kernel_matches_half_sweeps(torch_gpu, monkeypatch, precision
     assert rel(zs[0], zs[1]) < tol
 
 
+@pytest.mark.parametrize("precision,bound", [("fp64", 1e-10), ("mixed", 1e-9)])
+def test_dots_in_last_sweep_match_separate_kernel(torch_gpu, monkeypatch, precision, bound):
+    """The PCG dot products formed by the finest level's last post-smoothing sweep
+    give the solve of the separate dot-product kernel (TBFG_UNFUSED_DOTS)."""
+    n = (128, 96, 64)
+    print("case fused dots", precision, n, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 12, 1, 31)
+    out = []
+    for unfused in (False, True):
+        if unfused:
+            monkeypatch.setenv("TBFG_UNFUSED_DOTS", "1")
+        s = T.PCG(g, nLevels=4, tolPCG=1e-10, maxIterPCG=200, precision=precision, flexible=1)
+        psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+        assert s.converged
+        out.append((s.iterations, mean_free(to_host(psi))))
+    assert abs(out[0][0] - out[1][0]) <= 1
+    assert rel(out[0][1], out[1][1]) < bound
+
+
 @pytest.mark.parametrize("n,stretch,nlev,nb", [((64, 64, 64), 0, 4, 18), ((64, 48, 64), 1, 4, 20)])
 def test_pcg_fp64_matches_oracle(torch_gpu, n, stretch, nlev, nb):
     print("case pcg fp64", n, stretch, flush=True)
@@ -231,3 +251,27 @@ def test_projection_step_divergence_free(torch_gpu):
     F.makeVelocityDivFree(g, adt, drho, psi, dux, duy, duz)
     md = F.max_divergence(g, dux, duy, duz)
     assert md <= 2e-12 * np.linalg.norm(q), (md, np.linalg.norm(q))   # |r|_max <= ||r||_2 <= tol ||q||_2
+
+
+def test_config1_bubbles_128(torch_gpu):
+    """Config 1: 128^3 channel (Lx=2pi, Ly=2, Lz=pi, uniform), 18 bubbles D=0.3
+    (seed 1001, nref 10 for the sub-sampling), rho_g/rho_l = 1/20, beta = alpha dt / rho,
+    q = div u of the seeded velocity; 4 levels, 2+2 sweeps.  FP64 to 1e-12 and mixed to
+    1e-8 against the oracle solved to 1e-12."""
+    n = (128, 128, 128)
+    print("case config1", n, flush=True)
+    g, go = grids(n, 0)
+    rho, _ = F.make_bubble_density(g, 18, 0.3, 1001, nref=10)
+    ux, uy, uz = F.make_velocity(g, 1001)
+    q = O.divergence(go, ux, uy, uz)
+    beta = O.beta_from_rho(go, 1e-3, rho)
+    ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=4, tol=1e-12, max_iter=200, coarse_mode=0, flexible=1),
+                                   beta, q)
+    assert rc == 0
+    for precision, tol, bound in (("fp64", 1e-12, 1e-9), ("mixed", 1e-8, 1e-6)):
+        s = T.PCG(g, nLevels=4, tolPCG=tol, maxIterPCG=200, precision=precision, flexible=1)
+        psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+        print("  ", precision, "iterations", s.iterations, "oracle", it, flush=True)
+        assert s.converged
+        assert rel(mean_free(to_host(psi)), mean_free(ref)) < bound
