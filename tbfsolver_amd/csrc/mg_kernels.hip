@@ -8,6 +8,8 @@
 #include "reduce.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 namespace tbfg {
 
@@ -104,9 +106,8 @@ __global__ void k_apply_op(LevelDev<double> L, const double* __restrict__ beta, 
 // for the segment and its two neighbours in z (instead of three times), the
 // face factors of its x and y once, so a cell costs the four in-plane
 // neighbours of d and beta plus its share of the column.
-constexpr int kOpPlanes = 8;
-
-__global__ void __launch_bounds__(256) k_op_dot(LevelDev<double> L, const double* __restrict__ beta,
+template <int kOpPlanes>
+__global__ void __launch_bounds__(512) k_op_dot(LevelDev<double> L, const double* __restrict__ beta,
                                                 const double* __restrict__ d, double* __restrict__ qv, PcgState* st,
                                                 Reducer red)
 {
@@ -810,6 +811,79 @@ __global__ void __launch_bounds__(1024) k_tail(const TailPlan<T>* __restrict__ p
     tail_vcycle<T>(P, 0, P.user_last, true);
 }
 
+// Entries of a level's table block (LevelTables::build: ip, im, plo, phi of n
+// and rlo, rhi of the next level's n per axis).
+template <class T>
+__device__ __forceinline__ int tail_table_len(const TailPlan<T>& P, int m)
+{
+    const LevelDev<T>& g = P.lv[m].geo;
+    const bool c = m + 1 < P.nlev;
+    const LevelDev<T>& h = P.lv[c ? m + 1 : m].geo;
+    return 4 * (g.nx + g.ny + g.nz) + (c ? 2 * (h.nx + h.ny + h.nz) : 0);
+}
+
+// The same V-cycle with p, q, beta and the metric tables of every tail level
+// held in LDS (dynamic shared memory sized by the host): every phase then
+// costs LDS latency instead of an L2 round trip.  beta, the tables and q of the
+// first level come in at the start, p of the first level goes back to HBM at
+// the end (the next level up prolongs from it).
+template <class T>
+__global__ void __launch_bounds__(1024) k_tail_lds(const TailPlan<T>* __restrict__ plan, const int* done)
+{
+    extern __shared__ __align__(16) unsigned char tail_smem[];
+    __shared__ TailPlan<T> Ps;
+    if (done && *done) return;
+    {
+        static_assert(sizeof(TailPlan<T>) % 4 == 0, "plan copied as 32-bit words");
+        const unsigned* src = reinterpret_cast<const unsigned*>(plan);
+        unsigned* dst = reinterpret_cast<unsigned*>(&Ps);
+        for (unsigned t = threadIdx.x; t < sizeof(TailPlan<T>) / 4; t += blockDim.x) dst[t] = src[t];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T* base = reinterpret_cast<T*>(tail_smem);
+        for (int m = 0; m < Ps.nlev; ++m) {
+            const int n = Ps.lv[m].geo.nx * Ps.lv[m].geo.ny * Ps.lv[m].geo.nz;
+            Ps.lv[m].p = base;
+            Ps.lv[m].q = base + n;
+            Ps.lv[m].beta = base + 2 * n;
+            base += 3 * n;
+            // the table block, every pointer moved by the same offset
+            const T* g0 = Ps.lv[m].geo.a[0].ip;
+            for (int a = 0; a < 3; ++a) {
+                AxisDev<T>& ax = Ps.lv[m].geo.a[a];
+                ax.ip = base + (ax.ip - g0);
+                ax.im = base + (ax.im - g0);
+                ax.rlo = base + (ax.rlo - g0);
+                ax.rhi = base + (ax.rhi - g0);
+                ax.plo = base + (ax.plo - g0);
+                ax.phi = base + (ax.phi - g0);
+            }
+            base += (tail_table_len(Ps, m) + 3) & ~3;
+        }
+    }
+    __syncthreads();
+    for (int m = 0; m < Ps.nlev; ++m) {
+        const int n = Ps.lv[m].geo.nx * Ps.lv[m].geo.ny * Ps.lv[m].geo.nz;
+        T* b = const_cast<T*>(Ps.lv[m].beta);
+        const T* gb = plan->lv[m].beta;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) b[t] = gb[t];
+        T* tb = const_cast<T*>(Ps.lv[m].geo.a[0].ip);
+        const T* gt = plan->lv[m].geo.a[0].ip;
+        const int nt = tail_table_len(Ps, m);
+        for (int t = threadIdx.x; t < nt; t += blockDim.x) tb[t] = gt[t];
+        if (m == 0) {
+            const T* gq = plan->lv[0].q;
+            for (int t = threadIdx.x; t < n; t += blockDim.x) Ps.lv[0].q[t] = gq[t];
+        }
+    }
+    __syncthreads();
+    tail_vcycle<T>(Ps, 0, Ps.user_last, true);
+    const int n0 = Ps.lv[0].geo.nx * Ps.lv[0].geo.ny * Ps.lv[0].geo.nz;
+    T* gp = plan->lv[0].p;
+    for (int t = threadIdx.x; t < n0; t += blockDim.x) gp[t] = Ps.lv[0].p[t];
+}
+
 // Dense bottom operator: G = (M + c 1 1^T)^-1 (I - 1 1^T / n) (-V), with
 // M = -V A the V-scaled symmetric matrix of poissMat.f90:746-763.  x = G q is the
 // Lagrange-constrained exact solve (poissMat.f90:246-259,467-503, sum x = 0).
@@ -926,6 +1000,17 @@ static unsigned red_blocks1(size_t n)
     return b < kRedBlocks ? b : kRedBlocks;
 }
 
+// operator blocks: whole rows (up to 256 cells, so the x neighbours of a row
+// come from the same block), two rows, kOpPlanes planes per thread
+constexpr int kOpPlanes = 16;
+
+static dim3 op_block(const LevelDev<double>& L)
+{
+    int bx = next_pow2(L.nx < 1 ? 1 : L.nx);
+    if (bx > 256) bx = 256;
+    return dim3(bx, 2, 1);
+}
+
 static dim3 op_grid(const LevelDev<double>& L, dim3 blk)
 {
     return dim3(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), ceil_div(L.nz, kOpPlanes));
@@ -934,7 +1019,7 @@ static dim3 op_grid(const LevelDev<double>& L, dim3 blk)
 size_t reduction_slots(const LevelDev<double>& L)
 {
     const dim3 g3 = red_grid3(L, block2d(L.nx));
-    const dim3 go = op_grid(L, block2d(L.nx));
+    const dim3 go = op_grid(L, op_block(L));
     const size_t chunked = (size_t)g3.x * g3.y * g3.z * 5;  // dots: 5 sums per block
     const size_t flat = (size_t)kRedBlocks * 5;
     const size_t op = (size_t)go.x * go.y * go.z;
@@ -944,10 +1029,10 @@ size_t reduction_slots(const LevelDev<double>& L)
 int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d, double* qv, PcgState* st,
                   const Reducer& red, void* stream)
 {
-    dim3 blk = block2d(L.nx);
+    dim3 blk = op_block(L);
     dim3 grd = op_grid(L, blk);
     const unsigned nb = grd.x * grd.y * grd.z;
-    hipLaunchKernelGGL(k_op_dot, dim3(nb), blk, 0, S(stream), L, beta, d, qv, st, red);
+    hipLaunchKernelGGL(k_op_dot<kOpPlanes>, dim3(nb), blk, 0, S(stream), L, beta, d, qv, st, red);
     hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, S(stream), nb, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
@@ -1061,8 +1146,13 @@ int launch_convert(size_t n, const TS* src, TD* dst, void* stream)
 }
 
 template <class T>
-int launch_tail(const TailPlan<T>* plan_dev, const int* done, void* stream)
+int launch_tail(const TailPlan<T>* plan_dev, const int* done, size_t lds_bytes, void* stream)
 {
+    if (lds_bytes > 0) {
+        hipLaunchKernelGGL(k_tail_lds<T>, dim3(1), dim3(1024), lds_bytes, S(stream), plan_dev, done);
+        TBFG_HIP(hipGetLastError());
+        return TBFG_OK;
+    }
     hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), 0, S(stream), plan_dev, done);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
@@ -1103,8 +1193,26 @@ template int launch_prolong_add<double>(const LevelDev<double>&, const double*, 
 template int launch_prolong_add<float>(const LevelDev<float>&, const float*, float*, const int*, void*);
 template int launch_convert<double, float>(size_t, const double*, float*, void*);
 template int launch_convert<float, double>(size_t, const float*, double*, void*);
-template int launch_tail<double>(const TailPlan<double>*, const int*, void*);
-template int launch_tail<float>(const TailPlan<float>*, const int*, void*);
+template int launch_tail<double>(const TailPlan<double>*, const int*, size_t, void*);
+template int launch_tail<float>(const TailPlan<float>*, const int*, size_t, void*);
+
+template <class T>
+size_t tail_lds_capacity()
+{
+    int dev = 0, maxb = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    const size_t avail = maxb > (int)(sizeof(TailPlan<T>) + 2048) ? (size_t)maxb - sizeof(TailPlan<T>) - 2048 : 0;
+    if (avail == 0) return 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail_lds<T>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return avail;
+}
+template size_t tail_lds_capacity<float>();
+template size_t tail_lds_capacity<double>();
 template int launch_dense_setup<double>(const LevelDev<double>&, const double*, const double*, double*, double*,
                                         void*);
 template int launch_dense_setup<float>(const LevelDev<double>&, const float*, const double*, double*, double*,

@@ -49,7 +49,10 @@ int launch_prolong_add(const LevelDev<T>& L, const T* pc, T* p, const int* done,
 template <class TS, class TD>
 int launch_convert(size_t n, const TS* src, TD* dst, void* stream);
 template <class T>
-int launch_tail(const TailPlan<T>* plan_dev, const int* done, void* stream);
+int launch_tail(const TailPlan<T>* plan_dev, const int* done, size_t lds_bytes, void* stream);
+// dynamic LDS the LDS-resident tail kernel may use on the current device (0: none)
+template <class T>
+size_t tail_lds_capacity();
 template <class T>
 int launch_dense_setup(const LevelDev<double>& L, const T* beta, const double* vol, double* work, double* G,
                        void* stream);

@@ -156,13 +156,25 @@ static int build_plan(tbfg_pcg* h)
     P.cvol = h->cvol.as<T>();
     P.G = h->G.as<double>();
     P.st = h->st.as<PcgState>();
+    size_t lds = 0;
     for (int m = 0; m < P.nlev; ++m) {
         const int l = h->t0 + m;
         P.lv[m].geo = geo<T>(h, l);
         P.lv[m].p = h->p[l].as<T>();
         P.lv[m].q = h->q[l].as<T>();
         P.lv[m].beta = h->beta[l].as<T>();
+        lds += 3 * h->hl[l].cells() * sizeof(T);
+        // the level's table block (LevelTables::build), padded to 4 entries
+        size_t nt = 0;
+        for (int a = 0; a < 3; ++a)
+            nt += 4 * (size_t)h->hl[l].n[a] + (m + 1 < P.nlev ? 2 * (size_t)h->hl[l + 1].n[a] : 0);
+        lds += ((nt + 3) & ~(size_t)3) * sizeof(T);
     }
+    // the LDS-resident tail when its levels fit (the inner coarsest-level CG
+    // keeps its vectors in HBM and uses the other form)
+    h->tail_lds = 0;
+    if (h->prm.coarse_max_iter == 0 && !std::getenv("TBFG_TAIL_HBM") && lds <= tail_lds_capacity<T>())
+        h->tail_lds = lds;
     return upload(h->plan, &P, sizeof P);
 }
 
@@ -441,7 +453,7 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
                 TBFG_TRY(launch_resid_restrict<T, T>(geo<T>(h, l), b, q, cur[l], h->q[l + 1].as<T>(), done, stream));
             }
         }
-        TBFG_TRY(launch_tail<T>(h->plan.as<TailPlan<T>>(), done, stream));
+        TBFG_TRY(launch_tail<T>(h->plan.as<TailPlan<T>>(), done, h->tail_lds, stream));
         for (int l = t0 - 1; l >= 0; --l) {
             const T* b = (l == 0) ? beta0 : h->beta[l].as<T>();
             const T* q = (l == 0) ? r0 : h->q[l].as<T>();

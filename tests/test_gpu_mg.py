@@ -137,6 +137,26 @@ def test_dots_in_last_sweep_match_separate_kernel(torch_gpu, monkeypatch, precis
     assert rel(out[0][1], out[1][1]) < bound
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lds_tail_matches_hbm_tail(torch_gpu, monkeypatch, precision):
+    """The tail kernel with its levels in LDS gives the result of the HBM form
+    (TBFG_TAIL_HBM): the same arithmetic in the same order."""
+    n = (128, 96, 128)
+    print("case lds tail", precision, n, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 12, 1, 41)
+    out = []
+    for hbm in (False, True):
+        if hbm:
+            monkeypatch.setenv("TBFG_TAIL_HBM", "1")
+        s = T.PCG(g, nLevels=4, tolPCG=1e-10, maxIterPCG=200, precision=precision, flexible=1, coarse_max_iter=0)
+        psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+        assert s.converged
+        out.append((s.iterations, to_host(psi)))
+    assert out[0][0] == out[1][0]
+    assert rel(out[0][1], out[1][1]) < 1e-12
+
+
 @pytest.mark.parametrize("n,stretch,nlev,nb", [((64, 64, 64), 0, 4, 18), ((64, 48, 64), 1, 4, 20)])
 def test_pcg_fp64_matches_oracle(torch_gpu, n, stretch, nlev, nb):
     print("case pcg fp64", n, stretch, flush=True)
