@@ -545,15 +545,65 @@ __device__ __forceinline__ void prolong_cell(const LevelDev<T>& L, int ncx, int 
 }
 
 template <class T>
+struct Pair;
+template <>
+struct Pair<float> {
+    using type = float2;
+};
+template <>
+struct Pair<double> {
+    using type = double2;
+};
+
+// One thread per coarse cell: its 2x2x2 fine children all interpolate from the
+// 3x3x3 coarse neighbourhood, loaded once (the weights and the order of the
+// arithmetic are prolong_cell's), and each pair of x-children is one vector
+// access.
+template <class T>
 __global__ void k_prolong_add(LevelDev<T> L, const T* __restrict__ pc, T* __restrict__ p, const int* done)
 {
     if (done && *done) return;
-    const Blk3 bk = xcd_block3((L.nx + blockDim.x - 1) / blockDim.x, (L.ny + blockDim.y - 1) / blockDim.y);
-    const int i = bk.x * blockDim.x + threadIdx.x;
-    const int j = bk.y * blockDim.y + threadIdx.y;
-    const int k = bk.z;
-    if (i >= L.nx || j >= L.ny) return;
-    prolong_cell<T>(L, L.nx >> 1, L.ny >> 1, L.nz >> 1, pc, p, i, j, k);
+    const int nx = L.nx, ny = L.ny;
+    const int ncx = L.nx >> 1, ncy = L.ny >> 1, ncz = L.nz >> 1;
+    const Blk3 bk = xcd_block3((ncx + blockDim.x - 1) / blockDim.x, (ncy + blockDim.y - 1) / blockDim.y);
+    const int I = bk.x * blockDim.x + threadIdx.x;
+    const int J = bk.y * blockDim.y + threadIdx.y;
+    const int K = bk.z;
+    if (I >= ncx || J >= ncy) return;
+    // coarse neighbours: x, z periodic; y walls (the ghost is the wall cell)
+    const int xs[3] = {I > 0 ? I - 1 : ncx - 1, I, I + 1 < ncx ? I + 1 : 0};
+    const int ys[3] = {J > 0 ? J - 1 : 0, J, J + 1 < ncy ? J + 1 : ncy - 1};
+    const int zs[3] = {K > 0 ? K - 1 : ncz - 1, K, K + 1 < ncz ? K + 1 : 0};
+    T c[3][3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) c[a][b][e] = pc[xs[e] + (size_t)ncx * (ys[b] + (size_t)ncy * zs[a])];
+    const T wxl0 = L.a[0].plo[2 * I], wxh0 = L.a[0].phi[2 * I];
+    const T wxl1 = L.a[0].plo[2 * I + 1], wxh1 = L.a[0].phi[2 * I + 1];
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk) {
+        const int k = 2 * K + dk;
+        const T wzl = L.a[2].plo[k], wzh = L.a[2].phi[k];
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const int j = 2 * J + dj;
+            const T wyl = L.a[1].plo[j], wyh = L.a[1].phi[j];
+            // children (2I, j, k) and (2I+1, j, k): coarse x cells (I-1, I) and (I, I+1)
+            const T (&zl)[3][3] = c[dk], (&zh)[3][3] = c[dk + 1];
+            const T v0 = wzl * (wyl * (wxl0 * zl[dj][0] + wxh0 * zl[dj][1]) + wyh * (wxl0 * zl[dj + 1][0] + wxh0 * zl[dj + 1][1])) +
+                         wzh * (wyl * (wxl0 * zh[dj][0] + wxh0 * zh[dj][1]) + wyh * (wxl0 * zh[dj + 1][0] + wxh0 * zh[dj + 1][1]));
+            const T v1 = wzl * (wyl * (wxl1 * zl[dj][1] + wxh1 * zl[dj][2]) + wyh * (wxl1 * zl[dj + 1][1] + wxh1 * zl[dj + 1][2])) +
+                         wzh * (wyl * (wxl1 * zh[dj][1] + wxh1 * zh[dj][2]) + wyh * (wxl1 * zh[dj + 1][1] + wxh1 * zh[dj + 1][2]));
+            typename Pair<T>::type* q = reinterpret_cast<typename Pair<T>::type*>(p + 2 * I + (size_t)nx * (j + (size_t)ny * k));
+            typename Pair<T>::type f = *q;
+            f.x = f.x + v0;
+            f.y = f.y + v1;
+            *q = f;
+        }
+    }
 }
 
 template <class TS, class TD>
@@ -1130,8 +1180,8 @@ int launch_restrict(const LevelDev<T>& L, const TF* fine, T* coarse, void* strea
 template <class T>
 int launch_prolong_add(const LevelDev<T>& L, const T* pc, T* p, const int* done, void* stream)
 {
-    dim3 blk = block2d(L.nx);
-    const dim3 grd(ceil_div(L.nx, blk.x) * ceil_div(L.ny, blk.y) * L.nz);
+    dim3 blk = block2d(L.nx >> 1);
+    const dim3 grd(ceil_div(L.nx >> 1, blk.x) * ceil_div(L.ny >> 1, blk.y) * (L.nz >> 1));
     hipLaunchKernelGGL(k_prolong_add<T>, grd, blk, 0, S(stream), L, pc, p, done);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;

@@ -294,7 +294,11 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         if (l > 0 || h->mixed) TBFG_TRY(h->beta[l].alloc(n * es));
         if (l < t0) {
             h->fused[l] = h->mixed ? sweep_supported<float>(h->tabf[l]->view) : sweep_supported<double>(h->tabd[l]->view);
-            if (std::getenv("TBFG_HALF_SWEEPS")) h->fused[l] = 0;
+            // below ~1M cells a plane march of a few steps per workgroup is
+            // latency bound and the two half-sweep launches are quicker
+            size_t min_cells = (size_t)1 << 20;
+            if (const char* e = std::getenv("TBFG_SWEEP_MIN_CELLS")) min_cells = (size_t)std::atoll(e);
+            if (n < min_cells || std::getenv("TBFG_HALF_SWEEPS")) h->fused[l] = 0;
             if (h->fused[l]) TBFG_TRY(h->palt[l].alloc(n * es));
         }
     }

@@ -89,13 +89,14 @@ struct ColHist {
 // accumulate while the plane is written, one partial per workgroup, instead of
 // a separate pass over r, z and qv.  r and qv of a plane are loaded with the
 // plane registers, two steps ahead.
-template <class T, class TQ, int TX, int TY, int NT, bool FIRST, bool DOTS>
+template <class T, class TQ, int TX, int TY, int NT, bool FIRST, bool DOTS, int PD>
 __global__ void __launch_bounds__(NT)
     k_sweep(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q, const T* __restrict__ pin,
             T* __restrict__ pout, int kc, int tiles_x, int ntiles, const int* done, SweepDots dots)
 {
     using Sh = SweepShape<T, TX, TY, NT>;
     static_assert(Sh::NBLACK == NT, "one tile cell pair per thread");
+    static_assert(PD % 2 == 0, "the register sets alternate with the colour parity of the steps");
     constexpr int PX = Sh::PX, PS = Sh::PS, QX = Sh::QX, QS = Sh::QS, QY = Sh::QY;
     constexpr int SP = FIRST ? 1 : Sh::SP;
     constexpr int NRING = QX + TY;  // red cells of the one-cell ring
@@ -206,14 +207,19 @@ __global__ void __launch_bounds__(NT)
         fetch_pb(rp, rb, k0 - 1);
         stash(rp, rb, k0 - 1);
     }
-    T rpA[Sh::NP], rbA[Sh::NP], rqA[Sh::NQ];
-    T rpB[Sh::NP], rbB[Sh::NP], rqB[Sh::NQ];
-    double2 drA = {0.0, 0.0}, dqA = {0.0, 0.0}, drB = {0.0, 0.0}, dqB = {0.0, 0.0};
-    fetch(rpA, rbA, rqA, drA, dqA, k0);
-    // keep the first set's loads ahead of the second's: the wait before each
-    // plane step then covers only the loads of the set it stashes
-    __builtin_amdgcn_sched_barrier(0);
-    fetch(rpB, rbB, rqB, drB, dqB, k0 + 1);
+    // PD register sets: the loads of a plane are issued PD steps before the
+    // step that stashes them
+    T rps[PD][Sh::NP], rbs[PD][Sh::NP], rqs[PD][Sh::NQ];
+    double2 drs[PD], dqs[PD];
+#pragma unroll
+    for (int u = 0; u < PD; ++u) {
+        drs[u] = double2{0.0, 0.0};
+        dqs[u] = double2{0.0, 0.0};
+        fetch(rps[u], rbs[u], rqs[u], drs[u], dqs[u], k0 + u);
+        // keep each set's loads ahead of the next set's: the wait before a
+        // plane step then covers only the loads of the set it stashes
+        __builtin_amdgcn_sched_barrier(0);
+    }
     __syncthreads();  // tables and planes k0-2, k0-1 in LDS
 
     const T ipy = sTy[row + 1], imy = sTy[QY + row + 1];
@@ -238,7 +244,7 @@ __global__ void __launch_bounds__(NT)
         stash(rp, rb, k + 1);
         stash_plane<NT, T, Sh::NQ>(rq, sQ + slotQ(k) * QS, QS);
         const double2 r_pair = dr, qv_pair = dq;  // plane k-1
-        fetch(rp, rb, rq, dr, dq, k + 3);
+        fetch(rp, rb, rq, dr, dq, k + PD + 1);
         __syncthreads();
 
         const int kw = modz(k);
@@ -375,9 +381,10 @@ __global__ void __launch_bounds__(NT)
     };
 
     // steps k0-1 .. k1 in pairs (a final step k1+1 computes red cells only)
-    for (int k = k0 - 1; k <= k1; k += 2) {
-        step(k, rpA, rbA, rqA, drA, dqA, hA);
-        step(k + 1, rpB, rbB, rqB, drB, dqB, hB);
+    // steps k0-1 .. k1 in groups of PD (the steps past k1 compute red cells only)
+    for (int k = k0 - 1; k <= k1; k += PD) {
+#pragma unroll
+        for (int u = 0; u < PD; ++u) step(k + u, rps[u], rbs[u], rqs[u], drs[u], dqs[u], (u & 1) ? hB : hA);
     }
     if (DOTS) block_partial<5>(acc, dots.red);
 }
@@ -419,17 +426,27 @@ bool sweep_supported(const LevelDev<T>& L)
     return pick_shape(L, c);
 }
 
+// prefetch distance in plane steps: 4 in FP32 on long chunks; 2 in FP64, with
+// the dot products (registers) and on short chunks (whose steps would not
+// fill a group of 4)
+template <class T, bool DOTS>
+constexpr int sweep_pd_max()
+{
+    return sizeof(T) == 4 && !DOTS ? 4 : 2;
+}
+
 template <class T, class TQ, int TX, int TY, int NT, bool FIRST, bool DOTS>
 static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout,
                               const int* done, const SweepDots& dots, unsigned* nblocks_out, void* stream)
 {
+    constexpr int pdm = sweep_pd_max<T, DOTS>();
     static int per_cu = -1, ncu = 0;
     if (per_cu < 0) {
         int dev = 0;
         TBFG_HIP(hipGetDevice(&dev));
         TBFG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         int nb = 0;
-        TBFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS>, NT, 0));
+        TBFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, pdm>, NT, 0));
         per_cu = nb < 1 ? 1 : nb;
     }
     const int tiles_x = L.nx / TX;
@@ -441,14 +458,23 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
     int kc = (L.nz + chunks - 1) / chunks;
     if (kc < 4) kc = 4;
     if (kc > kSweepMaxChunk) kc = kSweepMaxChunk;
+    // kc + 2 plane steps run in groups of the prefetch distance: round kc up
+    // to waste none (while it stays within the LDS z tables)
+    const int pd = (pdm == 4 && kc >= 16) ? 4 : 2;
+    const int kr = ((kc + 2 + pd - 1) / pd) * pd - 2;
+    if (kr <= kSweepMaxChunk) kc = kr;
     chunks = (L.nz + kc - 1) / kc;
     const unsigned nblocks = (unsigned)ntiles * (unsigned)chunks;
     if (DOTS && (size_t)nblocks * 5 > dots.capacity) {
         set_error("launch_sweep: %u workgroups exceed the reduction scratch", nblocks);
         return TBFG_EINVAL;
     }
-    hipLaunchKernelGGL((k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS>), dim3(nblocks), dim3(NT), 0, (hipStream_t)stream, L,
-                       beta, q, pin, pout, kc, tiles_x, ntiles, done, dots);
+    if (pd == 4)
+        hipLaunchKernelGGL((k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, pdm>), dim3(nblocks), dim3(NT), 0,
+                           (hipStream_t)stream, L, beta, q, pin, pout, kc, tiles_x, ntiles, done, dots);
+    else
+        hipLaunchKernelGGL((k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, 2>), dim3(nblocks), dim3(NT), 0,
+                           (hipStream_t)stream, L, beta, q, pin, pout, kc, tiles_x, ntiles, done, dots);
     TBFG_HIP(hipGetLastError());
     if (nblocks_out) *nblocks_out = nblocks;
     return TBFG_OK;

@@ -100,11 +100,12 @@ def test_preconditioner_matches_oracle(torch_gpu, n, stretch, nlev):
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
 @pytest.mark.parametrize("n,nlev", [((128, 96, 64), 4), ((96, 32, 40), 3)])
 def test_full_sweep_kernel_matches_half_sweeps(torch_gpu, monkeypatch, precision, tol, n, nlev):
-    """The out-of-place full-sweep kernel (levels with nx % 32 == 0) gives the
+    """The out-of-place full-sweep kernel (levels its tiles divide) gives the
     V-cycle of the two in-place half-sweep launches (TBFG_HALF_SWEEPS forces
     those at handle creation)."""
     print("case full sweep vs half sweeps", precision, n, nlev, flush=True)
     g, go, rho, _, q, beta = bubble_case(n, 8, 1, 5)
+    monkeypatch.setenv("TBFG_SWEEP_MIN_CELLS", "0")  # the sweep kernel on every level it can tile
     zs = []
     for half in (False, True):
         if half:
@@ -124,6 +125,7 @@ def test_dots_in_last_sweep_match_separate_kernel(torch_gpu, monkeypatch, precis
     n = (128, 96, 64)
     print("case fused dots", precision, n, flush=True)
     g, go, rho, _, q, beta = bubble_case(n, 12, 1, 31)
+    monkeypatch.setenv("TBFG_SWEEP_MIN_CELLS", "0")  # the finest level smoothed by the sweep kernel
     out = []
     for unfused in (False, True):
         if unfused:
