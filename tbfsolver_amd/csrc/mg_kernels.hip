@@ -372,14 +372,16 @@ __global__ void k_rbgs(LevelDev<T> L, const T* __restrict__ beta, const TQ* __re
 }
 
 // fused residual (rbgs.f90:300-315) and restriction (interpolation.f90:97-118):
-// one thread per coarse cell, qc = sum_{children} wx wy wz (q - A p).
+// one thread per coarse cell, qc = sum_{children} wx wy wz (q - A p).  The
+// plane loop stays rolled: all eight children in flight need more registers
+// than a 1024-thread workgroup has, and the spill costs more than the overlap.
 template <class T, class TQ>
 __device__ __forceinline__ T resid_restrict_cell(const LevelDev<T>& L, const T* __restrict__ beta,
                                                  const TQ* __restrict__ q, const T* __restrict__ p, int I, int J,
                                                  int K)
 {
     T acc = 0;
-#pragma unroll
+#pragma unroll 1
     for (int dk = 0; dk < 2; ++dk) {
         const int k = 2 * K + dk;
         const T wz = dk ? L.a[2].rhi[K] : L.a[2].rlo[K];
@@ -835,15 +837,16 @@ __device__ void tail_coarse_pcg(const TailPlan<T>& P)
 }
 
 // V-cycle over tail levels top..bot (multiGrid.f90:186-232) with the coarsest
-// level solved by tail_coarse_pcg (or the dense solve when it is the bottom).
-template <class T>
+// level solved by tail_coarse_pcg (or the dense solve when it is the bottom;
+// COARSE_CG false: always, and the inner CG is not compiled in).
+template <class T, bool COARSE_CG>
 __device__ void tail_vcycle(const TailPlan<T>& P, int top, int bot, bool reset_top)
 {
     for (int l = top; l < bot; ++l) {
         tail_smooth<T>(P.lv[l], P.n_pre, (l > top) || reset_top);
         tail_resid_restrict<T>(P.lv[l], P.lv[l + 1]);
     }
-    if (P.bottom > P.user_last)
+    if (COARSE_CG && P.bottom > P.user_last)
         tail_coarse_pcg<T>(P);
     else
         tail_dense<T>(P, P.lv[bot]);
@@ -858,7 +861,7 @@ __global__ void __launch_bounds__(1024) k_tail(const TailPlan<T>* __restrict__ p
 {
     if (done && *done) return;
     const TailPlan<T>& P = *plan;
-    tail_vcycle<T>(P, 0, P.user_last, true);
+    tail_vcycle<T, true>(P, 0, P.user_last, true);
 }
 
 // Entries of a level's table block (LevelTables::build: ip, im, plo, phi of n
@@ -928,7 +931,7 @@ __global__ void __launch_bounds__(1024) k_tail_lds(const TailPlan<T>* __restrict
         }
     }
     __syncthreads();
-    tail_vcycle<T>(Ps, 0, Ps.user_last, true);
+    tail_vcycle<T, false>(Ps, 0, Ps.user_last, true);
     const int n0 = Ps.lv[0].geo.nx * Ps.lv[0].geo.ny * Ps.lv[0].geo.nz;
     T* gp = plan->lv[0].p;
     for (int t = threadIdx.x; t < n0; t += blockDim.x) gp[t] = Ps.lv[0].p[t];
