@@ -76,6 +76,39 @@ def copy_peak(torch, nbytes=1 << 30, reps=5):
     return 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
 
 
+def batch_leg(torch, reps=3, workers=8):
+    """64 uneven instances (seed 1003; FFT with constant density on even indices,
+    PCG + multigrid with 4..24 bubbles on odd ones), mixed precision, each solved
+    once per pass; passes timed with 8 host threads on 8 streams and with one."""
+    from tbfsolver_amd import batch as Bt
+
+    t0 = time.perf_counter()
+    specs = Bt.config3_instances(64, 1003)
+    insts = [Bt.Instance(*x) for x in specs]
+    setup = time.perf_counter() - t0
+    Bt.run_batch(insts, workers)  # warm-up
+    conc = min(Bt.run_batch(insts, workers) for _ in range(reps))
+    seq = Bt.run_batch(insts, 1)
+    # error against FP64 solves to 1e-12 on every third instance (both kinds)
+    errs = []
+    for inst in insts[::3]:
+        ref = Bt.Instance(inst.index, inst.n, inst.kind, specs[inst.index][3], precision="fp64", tol=1e-12)
+        ref.solve()
+        a, b = inst.psi - inst.psi.mean(), ref.psi - ref.psi.mean()
+        errs.append(float(torch.linalg.norm(a - b) / torch.linalg.norm(b)))
+        ref.close()
+    cells = sum(n[0] * n[1] * n[2] for _, n, _, _ in specs)
+    for inst in insts:
+        inst.close()
+    log("batch: 64 instances (%.1f M cells) %.1f ms per pass on %d streams, %.1f ms on one; max error %.1e"
+        % (cells / 1e6, conc * 1e3, workers, seq * 1e3, max(errs)))
+    return {"instances": 64, "cells_total": cells, "solves_per_s": round(64 / conc, 2),
+            "ms_per_pass": round(conc * 1e3, 3), "streams": workers,
+            "one_stream_solves_per_s": round(64 / seq, 2), "max_rel_error_vs_fp64_1e-12": max(errs),
+            "error_sample": "every third instance", "setup_s": round(setup, 1),
+            "precision": "mixed (FP32 FFT + FP64 refinement; FP32 multigrid in FP64 flexible PCG to 1e-8)"}
+
+
 def cpu_baseline(g_dims, rho, ux, uy, uz, q_host, beta_host):
     """The oracle (C, OpenMP, FP64, the reference algorithm with an exact
     coarsest solve) on the host cores: one full solve of the same workload."""
@@ -98,6 +131,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     ap.add_argument("--nref", type=int, default=50)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the pressure solve of one channel instance runs on a single "
@@ -215,6 +249,9 @@ def main():
         solve_gbs = out["iterations"] * per_it / (out["p50_ms"] * 1e-3) / 1e9
         out["solve_roofline"] = {"bytes_per_iteration": per_it, "achieved": round(solve_gbs, 1), "unit": "GB/s",
                                  "frac": round(solve_gbs / HBM_PEAK_GBS, 4)}
+
+    if not args.no_batch:
+        out["config3"] = batch_leg(torch)
 
     if not args.no_cpu:
         q_host = np.asfortranarray(q.cpu().numpy().T)

@@ -1,0 +1,114 @@
+"""Many independent channel instances on one GPU -- the batched workload of the
+benchmark family (64 uneven instances: sizes drawn from a seed, FFT solves with
+constant density on even indices, PCG + multigrid with bubbles on odd ones).
+
+The reference runs one instance per process.  Here every instance is its own
+solver handle with its own device fields; a pool of host threads drives the
+solves on separate HIP streams (default priority), so the small launches of
+different instances overlap on the GPU.  The library releases the Python
+interpreter lock during every call, and distinct handles are independent.
+
+  config3_instances(count, seed)  the instance list (sizes, kind, bubbles)
+  Instance                        one instance: grid, inputs, handle, solve()
+  run_batch(instances, workers)   all solves, wall time
+"""
+import concurrent.futures as cf
+import time
+
+import numpy as np
+
+from . import fields as F
+from .solvers import FastPoissonSolver, PCG, channel_grid
+
+NXZ = (32, 64, 96, 128, 192)
+NY = (32, 64, 96, 128)
+_M64 = (1 << 64) - 1
+
+
+def _uniform01(seed, idx):
+    """The library's counter-based generator (splitmix64 finaliser of seed*phi + idx + 1)."""
+    z = (seed * 0x9E3779B97F4A7C15 + idx + 1) & _M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    z ^= z >> 31
+    return (z >> 11) * (1.0 / 9007199254740992.0)
+
+
+def config3_instances(count=64, seed=1003):
+    """[(index, (nx, ny, nz), kind, bubbles)]: nx, nz from {32, 64, 96, 128, 192},
+    ny from {32, 64, 96, 128}; kind "FAST" on even indices, "MG" with 4..24
+    bubbles on odd ones."""
+    out = []
+    for i in range(count):
+        u = [_uniform01(seed, 4 * i + m) for m in range(4)]
+        n = (NXZ[int(u[0] * len(NXZ))], NY[int(u[1] * len(NY))], NXZ[int(u[2] * len(NXZ))])
+        kind = "FAST" if i % 2 == 0 else "MG"
+        nb = 4 + int(u[3] * 21) if kind == "MG" else 0
+        out.append((i, n, kind, nb))
+    return out
+
+
+class Instance:
+    """One channel instance: uniform grid (Lx = 2 pi, Ly = 2, Lz = pi), the
+    seeded velocity's divergence as right-hand side, and its solver."""
+
+    def __init__(self, index, n, kind, nbubbles, seed=1003, precision="mixed", tol=1e-8, adt=1e-3):
+        import torch
+
+        self.index, self.n, self.kind = index, tuple(n), kind
+        self.grid = channel_grid(n)
+        iseed = seed * 1000 + index
+        ux, uy, uz = F.make_velocity(self.grid, iseed)
+        dev = [torch.from_numpy(np.ascontiguousarray(a.T)).cuda() for a in (ux, uy, uz)]
+        self.q = torch.zeros(n[2], n[1], n[0], dtype=torch.float64, device="cuda")
+        F.computeSource(self.grid, dev[0], dev[1], dev[2], self.q)
+        self.psi = torch.zeros_like(self.q)
+        if kind == "FAST":
+            # constant density: s = div u / dt made mean free, as in config 0
+            self.q /= adt
+            self.q -= self.q.mean()
+            self.solver = FastPoissonSolver(self.grid, precision=precision)
+            self.beta = None
+        else:
+            rho, _ = F.make_bubble_density(self.grid, nbubbles, 0.3, iseed, nref=8)
+            self.rho = torch.from_numpy(np.ascontiguousarray(rho.T)).cuda()
+            self.beta = torch.zeros_like(self.q)
+            F.computeBeta(self.grid, adt, self.rho, self.beta)
+            self.solver = PCG(self.grid, nLevels=4, tolPCG=tol, maxIterPCG=200, precision=precision, flexible=1,
+                              coarse_max_iter=0 if precision == "mixed" else -1)
+        self.iterations = 0
+        torch.cuda.synchronize()
+
+    def solve(self, stream=None):
+        self.psi.zero_()
+        if self.kind == "FAST":
+            self.solver.solveFPS(self.psi, self.q, stream=stream)
+        else:
+            self.solver.solvePCG(self.psi, self.beta, self.q, stream=stream)
+            self.iterations = self.solver.iterations
+        return self.psi
+
+    def close(self):
+        self.solver.close()
+
+
+def run_batch(instances, workers=8):
+    """Solves every instance once, instance i on worker i % workers, each worker
+    on its own stream; returns the wall time in seconds."""
+    import torch
+
+    streams = [torch.cuda.Stream() for _ in range(max(1, workers))]
+    torch.cuda.synchronize()
+
+    def work(w):
+        s = streams[w]
+        with torch.cuda.stream(s):
+            for inst in instances[w::len(streams)]:
+                inst.solve(stream=s)
+        s.synchronize()
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=len(streams)) as ex:
+        list(ex.map(work, range(len(streams))))
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
