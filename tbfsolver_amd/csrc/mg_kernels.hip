@@ -159,6 +159,128 @@ __global__ void __launch_bounds__(512) k_op_dot(LevelDev<double> L, const double
     block_partial<1>(v, red);
 }
 
+// The same product marching through planes: a workgroup of NT threads owns a
+// TX x (NT/TX) tile and a chunk of planes; each plane of d and beta (tile + one-cell ring)
+// is staged once in LDS, loaded PD plane steps ahead in PD register sets, and
+// the stencil of plane k reads planes k-1, k, k+1 from LDS (one barrier per
+// plane).  The in-plane neighbours of a cell then cost no HBM traffic.
+constexpr int kOpMaxChunk = 64;
+
+template <int TX, int NT>
+struct OpTile {
+    static constexpr int TY = NT / TX;
+    static constexpr int OX = TX + 2, OS = OX * (TY + 2);
+    static constexpr int ON = (OS + NT - 1) / NT;
+};
+
+template <int TX, int NT, int PD>
+__global__ void __launch_bounds__(NT) k_op_march(LevelDev<double> L, const double* __restrict__ beta,
+                                                  const double* __restrict__ d, double* __restrict__ qv,
+                                                  PcgState* st, Reducer red, int kc, int tiles_x, int ntiles)
+{
+    using Sh = OpTile<TX, NT>;
+    constexpr int TY = Sh::TY, OX = Sh::OX, OS = Sh::OS, ON = Sh::ON;
+    __shared__ double sD[4 * OS];
+    __shared__ double sB[4 * OS];
+    __shared__ double sTz[2 * kOpMaxChunk];
+    if (st->done) return;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    unsigned id = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) id = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const int tile = (int)(id % (unsigned)ntiles);
+    const int chunk = (int)(id / (unsigned)ntiles);
+    const int k0 = min(chunk * kc, nz);
+    const int k1 = min(nz, k0 + kc);
+    const int tx0 = (tile % tiles_x) * TX;
+    const int ty0 = (tile / tiles_x) * TY;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int gx = tx0 + tx, gy = ty0 + ty;  // TX divides nx, TY divides ny
+    for (int t = threadIdx.x; t < k1 - k0; t += NT) {
+        sTz[t] = L.a[2].ip[k0 + t];
+        sTz[kOpMaxChunk + t] = L.a[2].im[k0 + t];
+    }
+    int off[ON];
+#pragma unroll
+    for (int m = 0; m < ON; ++m) {
+        const int t = min((int)threadIdx.x + m * NT, OS - 1);
+        const int yy = t / OX, xx = t - yy * OX;
+        const int jj = ty0 + yy - 1, ii = tx0 + xx - 1;
+        off[m] = nx * (jj < 0 ? 0 : (jj >= ny ? ny - 1 : jj)) + (ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii));
+    }
+    const size_t plane = (size_t)nx * ny;
+    auto modz = [&](int k) { return ((k % nz) + nz) % nz; };
+    auto slot = [&](int k) { return (k - k0 + 4) & 3; };
+    auto fetch = [&](double(&rd)[ON], double(&rb)[ON], int kp) {
+        const size_t o = plane * modz(kp);
+#pragma unroll
+        for (int m = 0; m < ON; ++m) {
+            rd[m] = d[o + off[m]];
+            rb[m] = beta[o + off[m]];
+        }
+    };
+    auto stash = [&](const double(&rd)[ON], const double(&rb)[ON], int k) {
+        double* D = sD + slot(k) * OS;
+        double* B = sB + slot(k) * OS;
+#pragma unroll
+        for (int m = 0; m < ON; ++m) {
+            const int t = min((int)threadIdx.x + m * NT, OS - 1);
+            D[t] = rd[m];
+            B[t] = rb[m];
+        }
+    };
+    double acc[1] = {0.0};
+    if (k0 < k1) {
+        const double ipx = L.a[0].ip[gx], imx = L.a[0].im[gx];
+        const double ipy = L.a[1].ip[gy], imy = L.a[1].im[gy];
+        const int ic = (ty + 1) * OX + tx + 1;
+        {
+            double rd[ON], rb[ON];
+            fetch(rd, rb, k0 - 1);
+            stash(rd, rb, k0 - 1);
+            fetch(rd, rb, k0);
+            stash(rd, rb, k0);
+        }
+        double rds[PD][ON], rbs[PD][ON];
+#pragma unroll
+        for (int u = 0; u < PD; ++u) {
+            fetch(rds[u], rbs[u], k0 + 1 + u);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        double* out = qv + (size_t)nx * gy + gx;
+        auto step = [&](int k, double(&rd)[ON], double(&rb)[ON]) {
+            stash(rd, rb, k + 1);
+            fetch(rd, rb, k + 1 + PD);  // (past the chunk end: loaded, not used)
+            __syncthreads();
+            if (k < k1) {
+                const double* D0 = sD + slot(k) * OS;
+                const double* Dp = sD + slot(k + 1) * OS;
+                const double* Dm = sD + slot(k - 1) * OS;
+                const double* B0 = sB + slot(k) * OS;
+                const double* Bp = sB + slot(k + 1) * OS;
+                const double* Bm = sB + slot(k - 1) * OS;
+                // face coefficients (rbgs.f90:220-226) and (A d)_c (pcg.f90:171-176), as apply_cell
+                const double bc = B0[ic];
+                const double aR = 0.5 * (B0[ic + 1] + bc) * ipx;
+                const double aL = 0.5 * (bc + B0[ic - 1]) * imx;
+                const double aT = 0.5 * (B0[ic + OX] + bc) * ipy;
+                const double aB = 0.5 * (bc + B0[ic - OX]) * imy;
+                const double aF = 0.5 * (Bp[ic] + bc) * sTz[k - k0];
+                const double aK = 0.5 * (bc + Bm[ic]) * sTz[kOpMaxChunk + k - k0];
+                const double dc = D0[ic];
+                const double v = aR * (D0[ic + 1] - dc) - aL * (dc - D0[ic - 1]) + aT * (D0[ic + OX] - dc) -
+                                 aB * (dc - D0[ic - OX]) + aF * (Dp[ic] - dc) - aK * (dc - Dm[ic]);
+                out[plane * k] = v;
+                acc[0] += dc * v;
+            }
+        };
+        for (int k = k0; k < k1; k += PD) {
+#pragma unroll
+            for (int u = 0; u < PD; ++u) step(k + u, rds[u], rbs[u]);
+        }
+    }
+    block_partial<1>(acc, red);
+}
+
 __global__ void k_fin_alpha(unsigned nblocks, PcgState* st, Reducer red)
 {
     if (st->done) return;
@@ -1069,19 +1191,63 @@ static dim3 op_grid(const LevelDev<double>& L, dim3 blk)
     return dim3(ceil_div(L.nx, blk.x), ceil_div(L.ny, blk.y), ceil_div(L.nz, kOpPlanes));
 }
 
+OpPlan op_plan(const LevelDev<double>& L)
+{
+    OpPlan m;
+    const char* env = std::getenv("TBFG_OP_TILE");
+    std::vector<std::pair<int, int>> cand = {{128, 512}, {64, 512}, {32, 256}};
+    if (env) {
+        int a = 0, b = 0;
+        if (std::sscanf(env, "%dx%d", &a, &b) != 2) return m;
+        cand = {{a, b}};
+    }
+    for (auto [tx, nt] : cand) {
+        if (L.nx % tx == 0 && L.ny % (nt / tx) == 0 && L.nz >= 2) {
+            m.tx = tx;
+            m.nt = nt;
+            break;
+        }
+    }
+    if (!m.tx) return m;
+    m.tiles_x = L.nx / m.tx;
+    m.ntiles = m.tiles_x * (L.ny / (m.nt / m.tx));
+    // about 6 x 256 threads resident per CU, chunks a multiple of 4 planes
+    const int chunks = std::max(1, (1536 * 256 / m.nt + m.ntiles - 1) / m.ntiles);
+    int kc = (L.nz + chunks - 1) / chunks;
+    kc = std::min(kOpMaxChunk, std::max(4, (kc + 3) / 4 * 4));
+    if (const char* e = std::getenv("TBFG_OP_KC")) kc = std::min(kOpMaxChunk, std::max(4, std::atoi(e) / 4 * 4));
+    m.kc = kc;
+    m.nblocks = (unsigned)m.ntiles * (unsigned)((L.nz + kc - 1) / kc);
+    return m;
+}
+
 size_t reduction_slots(const LevelDev<double>& L)
 {
     const dim3 g3 = red_grid3(L, block2d(L.nx));
     const dim3 go = op_grid(L, op_block(L));
     const size_t chunked = (size_t)g3.x * g3.y * g3.z * 5;  // dots: 5 sums per block
     const size_t flat = (size_t)kRedBlocks * 5;
-    const size_t op = (size_t)go.x * go.y * go.z;
+    const size_t op = std::max((size_t)go.x * go.y * go.z, (size_t)op_plan(L).nblocks);
     return std::max(std::max(chunked, flat), op);
 }
 
-int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d, double* qv, PcgState* st,
-                  const Reducer& red, void* stream)
+int launch_op_dot(const LevelDev<double>& L, const OpPlan& m, const double* beta, const double* d, double* qv,
+                  PcgState* st, const Reducer& red, void* stream)
 {
+    if (m.tx) {
+#define TBFG_OPM(TX_, NT_)                                                                                          \
+    if (m.tx == TX_ && m.nt == NT_)                                                                                 \
+        hipLaunchKernelGGL((k_op_march<TX_, NT_, 4>), dim3(m.nblocks), dim3(NT_), 0, S(stream), L, beta, d, qv, st, \
+                           red, m.kc, m.tiles_x, m.ntiles);
+        TBFG_OPM(128, 512) else TBFG_OPM(64, 512) else TBFG_OPM(32, 256) else TBFG_OPM(64, 256) else {
+            set_error("no operator tile %dx%d", m.tx, m.nt);
+            return TBFG_EINVAL;
+        }
+#undef TBFG_OPM
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, S(stream), m.nblocks, st, red);
+        TBFG_HIP(hipGetLastError());
+        return TBFG_OK;
+    }
     dim3 blk = op_block(L);
     dim3 grd = op_grid(L, blk);
     const unsigned nb = grd.x * grd.y * grd.z;

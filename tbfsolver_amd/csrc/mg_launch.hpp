@@ -7,9 +7,18 @@ namespace tbfg {
 // doubles of reduction scratch the CG kernels of a level need
 size_t reduction_slots(const LevelDev<double>& L);
 
+// the operator product's launch: plane-marching tiles of tx x (nt / tx) cells
+// and kc planes, or (tx == 0: no tile divides the level) column segments.
+// TBFG_OP_TILE ("TXxNT", or "columns") and TBFG_OP_KC override the choice.
+struct OpPlan {
+    int tx = 0, nt = 0, tiles_x = 0, ntiles = 0, kc = 0;
+    unsigned nblocks = 0;
+};
+OpPlan op_plan(const LevelDev<double>& L);
+
 int launch_apply_op(const LevelDev<double>& L, const double* beta, const double* p, double* q, void* stream);
-int launch_op_dot(const LevelDev<double>& L, const double* beta, const double* d, double* qv, PcgState* st,
-                  const Reducer& red, void* stream);
+int launch_op_dot(const LevelDev<double>& L, const OpPlan& plan, const double* beta, const double* d, double* qv,
+                  PcgState* st, const Reducer& red, void* stream);
 int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
                          double* r, float* r32, PcgState* st, const Reducer& red, void* stream);
 int launch_pcg_update(size_t n, double* psi, double* r, float* r32, const double* d, const double* qv,

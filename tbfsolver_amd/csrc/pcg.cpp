@@ -309,6 +309,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     TBFG_TRY(h->qv.alloc(h->n0 * 8));
     TBFG_TRY(h->st.alloc(sizeof(PcgState)));
     TBFG_TRY(h->partials.alloc(reduction_slots(h->tabd[0]->view) * sizeof(double)));
+    h->opp = op_plan(h->tabd[0]->view);
     TBFG_TRY(h->counter.alloc(256));
     TBFG_HIP(hipMemset(h->counter.p, 0, 256));
     // dense bottom operator
@@ -518,7 +519,7 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
         TBFG_TRY(launch_pcg_dir<T>(n, z, d, init, st, stream));
         {
             ProfScope ps(h, 2, stream);
-            TBFG_TRY(launch_op_dot(L0, beta, d, qv, st, red, stream));
+            TBFG_TRY(launch_op_dot(L0, h->opp, beta, d, qv, st, red, stream));
         }
         {
             ProfScope ps(h, 3, stream);

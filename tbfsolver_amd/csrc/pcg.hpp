@@ -3,7 +3,7 @@
 #pragma once
 #include <memory>
 
-#include "mg_kernels.hpp"
+#include "mg_launch.hpp"
 
 struct tbfg_pcg {
     tbfg_grid_desc g{};
@@ -25,6 +25,7 @@ struct tbfg_pcg {
     std::vector<char> fused;            // level smoothed by the out-of-place full-sweep kernel
     bool fuse_dots = true;              // PCG dot products formed by the finest level's last sweep
     size_t tail_lds = 0;                // dynamic LDS of the LDS-resident tail kernel (0: HBM form)
+    tbfg::OpPlan opp;                   // launch of the finest level's operator product
     tbfg::DevBuf r, d, qv;              // CG vectors (FP64)
     tbfg::DevBuf r32;                   // FP32 copy of r (mixed precision: the multigrid's right-hand side)
     tbfg::DevBuf st, partials, counter; // PcgState + reduction scratch

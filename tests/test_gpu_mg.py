@@ -118,6 +118,28 @@ def test_full_sweep_kernel_matches_half_sweeps(torch_gpu, monkeypatch, precision
     assert rel(zs[0], zs[1]) < tol
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n", [(128, 96, 64), (96, 64, 48), (48, 40, 32)])
+def test_plane_marching_operator_matches_columns(torch_gpu, monkeypatch, precision, n):
+    """qv = A d by the plane-marching tile kernel (128x4 tiles at nx = 128, 32x8
+    at nx = 96; nx = 48 has no tile, so both runs take columns) gives the solve
+    of the column-segment kernel (TBFG_OP_TILE=columns)."""
+    print("case operator tiles", precision, n, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 8, 1, 17)
+    out = []
+    for tile in (None, "columns"):
+        if tile:
+            monkeypatch.setenv("TBFG_OP_TILE", tile)
+        s = T.PCG(g, nLevels=3, tolPCG=1e-10, maxIterPCG=200, precision=precision, flexible=1)
+        psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+        assert s.converged
+        out.append((s.iterations, mean_free(to_host(psi))))
+        s.close()
+    assert abs(out[0][0] - out[1][0]) <= 1
+    assert rel(out[0][1], out[1][1]) < 1e-9
+
+
 @pytest.mark.parametrize("precision,bound", [("fp64", 1e-10), ("mixed", 1e-9)])
 def test_dots_in_last_sweep_match_separate_kernel(torch_gpu, monkeypatch, precision, bound):
     """The PCG dot products formed by the finest level's last post-smoothing sweep
