@@ -109,6 +109,45 @@ def batch_leg(torch, reps=3, workers=8):
             "precision": "mixed (FP32 FFT + FP64 refinement; FP32 multigrid in FP64 flexible PCG to 1e-8)"}
 
 
+def serving_leg(torch, rates=(50, 200, 500, 1000, 2000), workers=8, fp64_rates=(50, 200)):
+    """Open-loop Poisson arrivals over 256 resident instances (config-3 sizes,
+    70 % PCG + MG with bubbles, 30 % FFT), each request one projection step;
+    8 worker streams with continuous batching, admission by HBM footprint under
+    200 GB.  Beside it the FP64 unbatched loop (FP64 solvers, one request at a
+    time) on the same request streams."""
+    from tbfsolver_amd import serving as Sv
+
+    specs = Sv.serving_instances(256, 1004)
+    out = {"instances": 256, "mix": "70% PCG+MG (4..24 bubbles), 30% FFT", "hbm_budget_gb": 200,
+           "workers": workers, "request": "one projection step (source, solve from the last pressure, projection)"}
+
+    def leg(precision, nworkers, rate_list):
+        t0 = time.perf_counter()
+        loop = Sv.ServingLoop(specs, precision=precision, workers=nworkers, hbm_budget=200e9)
+        loop.preload()
+        setup = time.perf_counter() - t0
+        loop.run(Sv.poisson_arrivals(400.0, 1.0, 256, seed=99))  # first steps allocate their work arrays
+        rows = []
+        for rate in rate_list:
+            reqs = Sv.poisson_arrivals(float(rate), min(2.0, 800.0 / rate), 256, seed=rate)
+            r = loop.run(reqs)
+            rows.append({"rate_rps": rate, "requests": r["requests"], "throughput_rps": round(r["throughput_rps"], 1),
+                         "p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3)})
+            log("serving %s x%d at %d req/s: %d requests, %.1f req/s, p50 %.2f ms, p99 %.2f ms"
+                % (precision, nworkers, rate, r["requests"], r["throughput_rps"], r["p50_ms"], r["p99_ms"]))
+        resident_gb = loop.adm.used / 1e9
+        loop.close()
+        torch.cuda.empty_cache()
+        return rows, setup, resident_gb
+
+    rows, setup, gb = leg("mixed", workers, rates)
+    out.update({"mixed_batched": rows, "setup_s": round(setup, 1), "resident_gb": round(gb, 1)})
+    if fp64_rates:
+        rows64, _, _ = leg("fp64", 1, fp64_rates)
+        out["fp64_unbatched"] = rows64
+    return out
+
+
 def cpu_baseline(g_dims, rho, ux, uy, uz, q_host, beta_host):
     """The oracle (C, OpenMP, FP64, the reference algorithm with an exact
     coarsest solve) on the host cores: one full solve of the same workload."""
@@ -132,6 +171,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     ap.add_argument("--nref", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the pressure solve of one channel instance runs on a single "
@@ -252,6 +292,8 @@ def main():
 
     if not args.no_batch:
         out["config3"] = batch_leg(torch)
+    if not args.no_serve:
+        out["config4"] = serving_leg(torch)
 
     if not args.no_cpu:
         q_host = np.asfortranarray(q.cpu().numpy().T)

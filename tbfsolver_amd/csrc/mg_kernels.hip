@@ -6,6 +6,7 @@
 #include "mg_kernels.hpp"
 #include "mg_launch.hpp"
 #include "reduce.hpp"
+#include "smooth.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -477,7 +478,7 @@ __device__ __forceinline__ void rbgs_cell(const LevelDev<T>& L, const T* __restr
         const T ps = (s == c) ? self : p[s];
         num = -qc + (a.aR * p[e] + a.aL * p[w]) + (a.aT * pn + a.aB * ps) + (a.aF * p[f] + a.aK * p[b]);
     }
-    p[c] = num / den;
+    p[c] = smooth_div(num, den);
 }
 
 template <class T, class TQ, int COLOUR, bool FIRST>

@@ -19,6 +19,7 @@
 #include "mg_kernels.hpp"
 #include "mg_launch.hpp"
 #include "reduce.hpp"
+#include "smooth.hpp"
 
 #include <cstdio>
 #include <cstdlib>
@@ -27,8 +28,8 @@ namespace tbfg {
 
 namespace {
 
-constexpr int kSweepMaxChunk = 60;  // planes per workgroup (z face factors staged in LDS)
-constexpr int kZT = kSweepMaxChunk + 3;
+constexpr int kSweepMaxChunk = 58;  // planes per workgroup (z face factors staged in LDS)
+constexpr int kZT = kSweepMaxChunk + 6;
 
 template <class T, int TX, int TY, int NT>
 struct SweepShape {
@@ -37,9 +38,11 @@ struct SweepShape {
     static constexpr int NP = (PS + NT - 1) / NT;
     static constexpr int NQ = (QS + NT - 1) / NT;
     static constexpr int NBLACK = TY * (TX / 2);                    // black cells of a tile plane
-    // ring depths: a slot is refilled two steps after its last reader, so two
-    // barriers per plane step suffice
-    static constexpr int SP = 4, SB = 5, SQ = 3, SR = 3;
+    // ring depth 4 for every staged plane (the plane steps run in groups of 4,
+    // so the slot of a plane is a constant of the unrolled step): a slot is
+    // refilled at least one full step after its last reader, so two barriers
+    // per plane step suffice
+    static constexpr int RING = 4;
 };
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -50,6 +53,27 @@ __device__ __forceinline__ void stash_plane(const T (&reg)[N], T* __restrict__ d
 {
 #pragma unroll
     for (int m = 0; m < N; ++m) dst[min((int)threadIdx.x + m * NT, S - 1)] = reg[m];
+}
+
+// raw buffer loads: a plane's loads share one scalar byte offset (the plane
+// base) and differ by a fixed per-thread 32-bit offset, so no 64-bit address
+// arithmetic per load; a load outside the array's bytes returns 0
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, size_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
+                                             (int)(bytes < 0x7fffffffu ? bytes : 0x7fffffffu), 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
 template <class T>
@@ -98,15 +122,16 @@ __global__ void __launch_bounds__(NT)
     static_assert(Sh::NBLACK == NT, "one tile cell pair per thread");
     static_assert(PD % 2 == 0, "the register sets alternate with the colour parity of the steps");
     constexpr int PX = Sh::PX, PS = Sh::PS, QX = Sh::QX, QS = Sh::QS, QY = Sh::QY;
-    constexpr int SP = FIRST ? 1 : Sh::SP;
+    constexpr int RING = Sh::RING;
     constexpr int NRING = QX + TY;  // red cells of the one-cell ring
-    __shared__ T sP[SP * PS];
-    __shared__ T sB[Sh::SB * PS];
-    __shared__ T sQ[Sh::SQ * QS];
-    __shared__ T sR[Sh::SR * QS];
-    __shared__ T sTx[2 * QX];        // x face factors of the tile columns -1..TX
-    __shared__ T sTy[2 * QY];        // y face factors of the tile rows -1..TY
-    __shared__ T sTz[2 * kZT];      // z face factors of the planes k0-1..k1+1
+    __shared__ T sP[(FIRST ? 1 : RING) * PS];
+    __shared__ T sB[RING * PS];
+    __shared__ T sQ[RING * QS];
+    __shared__ T sR[RING * QS];
+    // face factors times 1/2 (exact), so a face coefficient is (beta + beta') * f
+    __shared__ T sTx[2 * QX];    // x, tile columns -1..TX
+    __shared__ T sTy[2 * QY];    // y, tile rows -1..TY
+    __shared__ T sTz[2 * kZT];  // z, planes k0-1.. of the chunk's plane steps
 
     if (done && *done) return;
     const int nx = L.nx, ny = L.ny, nz = L.nz;
@@ -125,45 +150,51 @@ __global__ void __launch_bounds__(NT)
     const int tx0 = (tile % tiles_x) * TX;
     const int ty0 = (tile / tiles_x) * TY;
 
-    auto slotP = [&](int k) { return FIRST ? 0 : (k - k0 + 4) % Sh::SP; };
-    auto slotB = [&](int k) { return (k - k0 + 5) % Sh::SB; };
-    auto slotQ = [&](int k) { return (k - k0 + 3) % Sh::SQ; };
-    auto slotR = [&](int k) { return (k - k0 + 3) % Sh::SR; };
-    auto modz = [&](int k) { return ((k % nz) + nz) % nz; };
+    // plane steps k0-1 .. k1 (and up to 3 more, which compute red cells only)
+    const int nsteps = (k1 - k0 + 2 + 3) & ~3;
+    auto slot = [&](int k) { return (k - k0 + RING) & (RING - 1); };
+    // the planes touched lie in [k0 - 2, k1 + 9] and nz >= 16
+    auto modz = [&](int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
 
+    const T half = (T)0.5;
     for (int t = threadIdx.x; t < QX; t += NT) {
         const int gx = wrapi(tx0 + t - 1, nx);
-        sTx[t] = L.a[0].ip[gx];
-        sTx[QX + t] = L.a[0].im[gx];
+        sTx[t] = half * L.a[0].ip[gx];
+        sTx[QX + t] = half * L.a[0].im[gx];
     }
     for (int t = threadIdx.x; t < QY; t += NT) {
         const int gy = clampi(ty0 + t - 1, ny);
-        sTy[t] = L.a[1].ip[gy];
-        sTy[QY + t] = L.a[1].im[gy];
+        sTy[t] = half * L.a[1].ip[gy];
+        sTy[QY + t] = half * L.a[1].im[gy];
     }
     // (vector loads of these inside the plane loop would wait behind the
     // prefetch of the next planes)
-    for (int t = threadIdx.x; t < k1 - k0 + 3; t += NT) {
-        const int kw = wrapi(k0 - 1 + t, nz);
-        sTz[t] = L.a[2].ip[kw];
-        sTz[kZT + t] = L.a[2].im[kw];
+    for (int t = threadIdx.x; t < nsteps + 1; t += NT) {
+        const int kw = modz(k0 - 1 + t);
+        sTz[t] = half * L.a[2].ip[kw];
+        sTz[kZT + t] = half * L.a[2].im[kw];
     }
 
-    // in-plane offsets of the cells this thread loads (fixed for the chunk)
-    int offp[Sh::NP], offq[Sh::NQ];
+    // in-plane byte offsets of the cells this thread loads (fixed for the
+    // chunk; unsigned, so a load is a uniform plane base plus this offset)
+    unsigned offp[Sh::NP], offq[Sh::NQ];
 #pragma unroll
     for (int m = 0; m < Sh::NP; ++m) {
         const int t = min((int)threadIdx.x + m * NT, PS - 1);
         const int yy = t / PX, xx = t - yy * PX;
-        offp[m] = nx * clampi(ty0 + yy - 2, ny) + wrapi(tx0 + xx - 2, nx);
+        offp[m] = (unsigned)(nx * clampi(ty0 + yy - 2, ny) + wrapi(tx0 + xx - 2, nx)) * (unsigned)sizeof(T);
     }
 #pragma unroll
     for (int m = 0; m < Sh::NQ; ++m) {
         const int t = min((int)threadIdx.x + m * NT, QS - 1);
         const int yy = t / QX, xx = t - yy * QX;
-        offq[m] = nx * clampi(ty0 + yy - 1, ny) + wrapi(tx0 + xx - 1, nx);
+        offq[m] = (unsigned)(nx * clampi(ty0 + yy - 1, ny) + wrapi(tx0 + xx - 1, nx)) * (unsigned)sizeof(TQ);
     }
     const size_t plane_sz = (size_t)nx * ny;
+    const size_t level_sz = plane_sz * nz;
+    const __amdgpu_buffer_rsrc_t rs_p = plane_rsrc(pin, level_sz * sizeof(T));
+    const __amdgpu_buffer_rsrc_t rs_b = plane_rsrc(beta, level_sz * sizeof(T));
+    const __amdgpu_buffer_rsrc_t rs_q = plane_rsrc(q, level_sz * sizeof(TQ));
     // this thread's tile cell pair and the column used at steps of each parity
     const int row = threadIdx.x / (TX / 2), h = threadIdx.x - row * (TX / 2);
     const int gy = ty0 + row;
@@ -172,30 +203,29 @@ __global__ void __launch_bounds__(NT)
 
     // planes of p and beta at kp, of q at kp - 1, into registers
     auto fetch_pb = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], int kp) {
-        const T* pp = pin + plane_sz * modz(kp);
-        const T* bp = beta + plane_sz * modz(kp);
+        const unsigned so = (unsigned)(plane_sz * modz(kp) * sizeof(T));
 #pragma unroll
         for (int m = 0; m < Sh::NP; ++m) {
-            if (!FIRST) rp[m] = pp[offp[m]];
-            rb[m] = bp[offp[m]];
+            if (!FIRST) rp[m] = buf_load<T>(rs_p, offp[m], so);
+            rb[m] = buf_load<T>(rs_b, offp[m], so);
         }
     };
     // ... and, for DOTS, r and qv of this thread's pair on plane kp - 2 (the
     // black plane of the step that stashes these registers)
     auto fetch = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], double2& dr, double2& dq, int kp) {
         fetch_pb(rp, rb, kp);
-        const TQ* qp = q + plane_sz * modz(kp - 1);
+        const unsigned so = (unsigned)(plane_sz * modz(kp - 1) * sizeof(TQ));
 #pragma unroll
-        for (int m = 0; m < Sh::NQ; ++m) rq[m] = (T)qp[offq[m]];
+        for (int m = 0; m < Sh::NQ; ++m) rq[m] = (T)buf_load<TQ>(rs_q, offq[m], so);
         if (DOTS) {
             const size_t o = plane_sz * modz(kp - 2) + pair_off;
             dr = *reinterpret_cast<const double2*>(dots.r + o);
             if (!dots.init) dq = *reinterpret_cast<const double2*>(dots.qv + o);
         }
     };
-    auto stash = [&](const T(&rp)[Sh::NP], const T(&rb)[Sh::NP], int k) {
-        if (!FIRST) stash_plane<NT, T, Sh::NP>(rp, sP + slotP(k) * PS, PS);
-        stash_plane<NT, T, Sh::NP>(rb, sB + slotB(k) * PS, PS);
+    auto stash = [&](const T(&rp)[Sh::NP], const T(&rb)[Sh::NP], int sl) {
+        if (!FIRST) stash_plane<NT, T, Sh::NP>(rp, sP + sl * PS, PS);
+        stash_plane<NT, T, Sh::NP>(rb, sB + sl * PS, PS);
     };
 
     // prologue: planes k0-2 and k0-1 of p and beta into LDS; the next two
@@ -203,9 +233,9 @@ __global__ void __launch_bounds__(NT)
     {
         T rp[Sh::NP], rb[Sh::NP];
         fetch_pb(rp, rb, k0 - 2);
-        stash(rp, rb, k0 - 2);
+        stash(rp, rb, slot(k0 - 2));
         fetch_pb(rp, rb, k0 - 1);
-        stash(rp, rb, k0 - 1);
+        stash(rp, rb, slot(k0 - 1));
     }
     // PD register sets: the loads of a plane are issued PD steps before the
     // step that stashes them
@@ -224,38 +254,40 @@ __global__ void __launch_bounds__(NT)
 
     const T ipy = sTy[row + 1], imy = sTy[QY + row + 1];
 
-    const T half = (T)0.5;
-    // column of the steps with the parity of k: red (gx + gy + k) odd
-    auto colx = [&](int k) { return 2 * h + ((tx0 + gy + modz(k) + 1) & 1); };
+    // column of the steps with the parity of k: red (gx + gy + k) odd (nz is
+    // even, so the parity of a plane index is that of its periodic image)
+    auto colx = [&](int k) { return 2 * h + ((tx0 + gy + k + 1) & 1); };
     ColHist<T> hA, hB;  // steps k0-1, k0+1, ... and k0, k0+2, ...
     {
         const int xa = colx(k0 - 1), xb = colx(k0);
-        hA.bf = sB[slotB(k0 - 2) * PS + (row + 2) * PX + xa + 2];
-        hB.bf = sB[slotB(k0 - 1) * PS + (row + 2) * PX + xb + 2];
-        hA.pf = FIRST ? (T)0 : sP[slotP(k0 - 2) * PS + (row + 2) * PX + xa + 2];
-        hB.pf = FIRST ? (T)0 : sP[slotP(k0 - 1) * PS + (row + 2) * PX + xb + 2];
+        hA.bf = sB[slot(k0 - 2) * PS + (row + 2) * PX + xa + 2];
+        hB.bf = sB[slot(k0 - 1) * PS + (row + 2) * PX + xb + 2];
+        hA.pf = FIRST ? (T)0 : sP[(FIRST ? 0 : slot(k0 - 2)) * PS + (row + 2) * PX + xa + 2];
+        hB.pf = FIRST ? (T)0 : sP[(FIRST ? 0 : slot(k0 - 1)) * PS + (row + 2) * PX + xb + 2];
         hA.bc = hB.bc = hA.r = hB.r = (T)0;  // first read two steps later
     }
 
-    auto step = [&](int k, T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], double2& dr, double2& dq,
+    // S: the ring slot of plane k (a constant of the unrolled step)
+    auto step = [&](int k, int S, T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], double2& dr, double2& dq,
                     ColHist<T>& H) {
+        const int Sp = (S + 1) & (RING - 1), Sm = (S + RING - 1) & (RING - 1);
+        const int SpP = FIRST ? 0 : Sp, SP0 = FIRST ? 0 : S, SmP = FIRST ? 0 : Sm;
         // planes k+1 (p, beta) and k (q) into LDS, then start loading planes
-        // k+3 and k+2 into the same registers (past the chunk end: unused)
-        stash(rp, rb, k + 1);
-        stash_plane<NT, T, Sh::NQ>(rq, sQ + slotQ(k) * QS, QS);
+        // k+PD+1 and k+PD into the same registers (past the chunk end: unused)
+        stash(rp, rb, Sp);
+        stash_plane<NT, T, Sh::NQ>(rq, sQ + S * QS, QS);
         const double2 r_pair = dr, qv_pair = dq;  // plane k-1
         fetch(rp, rb, rq, dr, dq, k + PD + 1);
         __syncthreads();
 
-        const int kw = modz(k);
-        const T* B0 = sB + slotB(k) * PS;
-        const T* Bp = sB + slotB(k + 1) * PS;
-        const T* Bm = sB + slotB(k - 1) * PS;
-        const T* P0 = sP + slotP(k) * PS;
-        const T* Pp = sP + slotP(k + 1) * PS;
-        const T* Pm = sP + slotP(k - 1) * PS;
-        const T* Q0 = sQ + slotQ(k) * QS;
-        T* R0 = sR + slotR(k) * QS;
+        const T* B0 = sB + S * PS;
+        const T* Bp = sB + Sp * PS;
+        const T* Bm = sB + Sm * PS;
+        const T* P0 = sP + SP0 * PS;
+        const T* Pp = sP + SpP * PS;
+        const T* Pm = sP + SmP * PS;
+        const T* Q0 = sQ + S * QS;
+        T* R0 = sR + S * QS;
 
         // ---- red cell of plane k at this thread's column
         const int xr = colx(k);
@@ -268,12 +300,12 @@ __global__ void __launch_bounds__(NT)
             const T azp = sTz[k - k0 + 1], azm = sTz[kZT + k - k0 + 1];
             const T bc = B0[ipc];
             const T bf = Bp[ipc];
-            const T aR = half * (B0[ipc + 1] + bc) * ipx;
-            const T aL = half * (bc + B0[ipc - 1]) * imx;
-            const T aT = half * (B0[ipc + PX] + bc) * ipy;
-            const T aB = half * (bc + B0[ipc - PX]) * imy;
-            const T aF = half * (bf + bc) * azp;
-            const T aK = half * (bc + H.bf) * azm;
+            const T aR = (B0[ipc + 1] + bc) * ipx;
+            const T aL = (bc + B0[ipc - 1]) * imx;
+            const T aT = (B0[ipc + PX] + bc) * ipy;
+            const T aB = (bc + B0[ipc - PX]) * imy;
+            const T aF = (bf + bc) * azp;
+            const T aK = (bc + H.bf) * azm;
             const T den = aR + aL + aT + aB + aF + aK;
             const T qc = Q0[iqc];
             T num;
@@ -287,7 +319,7 @@ __global__ void __launch_bounds__(NT)
                 num = -qc + (aR * P0[ipc + 1] + aL * P0[ipc - 1]) + (aT * P0[ipc + PX] + aB * P0[ipc - PX]) +
                       (aF * pf + aK * H.pf);
             }
-            red_c = num / den;
+            red_c = smooth_div(num, den);
             R0[iqc] = red_c;
             b_c = bc;
             b_f = bf;
@@ -299,24 +331,24 @@ __global__ void __launch_bounds__(NT)
             if (t < QX) {  // rows -1 and TY, every second cell
                 qrow = t < QX / 2 ? 0 : QY - 1;
                 const int hh = t < QX / 2 ? t : t - QX / 2;
-                xx = 2 * hh + ((tx0 + ty0 + qrow - 1 + kw) & 1);
+                xx = 2 * hh + ((tx0 + ty0 + qrow - 1 + k) & 1);
             } else if (t < QX + TY / 2) {  // column -1, every second row
                 xx = 0;
-                qrow = 1 + 2 * (t - QX) + ((tx0 + ty0 + kw) & 1);
+                qrow = 1 + 2 * (t - QX) + ((tx0 + ty0 + k) & 1);
             } else {  // column TX
                 xx = QX - 1;
-                qrow = 1 + 2 * (t - QX - TY / 2) + ((tx0 + TX + ty0 + kw + 1) & 1);
+                qrow = 1 + 2 * (t - QX - TY / 2) + ((tx0 + TX + ty0 + k + 1) & 1);
             }
             const int gyr = ty0 + qrow - 1;
             if (gyr < 0 || gyr >= ny) continue;
             const int ip = (qrow + 1) * PX + xx + 1;
             const T bc = B0[ip];
-            const T aR = half * (B0[ip + 1] + bc) * sTx[xx];
-            const T aL = half * (bc + B0[ip - 1]) * sTx[QX + xx];
-            const T aT = half * (B0[ip + PX] + bc) * sTy[qrow];
-            const T aB = half * (bc + B0[ip - PX]) * sTy[QY + qrow];
-            const T aF = half * (Bp[ip] + bc) * sTz[k - k0 + 1];
-            const T aK = half * (bc + Bm[ip]) * sTz[kZT + k - k0 + 1];
+            const T aR = (B0[ip + 1] + bc) * sTx[xx];
+            const T aL = (bc + B0[ip - 1]) * sTx[QX + xx];
+            const T aT = (B0[ip + PX] + bc) * sTy[qrow];
+            const T aB = (bc + B0[ip - PX]) * sTy[QY + qrow];
+            const T aF = (Bp[ip] + bc) * sTz[k - k0 + 1];
+            const T aK = (bc + Bm[ip]) * sTz[kZT + k - k0 + 1];
             const T den = aR + aL + aT + aB + aF + aK;
             const T qc = Q0[qrow * QX + xx];
             T num;
@@ -326,7 +358,7 @@ __global__ void __launch_bounds__(NT)
                 num = -qc + (aR * P0[ip + 1] + aL * P0[ip - 1]) + (aT * P0[ip + PX] + aB * P0[ip - PX]) +
                       (aF * Pp[ip] + aK * Pm[ip]);
             }
-            R0[qrow * QX + xx] = num / den;
+            R0[qrow * QX + xx] = smooth_div(num, den);
         }
         __syncthreads();
 
@@ -334,23 +366,23 @@ __global__ void __launch_bounds__(NT)
         if (k > k0 && k <= k1) {
             const int kb = k - 1;
             const T azp = sTz[kb - k0 + 1], azm = sTz[kZT + kb - k0 + 1];
-            const T* B1 = sB + slotB(kb) * PS;
-            const T* Q1 = sQ + slotQ(kb) * QS;
-            const T* R1 = sR + slotR(kb) * QS;
+            const T* B1 = sB + Sm * PS;
+            const T* Q1 = sQ + Sm * QS;
+            const T* R1 = sR + Sm * QS;
             const T bc = H.bf;  // beta(k-1) at this column
-            const T aR = half * (B1[ipc + 1] + bc) * ipx;
-            const T aL = half * (bc + B1[ipc - 1]) * imx;
-            const T aT = half * (B1[ipc + PX] + bc) * ipy;
-            const T aB = half * (bc + B1[ipc - PX]) * imy;
-            const T aF = half * (b_c + bc) * azp;  // beta(k)
-            const T aK = half * (bc + H.bc) * azm; // beta(k-2)
+            const T aR = (B1[ipc + 1] + bc) * ipx;
+            const T aL = (bc + B1[ipc - 1]) * imx;
+            const T aT = (B1[ipc + PX] + bc) * ipy;
+            const T aB = (bc + B1[ipc - PX]) * imy;
+            const T aF = (b_c + bc) * azp;  // beta(k)
+            const T aK = (bc + H.bc) * azm; // beta(k-2)
             const T den = aR + aL + aT + aB + aF + aK;
             const T self = FIRST ? (T)0 : p_self;
             const T pn = wall_n ? self : R1[iqc + QX];
             const T ps = wall_s ? self : R1[iqc - QX];
             const T num = -Q1[iqc] + (aR * R1[iqc + 1] + aL * R1[iqc - 1]) + (aT * pn + aB * ps) +
                           (aF * red_c + aK * H.r);
-            const T vb = num / den;
+            const T vb = smooth_div(num, den);
             const int par = xr - 2 * h;
             const T vr = R1[iqc + 1 - 2 * par];  // the red cell sharing the aligned pair
             typename Vec2<T>::type v;
@@ -380,11 +412,12 @@ __global__ void __launch_bounds__(NT)
         H.r = red_c;
     };
 
-    // steps k0-1 .. k1 in pairs (a final step k1+1 computes red cells only)
-    // steps k0-1 .. k1 in groups of PD (the steps past k1 compute red cells only)
-    for (int k = k0 - 1; k <= k1; k += PD) {
+    // steps k0-1 .. in groups of 4 (slot of step k0-1+4m+u: (u + 3) & 3)
+    for (int k = k0 - 1; k < k0 - 1 + nsteps; k += 4) {
 #pragma unroll
-        for (int u = 0; u < PD; ++u) step(k + u, rps[u], rbs[u], rqs[u], drs[u], dqs[u], (u & 1) ? hB : hA);
+        for (int u = 0; u < 4; ++u)
+            step(k + u, (u + 3) & 3, rps[u % PD], rbs[u % PD], rqs[u % PD], drs[u % PD], dqs[u % PD],
+                 (u & 1) ? hB : hA);
     }
     if (DOTS) block_partial<5>(acc, dots.red);
 }
@@ -402,7 +435,9 @@ constexpr SweepCfg kShapesD[] = {{64, 4, 128}, {64, 8, 256}, {32, 8, 128}};
 template <class T>
 static bool pick_shape(const LevelDev<T>& L, SweepCfg& out)
 {
-    if (L.nz < 2 || L.nz % 2 != 0 || L.ny % 2 != 0) return false;
+    // nz even (the colour of a plane is its index's parity), nz >= 16 (a
+    // chunk's plane steps wrap around z at most once)
+    if (L.nz < 16 || L.nz % 2 != 0 || L.ny % 2 != 0) return false;
     const SweepCfg* tab = sizeof(T) == 4 ? kShapesF : kShapesD;
     const int n = sizeof(T) == 4 ? (int)(sizeof kShapesF / sizeof kShapesF[0]) : (int)(sizeof kShapesD / sizeof kShapesD[0]);
     int want_tx = 0, want_ty = 0, want_nt = 0;
@@ -456,13 +491,11 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
     const int target = per_cu * (ncu > 0 ? ncu : 256);
     int chunks = (target + ntiles - 1) / ntiles;
     int kc = (L.nz + chunks - 1) / chunks;
-    if (kc < 4) kc = 4;
+    // kc + 2 plane steps run in groups of 4: kc = 4m + 2, from 6 to kSweepMaxChunk
+    kc = ((kc + 2 + 3) / 4) * 4 - 2;
+    if (kc < 6) kc = 6;
     if (kc > kSweepMaxChunk) kc = kSweepMaxChunk;
-    // kc + 2 plane steps run in groups of the prefetch distance: round kc up
-    // to waste none (while it stays within the LDS z tables)
     const int pd = (pdm == 4 && kc >= 16) ? 4 : 2;
-    const int kr = ((kc + 2 + pd - 1) / pd) * pd - 2;
-    if (kr <= kSweepMaxChunk) kc = kr;
     chunks = (L.nz + kc - 1) / kc;
     const unsigned nblocks = (unsigned)ntiles * (unsigned)chunks;
     if (DOTS && (size_t)nblocks * 5 > dots.capacity) {
