@@ -37,6 +37,7 @@ extern "C" {
 #define TBFG_ENOTCONVERGED 3 /* max iterations reached; result still written */
 #define TBFG_ENUMERIC 4      /* non-finite values in input or iteration      */
 #define TBFG_EDEVICE 5       /* HIP runtime error                            */
+#define TBFG_EIO 6           /* file missing, short or not writable          */
 
 #define TBFG_PREC_FP64 0  /* all arithmetic and storage in FP64                  */
 #define TBFG_PREC_MIXED 1 /* FP32 inner solver inside an FP64 outer iteration    */
@@ -205,6 +206,33 @@ int tbfg_project_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rh
  * multiplied by dt) in *out.  Blocking. */
 int tbfg_max_divergence_dev(const tbfg_grid_desc* grid, const double* ux, const double* uy, const double* uz,
                             double* out, void* stream);
+
+/* ------------------------------------------------------------- field files (host) */
+
+/* Field types of the reference (setIndexesBounds, field.f90:1746-1789): cell
+ * centres, and faces staggered in x, y, z (file range 0..n on that axis). */
+#define TBFG_FIELD_CL 0
+#define TBFG_FIELD_SX 1
+#define TBFG_FIELD_SY 2
+#define TBFG_FIELD_SZ 3
+
+/* Number of values a field file of this type holds for an n[0] x n[1] x n[2] grid. */
+int tbfg_field_count(int type, const int32_t n[3], int64_t* count);
+
+/* writeField (field.f90:453-476): the file holds the int32 count, then the FP64
+ * values of the type's index range, column-major (x fastest).  f is a host array
+ * in the engine layout: nx*ny*nz values (cl; sx and sz hold faces 1..n, whose
+ * periodic image is written as face 0), nx*(ny+1)*nz for sy (faces 0..ny). */
+int tbfg_field_write(const char* path, int type, const int32_t n[3], const double* f);
+
+/* readField / readInternalField (field.f90:331-346,419-443): reads such a file
+ * into the engine layout; a count other than the grid's is TBFG_EINVAL (the
+ * reference aborts: "Dimension of field ... not equal to mesh size"). */
+int tbfg_field_read(const char* path, int type, const int32_t n[3], double* f);
+
+/* The name of time folder n (writeTimeFolder, time.f90:317: format I6, adjusted
+ * left): "0", "12", ...  out receives at most len bytes including the NUL. */
+int tbfg_time_folder_name(int32_t n, char* out, int32_t len);
 
 /* --------------------------------------------------------- input mirrors (host) */
 

@@ -15,11 +15,12 @@ TBFG_ENOMEM = 2
 TBFG_ENOTCONVERGED = 3
 TBFG_ENUMERIC = 4
 TBFG_EDEVICE = 5
+TBFG_EIO = 6
 PREC_FP64 = 0
 PREC_MIXED = 1
 
 STATUS_NAMES = {0: "TBFG_OK", 1: "TBFG_EINVAL", 2: "TBFG_ENOMEM", 3: "TBFG_ENOTCONVERGED",
-                4: "TBFG_ENUMERIC", 5: "TBFG_EDEVICE"}
+                4: "TBFG_ENUMERIC", 5: "TBFG_EDEVICE", 6: "TBFG_EIO"}
 
 
 class TbfgError(RuntimeError):
@@ -92,6 +93,10 @@ SIGNATURES = {
                                        _P]),
     "tbfg_project_fast_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, _P, _P, _P, _P, _P, _P, _P,
                                         _P]),
+    "tbfg_field_count": (C.c_int, [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "tbfg_field_write": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int32), _D]),
+    "tbfg_field_read": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int32), _D]),
+    "tbfg_time_folder_name": (C.c_int, [C.c_int32, C.c_char_p, C.c_int32]),
     "tbfg_make_velocity": (C.c_int, [C.POINTER(GridDesc), C.c_uint64, _D, _D, _D]),
     "tbfg_make_bubble_density": (C.c_int, [C.POINTER(GridDesc), C.c_int32, C.c_double, C.c_uint64,
                                            C.c_int32, C.c_double, C.c_double, _D, _D]),

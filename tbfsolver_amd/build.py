@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtbfgpu.so")
 
-SOURCES = ["common.cpp", "inputs.cpp", "pcg.cpp", "abi.cpp", "mg_kernels.hip", "sweep.hip", "fields.hip", "fps.hip"]
+SOURCES = ["common.cpp", "inputs.cpp", "fieldio.cpp", "pcg.cpp", "abi.cpp", "mg_kernels.hip", "sweep.hip", "fields.hip",
+           "fps.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TBFG_ARCH", "gfx950")
