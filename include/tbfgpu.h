@@ -207,6 +207,16 @@ int tbfg_project_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rh
 int tbfg_max_divergence_dev(const tbfg_grid_desc* grid, const double* ux, const double* uy, const double* uz,
                             double* out, void* stream);
 
+/* updateMaterialProps + smoothVF (VOF.f90:197-262): two passes of cellToVertex /
+ * vertexToCell (interpolation.f90:188-276) smooth the volume fraction c into cs
+ * (ghosts: periodic x, z; the wall row in y), then rho = rho_g cs + (1-cs) rho_l
+ * and mu = mu_g cs + (1-cs) mu_l (mu may be NULL).  Device arrays of nx*ny*nz
+ * cells; work: tbfg_material_work_doubles(grid) doubles of device scratch
+ * (the vertex field).  c and cs must not alias. */
+int64_t tbfg_material_work_doubles(const tbfg_grid_desc* grid);
+int tbfg_material_props_dev(const tbfg_grid_desc* grid, double rho_l, double rho_g, double mu_l, double mu_g,
+                            const double* c, double* cs, double* rho, double* mu, double* work, void* stream);
+
 /* ------------------------------------------------------------- field files (host) */
 
 /* Field types of the reference (setIndexesBounds, field.f90:1746-1789): cell

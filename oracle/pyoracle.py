@@ -209,6 +209,15 @@ def source_fast(g, alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2):
     return s
 
 
+def material_props(g, c, rho_l=1.0, rho_g=1.0 / 20.0, mu_l=1.0, mu_g=1.0 / 50.0):
+    """(cs, rho, mu): the smoothed volume fraction and the material properties."""
+    c = _f(c)
+    cs, rho, mu = (np.zeros(shape(g), order="F") for _ in range(3))
+    lib().or_material_props(C.byref(g), C.c_double(rho_l), C.c_double(rho_g), C.c_double(mu_l), C.c_double(mu_g),
+                            _p(c), _p(cs), _p(rho), _p(mu))
+    return cs, rho, mu
+
+
 def volume_mean(g, p):
     p = _f(p)
     lib().or_volume_mean.restype = C.c_double

@@ -981,3 +981,81 @@ double or_max_abs(int64_t n, const double* a)
     }
     return m;
 }
+
+/* ------------------------------------------------------ material properties */
+
+/* cellToVertex (interpolation.f90:188-226): trilinear interpolation of a cell
+ * field to the vertices 0..n (faces) from the 8 surrounding centres, ghosts by
+ * periodic wrap in x, z and zero gradient in y (the VOF field's walls). */
+static void cell_to_vertex(const lev_t* L, const double* q, double* qv)
+{
+    const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
+    const ax_t *X = &L->ax[0], *Y = &L->ax[1], *Z = &L->ax[2];
+#define QG(i, j, k) q[IDX(wrapi(i, nx), clampi(j, ny), wrapi(k, nz))]
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                const double w1 = (X->c[i + 1] - X->f[i]) * (Y->f[j] - Y->c[j]) * (Z->c[k + 1] - Z->f[k]);
+                const double w2 = (X->f[i] - X->c[i]) * (Y->f[j] - Y->c[j]) * (Z->c[k + 1] - Z->f[k]);
+                const double w3 = (X->c[i + 1] - X->f[i]) * (Y->f[j] - Y->c[j]) * (Z->f[k] - Z->c[k]);
+                const double w4 = (X->f[i] - X->c[i]) * (Y->f[j] - Y->c[j]) * (Z->f[k] - Z->c[k]);
+                const double w5 = (X->c[i + 1] - X->f[i]) * (Y->c[j + 1] - Y->f[j]) * (Z->c[k + 1] - Z->f[k]);
+                const double w6 = (X->f[i] - X->c[i]) * (Y->c[j + 1] - Y->f[j]) * (Z->c[k + 1] - Z->f[k]);
+                const double w7 = (X->c[i + 1] - X->f[i]) * (Y->c[j + 1] - Y->f[j]) * (Z->f[k] - Z->c[k]);
+                const double w8 = (X->f[i] - X->c[i]) * (Y->c[j + 1] - Y->f[j]) * (Z->f[k] - Z->c[k]);
+                const double V = (X->c[i + 1] - X->c[i]) * (Y->c[j + 1] - Y->c[j]) * (Z->c[k + 1] - Z->c[k]);
+                qv[i + (size_t)(nx + 1) * (j + (size_t)(ny + 1) * k)] =
+                    (w1 * QG(i, j + 1, k) + w2 * QG(i + 1, j + 1, k) + w3 * QG(i, j + 1, k + 1) +
+                     w4 * QG(i + 1, j + 1, k + 1) + w5 * QG(i, j, k) + w6 * QG(i + 1, j, k) + w7 * QG(i, j, k + 1) +
+                     w8 * QG(i + 1, j, k + 1)) / V;
+            }
+#undef QG
+}
+
+/* vertexToCell (interpolation.f90:231-276): the cell value from its 8 vertices. */
+static void vertex_to_cell(const lev_t* L, const double* qv, double* q)
+{
+    const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
+    const ax_t *X = &L->ax[0], *Y = &L->ax[1], *Z = &L->ax[2];
+#define QV(i, j, k) qv[(i) + (size_t)(nx + 1) * ((j) + (size_t)(ny + 1) * (k))]
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                const double w1 = (X->f[i] - X->c[i]) * (Y->c[j] - Y->f[j - 1]) * (Z->f[k] - Z->c[k]);
+                const double w2 = (X->c[i] - X->f[i - 1]) * (Y->c[j] - Y->f[j - 1]) * (Z->f[k] - Z->c[k]);
+                const double w3 = (X->f[i] - X->c[i]) * (Y->c[j] - Y->f[j - 1]) * (Z->c[k] - Z->f[k - 1]);
+                const double w4 = (X->c[i] - X->f[i - 1]) * (Y->c[j] - Y->f[j - 1]) * (Z->c[k] - Z->f[k - 1]);
+                const double w5 = (X->f[i] - X->c[i]) * (Y->f[j] - Y->c[j]) * (Z->f[k] - Z->c[k]);
+                const double w6 = (X->c[i] - X->f[i - 1]) * (Y->f[j] - Y->c[j]) * (Z->f[k] - Z->c[k]);
+                const double w7 = (X->f[i] - X->c[i]) * (Y->f[j] - Y->c[j]) * (Z->c[k] - Z->f[k - 1]);
+                const double w8 = (X->c[i] - X->f[i - 1]) * (Y->f[j] - Y->c[j]) * (Z->c[k] - Z->f[k - 1]);
+                const double V = (X->f[i] - X->f[i - 1]) * (Y->f[j] - Y->f[j - 1]) * (Z->f[k] - Z->f[k - 1]);
+                q[IDX(i, j, k)] = (w1 * QV(i - 1, j, k - 1) + w2 * QV(i, j, k - 1) + w3 * QV(i - 1, j, k) +
+                                   w4 * QV(i, j, k) + w5 * QV(i - 1, j - 1, k - 1) + w6 * QV(i, j - 1, k - 1) +
+                                   w7 * QV(i - 1, j - 1, k) + w8 * QV(i, j - 1, k)) / V;
+            }
+#undef QV
+}
+
+/* updateMaterialProps + smoothVF (VOF.f90:197-262): two cell -> vertex -> cell
+ * passes of the volume fraction c give cs; rho = rho_g cs + (1 - cs) rho_l and
+ * mu likewise (mu may be NULL). */
+void or_material_props(const or_grid_desc* g, double rho_l, double rho_g, double mu_l, double mu_g,
+                       const double* c, double* cs, double* rho, double* mu)
+{
+    lev_t L;
+    lev_build(&L, g, 0);
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const size_t n = (size_t)nx * ny * nz;
+    double* qv = (double*)malloc(sizeof(double) * (size_t)(nx + 1) * (ny + 1) * (nz + 1));
+    cell_to_vertex(&L, c, qv);
+    vertex_to_cell(&L, qv, cs);
+    cell_to_vertex(&L, cs, qv);
+    vertex_to_cell(&L, qv, cs);
+    for (size_t t = 0; t < n; ++t) {
+        rho[t] = rho_g * cs[t] + (1.0 - cs[t]) * rho_l;
+        if (mu) mu[t] = mu_g * cs[t] + (1.0 - cs[t]) * mu_l;
+    }
+    free(qv);
+    lev_free(&L);
+}

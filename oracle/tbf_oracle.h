@@ -94,6 +94,8 @@ void or_source_fast(const or_grid_desc* g, double alpha_dt, double rho0, const d
                     const double* ux, const double* uy, const double* uz,
                     const double* psi1, const double* psi2, double* s);
 double or_max_abs(int64_t n, const double* a);
+void or_material_props(const or_grid_desc* g, double rho_l, double rho_g, double mu_l, double mu_g,
+                       const double* c, double* cs, double* rho, double* mu);
 double or_volume_mean(const or_grid_desc* g, const double* p);
 
 #ifdef __cplusplus

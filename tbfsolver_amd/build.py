@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtbfgpu.so")
 
 SOURCES = ["common.cpp", "inputs.cpp", "fieldio.cpp", "pcg.cpp", "abi.cpp", "mg_kernels.hip", "sweep.hip", "fields.hip",
-           "fps.hip"]
+           "fps.hip", "vof.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TBFG_ARCH", "gfx950")

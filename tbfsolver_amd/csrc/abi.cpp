@@ -342,6 +342,17 @@ int tbfg_max_divergence_dev(const tbfg_grid_desc* grid, const double* ux, const 
     ABI_GUARD(return max_divergence_dev(grid, ux, uy, uz, out, stream);)
 }
 
+int64_t tbfg_material_work_doubles(const tbfg_grid_desc* grid)
+{
+    return material_work_doubles(grid);
+}
+
+int tbfg_material_props_dev(const tbfg_grid_desc* grid, double rho_l, double rho_g, double mu_l, double mu_g,
+                            const double* c, double* cs, double* rho, double* mu, double* work, void* stream)
+{
+    ABI_GUARD(return material_props_dev(grid, rho_l, rho_g, mu_l, mu_g, c, cs, rho, mu, work, stream);)
+}
+
 // ---------------------------------------------------------------- inputs
 
 int tbfg_make_velocity(const tbfg_grid_desc* grid, uint64_t seed, double* ux, double* uy, double* uz)

@@ -17,4 +17,7 @@ int project_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, cons
                      const double* psi1, const double* psi2, double* ux, double* uy, double* uz, void* stream);
 int project_mg_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, const double* psi, double* ux,
                    double* uy, double* uz, void* stream);
+int64_t material_work_doubles(const tbfg_grid_desc* g);
+int material_props_dev(const tbfg_grid_desc* g, double rho_l, double rho_g, double mu_l, double mu_g, const double* c,
+                       double* cs, double* rho, double* mu, double* work, void* stream);
 }  // namespace tbfg

@@ -44,6 +44,13 @@ struct SweepDots {
 };
 template <class T>
 bool sweep_supported(const LevelDev<T>& L);
+// one full red/black sweep of a small level, out of place, from bricks staged
+// in LDS (levels whose dimensions the brick divides)
+template <class T>
+bool brick_supported(const LevelDev<T>& L);
+template <class T, class TQ>
+int launch_brick_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
+                       const int* done, void* stream);
 template <class T, class TQ>
 int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
                  const int* done, void* stream, const SweepDots* dots = nullptr, unsigned* nblocks_out = nullptr);

@@ -299,6 +299,10 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
             size_t min_cells = (size_t)1 << 20;
             if (const char* e = std::getenv("TBFG_SWEEP_MIN_CELLS")) min_cells = (size_t)std::atoll(e);
             if (n < min_cells || std::getenv("TBFG_HALF_SWEEPS")) h->fused[l] = 0;
+            // below that, one launch per sweep from bricks staged in LDS
+            if (!h->fused[l] && !std::getenv("TBFG_HALF_SWEEPS") && !std::getenv("TBFG_NO_BRICKS") &&
+                (h->mixed ? brick_supported<float>(h->tabf[l]->view) : brick_supported<double>(h->tabd[l]->view)))
+                h->fused[l] = 2;
             if (h->fused[l]) TBFG_TRY(h->palt[l].alloc(n * es));
         }
     }
@@ -404,7 +408,11 @@ static int smooth(tbfg_pcg* h, int l, const T* b, const T* q, T*& p, T*& alt, in
     const LevelDev<T>& L = geo<T>(h, l);
     for (int s = 0; s < nsweeps; ++s) {
         const bool f0 = first && s == 0;
-        if (h->fused[l]) {
+        if (h->fused[l] == 2) {
+            ProfScope ps(h, l == 0 ? 0 : -1, stream);
+            TBFG_TRY(launch_brick_sweep<T, T>(L, b, q, p, alt, f0, done, stream));
+            std::swap(p, alt);
+        } else if (h->fused[l]) {
             ProfScope ps(h, (l == 0 && !f0) ? 0 : -1, stream);
             const bool last = s == nsweeps - 1;
             TBFG_TRY(launch_sweep<T, T>(L, b, q, p, alt, f0, done, stream, last ? last_dots : nullptr,
@@ -429,7 +437,7 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
                   int dots_init = -1, bool* dots_done = nullptr)
 {
     if (dots_done) *dots_done = false;
-    const bool want_dots = dots_init >= 0 && h->fused[0] && h->prm.n_post > 0 && h->t0 > 0;
+    const bool want_dots = dots_init >= 0 && h->fused[0] == 1 && h->prm.n_post > 0 && h->t0 > 0;
     SweepDots sd;
     if (want_dots) {
         sd.r = h->r.as<double>();

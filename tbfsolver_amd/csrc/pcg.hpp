@@ -22,7 +22,7 @@ struct tbfg_pcg {
     // level 0: p = z, q = r (double), beta = caller's (FP64) or a float copy (mixed)
     std::vector<tbfg::DevBuf> p, q, beta;
     std::vector<tbfg::DevBuf> palt;     // second p buffer of the levels smoothed out of place
-    std::vector<char> fused;            // level smoothed by the out-of-place full-sweep kernel
+    std::vector<char> fused;            // level smoothed out of place, one launch per sweep: 1 plane march, 2 bricks
     bool fuse_dots = true;              // PCG dot products formed by the finest level's last sweep
     size_t tail_lds = 0;                // dynamic LDS of the LDS-resident tail kernel (0: HBM form)
     tbfg::OpPlan opp;                   // launch of the finest level's operator product

@@ -422,7 +422,189 @@ __global__ void __launch_bounds__(NT)
     if (DOTS) block_partial<5>(acc, dots.red);
 }
 
+// One full red/black sweep of a small level per launch (rbgs.f90:205-259),
+// out of place like k_sweep.  A workgroup stages a BX x BY x BZ brick of p and
+// beta with a two-cell ring and of q with a one-cell ring in LDS (a level this
+// small stays in L2, so the ring's re-reads cost little), computes the red
+// cells of the brick and its one-cell ring, then the black cells of the brick,
+// and writes the brick.  Ring cells are recomputed from pin by every workgroup
+// that needs them, so the launch equals the red and the black half-sweep
+// launches, with the same arithmetic as rbgs_cell.  Where a plane march would
+// take a few steps per workgroup (latency bound), all of a brick's planes load
+// at once.
+constexpr int kBrickThreads = 256;
+
+template <class T, class TQ, int BX, int BY, int BZ, bool FIRST>
+__global__ void __launch_bounds__(kBrickThreads)
+    k_brick_sweep(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q, const T* __restrict__ pin,
+                  T* __restrict__ pout, const int* done)
+{
+    constexpr int NT = kBrickThreads;
+    constexpr int PX = BX + 4, PY = BY + 4, PZ = BZ + 4, PS = PX * PY * PZ;
+    constexpr int QX = BX + 2, QY = BY + 2, QZ = BZ + 2, QS = QX * QY * QZ;
+    static_assert(BX % 2 == 0, "cell pairs along x");
+    __shared__ T sP[FIRST ? 1 : PS];
+    __shared__ T sB[PS];
+    __shared__ T sQ[QS];
+    __shared__ T sR[QS];  // new red values, brick + one-cell ring
+    __shared__ T sT[2 * (QX + QY + QZ)];  // face factors ip, im of the ring-1 rows/columns/planes
+
+    if (done && *done) return;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int nbx = nx / BX, nby = ny / BY;
+    const int bid = (int)blockIdx.x;
+    const int x0 = (bid % nbx) * BX, y0 = ((bid / nbx) % nby) * BY, z0 = (bid / (nbx * nby)) * BZ;
+    const size_t plane = (size_t)nx * ny;
+
+    // every load of the brick issued before the first LDS store (one wait)
+    constexpr int NPL = (PS + NT - 1) / NT, NQL = (QS + NT - 1) / NT;
+    T rb[NPL], rp[NPL], rq[NQL];
+#pragma unroll
+    for (int m = 0; m < NPL; ++m) {
+        const int t = min((int)threadIdx.x + m * NT, PS - 1);
+        const int xx = t % PX, r = t / PX, yy = r % PY, zz = r / PY;
+        const size_t g = plane * wrapi(z0 + zz - 2, nz) + (size_t)nx * clampi(y0 + yy - 2, ny) + wrapi(x0 + xx - 2, nx);
+        rb[m] = beta[g];
+        if (!FIRST) rp[m] = pin[g];
+    }
+#pragma unroll
+    for (int m = 0; m < NQL; ++m) {
+        const int t = min((int)threadIdx.x + m * NT, QS - 1);
+        const int xx = t % QX, r = t / QX, yy = r % QY, zz = r / QY;
+        rq[m] = (T)q[plane * wrapi(z0 + zz - 1, nz) + (size_t)nx * clampi(y0 + yy - 1, ny) + wrapi(x0 + xx - 1, nx)];
+    }
+#pragma unroll
+    for (int m = 0; m < NPL; ++m) {
+        const int t = min((int)threadIdx.x + m * NT, PS - 1);
+        sB[t] = rb[m];
+        if (!FIRST) sP[t] = rp[m];
+    }
+#pragma unroll
+    for (int m = 0; m < NQL; ++m) sQ[min((int)threadIdx.x + m * NT, QS - 1)] = rq[m];
+    for (int t = threadIdx.x; t < QX + QY + QZ; t += NT) {
+        const T *fp, *fm;
+        int g;
+        if (t < QX) {
+            fp = L.a[0].ip;
+            fm = L.a[0].im;
+            g = wrapi(x0 + t - 1, nx);
+        } else if (t < QX + QY) {
+            fp = L.a[1].ip;
+            fm = L.a[1].im;
+            g = clampi(y0 + t - QX - 1, ny);
+        } else {
+            fp = L.a[2].ip;
+            fm = L.a[2].im;
+            g = wrapi(z0 + t - QX - QY - 1, nz);
+        }
+        sT[t] = fp[g];
+        sT[QX + QY + QZ + t] = fm[g];
+    }
+    __syncthreads();
+    constexpr int TM = QX + QY + QZ;  // offset of the im factors
+    const T half = (T)0.5;
+
+    // red cells (gx + gy + gz odd) of the brick and its ring; ghost rows
+    // beyond the walls are not cells (their values are never read)
+    constexpr int NRED = (QX / 2) * QY * QZ;
+    for (int t = threadIdx.x; t < NRED; t += NT) {
+        const int hh = t % (QX / 2), r = t / (QX / 2), yy = r % QY, zz = r / QY;
+        const int gy = y0 + yy - 1, gz = z0 + zz - 1;
+        if (gy < 0 || gy >= ny) continue;
+        const int xx = 2 * hh + ((x0 + gy + gz) & 1);  // ring-1 coordinates (nx, nz even)
+        const int ip = ((zz + 1) * PY + yy + 1) * PX + xx + 1;
+        const T bc = sB[ip];
+        const T aR = half * (sB[ip + 1] + bc) * sT[xx];
+        const T aL = half * (bc + sB[ip - 1]) * sT[TM + xx];
+        const T aT = half * (sB[ip + PX] + bc) * sT[QX + yy];
+        const T aB = half * (bc + sB[ip - PX]) * sT[TM + QX + yy];
+        const T aF = half * (sB[ip + PX * PY] + bc) * sT[QX + QY + zz];
+        const T aK = half * (bc + sB[ip - PX * PY]) * sT[TM + QX + QY + zz];
+        const T den = aR + aL + aT + aB + aF + aK;
+        const T qc = sQ[(zz * QY + yy) * QX + xx];
+        T num;
+        if (FIRST) {
+            num = -qc;
+        } else {
+            // rows beyond the walls hold the wall row: the ghost is the cell itself
+            num = -qc + (aR * sP[ip + 1] + aL * sP[ip - 1]) + (aT * sP[ip + PX] + aB * sP[ip - PX]) +
+                  (aF * sP[ip + PX * PY] + aK * sP[ip - PX * PY]);
+        }
+        sR[(zz * QY + yy) * QX + xx] = smooth_div(num, den);
+    }
+    __syncthreads();
+
+    // black cells of the brick, written with their red neighbour in x as pairs
+    constexpr int NBLK = (BX / 2) * BY * BZ;
+    for (int t = threadIdx.x; t < NBLK; t += NT) {
+        const int hh = t % (BX / 2), r = t / (BX / 2), yy = r % BY, zz = r / BY;
+        const int gy = y0 + yy, gz = z0 + zz;
+        const int pr = (x0 + gy + gz + 1) & 1;  // red offset in the pair
+        const int xb = 2 * hh + (pr ^ 1);       // brick coordinates of the black cell
+        const int ip = ((zz + 2) * PY + yy + 2) * PX + xb + 2;
+        const int iq = ((zz + 1) * QY + yy + 1) * QX + xb + 1;
+        const int tx = xb + 1, ty = QX + yy + 1, tz = QX + QY + zz + 1;
+        const T bc = sB[ip];
+        const T aR = half * (sB[ip + 1] + bc) * sT[tx];
+        const T aL = half * (bc + sB[ip - 1]) * sT[TM + tx];
+        const T aT = half * (sB[ip + PX] + bc) * sT[ty];
+        const T aB = half * (bc + sB[ip - PX]) * sT[TM + ty];
+        const T aF = half * (sB[ip + PX * PY] + bc) * sT[tz];
+        const T aK = half * (bc + sB[ip - PX * PY]) * sT[TM + tz];
+        const T den = aR + aL + aT + aB + aF + aK;
+        const T self = FIRST ? (T)0 : sP[ip];
+        const T pn = gy == ny - 1 ? self : sR[iq + QX];
+        const T ps = gy == 0 ? self : sR[iq - QX];
+        const T num = -sQ[iq] + (aR * sR[iq + 1] + aL * sR[iq - 1]) + (aT * pn + aB * ps) +
+                      (aF * sR[iq + QX * QY] + aK * sR[iq - QX * QY]);
+        const T vb = smooth_div(num, den);
+        const T vr = sR[iq + 1 - 2 * (pr ^ 1)];  // the red cell of the pair
+        typename Vec2<T>::type v;
+        v.x = pr ? vb : vr;
+        v.y = pr ? vr : vb;
+        *reinterpret_cast<typename Vec2<T>::type*>(pout + plane * gz + (size_t)nx * gy + x0 + 2 * hh) = v;
+    }
+}
+
 }  // namespace
+
+template <class T>
+static void brick_dims(int& bx, int& by, int& bz)
+{
+    bx = 16;
+    by = 8;
+    bz = sizeof(T) == 4 ? 8 : 4;
+}
+
+template <class T>
+bool brick_supported(const LevelDev<T>& L)
+{
+    int bx, by, bz;
+    brick_dims<T>(bx, by, bz);
+    return L.nx % bx == 0 && L.ny % by == 0 && L.nz % bz == 0 && L.nz % 2 == 0;
+}
+
+template <class T, class TQ>
+int launch_brick_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
+                       const int* done, void* stream)
+{
+    if (!brick_supported(L)) {
+        set_error("launch_brick_sweep: level %dx%dx%d not supported", L.nx, L.ny, L.nz);
+        return TBFG_EINVAL;
+    }
+    int bx, by, bz;
+    brick_dims<T>(bx, by, bz);
+    const dim3 grid((unsigned)((L.nx / bx) * (L.ny / by) * (L.nz / bz)));
+    constexpr int BZ = sizeof(T) == 4 ? 8 : 4;
+    if (first)
+        hipLaunchKernelGGL((k_brick_sweep<T, TQ, 16, 8, BZ, true>), grid, dim3(kBrickThreads), 0, (hipStream_t)stream,
+                           L, beta, q, pin, pout, done);
+    else
+        hipLaunchKernelGGL((k_brick_sweep<T, TQ, 16, 8, BZ, false>), grid, dim3(kBrickThreads), 0,
+                           (hipStream_t)stream, L, beta, q, pin, pout, done);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
 
 struct SweepCfg {
     int tx, ty, nt;
@@ -558,6 +740,12 @@ int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin,
     return TBFG_EINVAL;
 }
 
+template bool brick_supported<float>(const LevelDev<float>&);
+template bool brick_supported<double>(const LevelDev<double>&);
+template int launch_brick_sweep<float, float>(const LevelDev<float>&, const float*, const float*, const float*,
+                                              float*, bool, const int*, void*);
+template int launch_brick_sweep<double, double>(const LevelDev<double>&, const double*, const double*,
+                                                const double*, double*, bool, const int*, void*);
 template bool sweep_supported<float>(const LevelDev<float>&);
 template bool sweep_supported<double>(const LevelDev<double>&);
 template int launch_sweep<float, float>(const LevelDev<float>&, const float*, const float*, const float*, float*,

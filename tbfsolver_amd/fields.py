@@ -53,6 +53,22 @@ def makeVelocityDivFreeFast(grid, alpha_dt, rho0, rho, psi, ux, uy, uz, psi1=Non
                                       stream_ptr(stream)))
 
 
+def updateMaterialProps(grid, c, cs, rho, mu=None, rho_l=1.0, rho_g=1.0 / 20.0, mu_l=1.0, mu_g=1.0 / 50.0,
+                        work=None, stream=None):
+    """updateMaterialProps (VOF.f90:197-229): cs = smoothVF(c), rho and mu from
+    cs.  work: a device tensor of tbfg_material_work_doubles doubles (allocated
+    here when None)."""
+    if work is None:
+        import torch
+
+        work = torch.empty(int(lib().tbfg_material_work_doubles(C.byref(grid))), dtype=torch.float64,
+                           device=c.device)
+    check(lib().tbfg_material_props_dev(C.byref(grid), float(rho_l), float(rho_g), float(mu_l), float(mu_g), ptr(c),
+                                        ptr(cs), ptr(rho), ptr(mu) if mu is not None else None, ptr(work),
+                                        stream_ptr(stream)))
+    return cs
+
+
 def max_divergence(grid, ux, uy, uz, stream=None):
     out = C.c_double(0.0)
     check(lib().tbfg_max_divergence_dev(C.byref(grid), ptr(ux), ptr(uy), ptr(uz), C.byref(out),

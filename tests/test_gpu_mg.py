@@ -118,6 +118,28 @@ def test_full_sweep_kernel_matches_half_sweeps(torch_gpu, monkeypatch, precision
     assert rel(zs[0], zs[1]) < tol
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+@pytest.mark.parametrize("n,nlev", [((64, 48, 64), 3), ((96, 32, 32), 2)])
+def test_brick_sweep_matches_half_sweeps(torch_gpu, monkeypatch, precision, tol, n, nlev):
+    """The brick full-sweep kernel (levels the 16x8x8 brick, 16x8x4 in FP64,
+    divides; the plane march switched off) gives the V-cycle of the two
+    half-sweep launches."""
+    print("case brick sweep vs half sweeps", precision, n, nlev, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 8, 1, 7)
+    monkeypatch.setenv("TBFG_SWEEP_MIN_CELLS", str(10 ** 12))
+    zs = []
+    for half in (False, True):
+        if half:
+            monkeypatch.setenv("TBFG_HALF_SWEEPS", "1")
+        s = T.PCG(g, nLevels=nlev, precision=precision)
+        z = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.precondition(to_dev(beta, torch_gpu), to_dev(q, torch_gpu), z)
+        zs.append(to_host(z))
+        s.close()
+    assert np.all(np.isfinite(zs[0]))
+    assert rel(zs[0], zs[1]) < tol
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("n", [(128, 96, 64), (96, 64, 48), (48, 40, 32)])
 def test_plane_marching_operator_matches_columns(torch_gpu, monkeypatch, precision, n):

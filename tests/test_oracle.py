@@ -418,3 +418,42 @@ def test_fast_source_variable_density_is_conservative():
     assert abs(np.sum(vol(g) * s)) < 1e-11 * np.abs(vol(g) * s).sum()
     s0 = O.source_fast(g, 1e-3, 1.0, np.ones(shape), z3, zy, z3, psi1, psi2)
     assert np.abs(s0).max() == 0
+
+
+# ---------------------------------------------------------------- material properties
+def _trilinear(go, a):
+    """a0 + a1 x + a2 y + a3 z + a4 xy + a5 yz + a6 xz + a7 xyz at the cell centres."""
+    n = O.shape(go)
+    xs = [O.axis_coords(n[d], go.L[d], go.stretch[d], go.gamma, 0)[0] for d in range(3)]
+    X, Y, Z = np.meshgrid(*xs, indexing="ij")
+    return a[0] + a[1] * X + a[2] * Y + a[3] * Z + a[4] * X * Y + a[5] * Y * Z + a[6] * X * Z + a[7] * X * Y * Z
+
+
+def test_material_props_constant_and_mixture():
+    go = O.desc((12, 10, 8), stretch=(0, 1, 0), gamma=1.5)
+    c = np.full(O.shape(go), 0.37)
+    cs, rho, mu = O.material_props(go, c, rho_l=1.0, rho_g=0.05, mu_l=2.0, mu_g=0.1)
+    assert np.allclose(cs, 0.37, rtol=0, atol=1e-15)  # the weights of both passes sum to the volume
+    assert np.allclose(rho, 0.05 * 0.37 + 0.63 * 1.0, rtol=1e-15)
+    assert np.allclose(mu, 0.1 * 0.37 + 0.63 * 2.0, rtol=1e-15)
+
+
+def test_material_props_reproduce_trilinear_fields_inside():
+    """Both interpolations are trilinear, so a trilinear field comes back
+    unchanged wherever the two passes see no ghost (stretched on every axis, so
+    a swapped weight or vertex shows)."""
+    go = O.desc((14, 12, 10), stretch=(1, 1, 1), gamma=1.3)
+    c = _trilinear(go, [0.3, 0.05, 0.1, -0.04, 0.02, -0.03, 0.01, 0.005])
+    cs, _, _ = O.material_props(go, c)
+    inner = (slice(2, -2),) * 3
+    assert np.abs(cs[inner] - c[inner]).max() < 1e-13
+    assert np.abs(cs - c).max() > 1e-4  # the ghosts (wrap, wall clamp) do change the border
+
+
+def test_material_props_wall_symmetry():
+    go = O.desc((8, 12, 6), stretch=(0, 1, 0), gamma=1.5)
+    c = np.random.default_rng(3).random(O.shape(go))
+    c = 0.5 * (c + c[:, ::-1, :])  # symmetric about the channel centre plane
+    cs, _, _ = O.material_props(go, c)
+    assert np.abs(cs - cs[:, ::-1, :]).max() < 1e-14
+    assert np.abs(cs - c).max() > 1e-3
