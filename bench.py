@@ -109,7 +109,7 @@ def batch_leg(torch, reps=3, workers=8):
             "precision": "mixed (FP32 FFT + FP64 refinement; FP32 multigrid in FP64 flexible PCG to 1e-8)"}
 
 
-def serving_leg(torch, rates=(50, 200, 500, 1000, 2000), workers=8, fp64_rates=(50, 200)):
+def serving_leg(torch, rates=(50, 200, 500, 1000, 2000), workers=8, fp64_rates=(50, 200, 500, 1000)):
     """Open-loop Poisson arrivals over 256 resident instances (config-3 sizes,
     70 % PCG + MG with bubbles, 30 % FFT), each request one projection step;
     8 worker streams with continuous batching, admission by HBM footprint under
@@ -275,7 +275,8 @@ def main():
         copy_gbs = copy_peak(torch)
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                           "kernel": "k_sweep<float> finest-level red/black sweep (out of place)",
+                           "kernel": "k_sweep<float, float, 64, 8, 256, false, false, 4>: finest-level red/black "
+                                     "sweep from a non-zero start, out of place (16 B per cell)",
                            "avg_us": round(avg_s * 1e6, 2), "launches": launches,
                            "algorithmic_bytes_per_launch": bytes_per,
                            "copy_kernel_gbs": round(copy_gbs, 1)}

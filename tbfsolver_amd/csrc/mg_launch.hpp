@@ -42,6 +42,11 @@ struct SweepDots {
     Reducer red{nullptr};
     size_t capacity = 0;  // doubles of red.partials
 };
+// events that a timed launch records around the kernel itself (the
+// dispatch's own begin and end, hipExtLaunchKernel)
+struct KernelEvents {
+    hipEvent_t start, stop;
+};
 template <class T>
 bool sweep_supported(const LevelDev<T>& L);
 // one full red/black sweep of a small level, out of place, from bricks staged
@@ -53,7 +58,8 @@ int launch_brick_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T
                        const int* done, void* stream);
 template <class T, class TQ>
 int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
-                 const int* done, void* stream, const SweepDots* dots = nullptr, unsigned* nblocks_out = nullptr);
+                 const int* done, void* stream, const SweepDots* dots = nullptr, unsigned* nblocks_out = nullptr,
+                 const KernelEvents* kev = nullptr);
 int launch_fin_dots(unsigned nblocks, size_t ncell, int init, PcgState* st, const Reducer& red, void* stream);
 template <class T, class TQ>
 int launch_resid_restrict(const LevelDev<T>& L, const T* beta, const TQ* q, const T* p, T* qc, const int* done,

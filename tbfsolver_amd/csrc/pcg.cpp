@@ -57,19 +57,23 @@ namespace tbfg {
 static constexpr int kProfPairs = 1024;
 
 // records an event pair around a launch of the profiled kernel family
+// ext: the launch records the pair itself around the kernel (events())
 struct ProfScope {
     tbfg_pcg* h;
     hipStream_t s;
-    bool on;
-    ProfScope(tbfg_pcg* h_, int kernel, void* stream) : h(h_), s((hipStream_t)stream)
+    bool on, ext;
+    KernelEvents kev{};
+    ProfScope(tbfg_pcg* h_, int kernel, void* stream, bool ext_ = false) : h(h_), s((hipStream_t)stream), ext(ext_)
     {
         on = kernel >= 0 && h->prof_kernel == kernel && h->prof_used < kProfPairs;
-        if (on) (void)hipEventRecord(h->prof_ev[2 * h->prof_used], s);
+        if (on && ext) kev = KernelEvents{h->prof_ev[2 * h->prof_used], h->prof_ev[2 * h->prof_used + 1]};
+        if (on && !ext) (void)hipEventRecord(h->prof_ev[2 * h->prof_used], s);
     }
+    const KernelEvents* events() const { return on && ext ? &kev : nullptr; }
     ~ProfScope()
     {
         if (on) {
-            (void)hipEventRecord(h->prof_ev[2 * h->prof_used + 1], s);
+            if (!ext) (void)hipEventRecord(h->prof_ev[2 * h->prof_used + 1], s);
             h->prof_used++;
         }
     }
@@ -413,10 +417,12 @@ static int smooth(tbfg_pcg* h, int l, const T* b, const T* q, T*& p, T*& alt, in
             TBFG_TRY(launch_brick_sweep<T, T>(L, b, q, p, alt, f0, done, stream));
             std::swap(p, alt);
         } else if (h->fused[l]) {
-            ProfScope ps(h, (l == 0 && !f0) ? 0 : -1, stream);
             const bool last = s == nsweeps - 1;
+            // the timed finest-level sweep: from a non-zero start, without the dot products
+            const bool plain = l == 0 && !f0 && !(last && last_dots);
+            ProfScope ps(h, plain ? 0 : -1, stream, true);
             TBFG_TRY(launch_sweep<T, T>(L, b, q, p, alt, f0, done, stream, last ? last_dots : nullptr,
-                                        last ? nb_out : nullptr));
+                                        last ? nb_out : nullptr, ps.events()));
             std::swap(p, alt);
         } else {
             ProfScope ps(h, l == 0 ? 0 : -1, stream);

@@ -21,6 +21,8 @@
 #include "reduce.hpp"
 #include "smooth.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <cstdio>
 #include <cstdlib>
 
@@ -654,7 +656,8 @@ constexpr int sweep_pd_max()
 
 template <class T, class TQ, int TX, int TY, int NT, bool FIRST, bool DOTS>
 static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout,
-                              const int* done, const SweepDots& dots, unsigned* nblocks_out, void* stream)
+                              const int* done, const SweepDots& dots, unsigned* nblocks_out, void* stream,
+                              const KernelEvents* kev)
 {
     constexpr int pdm = sweep_pd_max<T, DOTS>();
     static int per_cu = -1, ncu = 0;
@@ -684,12 +687,13 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
         set_error("launch_sweep: %u workgroups exceed the reduction scratch", nblocks);
         return TBFG_EINVAL;
     }
-    if (pd == 4)
-        hipLaunchKernelGGL((k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, pdm>), dim3(nblocks), dim3(NT), 0,
-                           (hipStream_t)stream, L, beta, q, pin, pout, kc, tiles_x, ntiles, done, dots);
+    auto k = pd == 4 ? k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, pdm> : k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, 2>;
+    if (kev)
+        hipExtLaunchKernelGGL(k, dim3(nblocks), dim3(NT), 0, (hipStream_t)stream, kev->start, kev->stop, 0, L, beta,
+                              q, pin, pout, kc, tiles_x, ntiles, done, dots);
     else
-        hipLaunchKernelGGL((k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, 2>), dim3(nblocks), dim3(NT), 0,
-                           (hipStream_t)stream, L, beta, q, pin, pout, kc, tiles_x, ntiles, done, dots);
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), 0, (hipStream_t)stream, L, beta, q, pin, pout, kc, tiles_x,
+                           ntiles, done, dots);
     TBFG_HIP(hipGetLastError());
     if (nblocks_out) *nblocks_out = nblocks;
     return TBFG_OK;
@@ -697,21 +701,23 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
 
 template <class T, class TQ, int TX, int TY, int NT>
 static int launch_sweep_first(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
-                              const int* done, const SweepDots* dots, unsigned* nblocks_out, void* stream)
+                              const int* done, const SweepDots* dots, unsigned* nblocks_out, void* stream,
+                              const KernelEvents* kev)
 {
     const SweepDots none{};
     if (dots)  // the dot products follow a post-smoothing sweep, never the first
         return launch_sweep_shape<T, TQ, TX, TY, NT, false, true>(L, beta, q, pin, pout, done, *dots, nblocks_out,
-                                                                  stream);
+                                                                  stream, kev);
     return first ? launch_sweep_shape<T, TQ, TX, TY, NT, true, false>(L, beta, q, pin, pout, done, none,
-                                                                      nblocks_out, stream)
+                                                                      nblocks_out, stream, kev)
                  : launch_sweep_shape<T, TQ, TX, TY, NT, false, false>(L, beta, q, pin, pout, done, none,
-                                                                       nblocks_out, stream);
+                                                                       nblocks_out, stream, kev);
 }
 
 template <class T, class TQ>
 int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
-                 const int* done, void* stream, const SweepDots* dots, unsigned* nblocks_out)
+                 const int* done, void* stream, const SweepDots* dots, unsigned* nblocks_out,
+                 const KernelEvents* kev)
 {
     if (dots && first) {
         set_error("launch_sweep: the fused dot products need a sweep from a non-zero start");
@@ -724,7 +730,7 @@ int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin,
     }
 #define TBFG_SWEEP_CASE(X, Y, N)                                                             \
     if (c.tx == X && c.ty == Y && c.nt == N)                                                \
-        return launch_sweep_first<T, TQ, X, Y, N>(L, beta, q, pin, pout, first, done, dots, nblocks_out, stream);
+        return launch_sweep_first<T, TQ, X, Y, N>(L, beta, q, pin, pout, first, done, dots, nblocks_out, stream, kev);
     if constexpr (sizeof(T) == 4) {
         TBFG_SWEEP_CASE(64, 8, 256)
         TBFG_SWEEP_CASE(64, 16, 512)
@@ -749,8 +755,9 @@ template int launch_brick_sweep<double, double>(const LevelDev<double>&, const d
 template bool sweep_supported<float>(const LevelDev<float>&);
 template bool sweep_supported<double>(const LevelDev<double>&);
 template int launch_sweep<float, float>(const LevelDev<float>&, const float*, const float*, const float*, float*,
-                                        bool, const int*, void*, const SweepDots*, unsigned*);
+                                        bool, const int*, void*, const SweepDots*, unsigned*, const KernelEvents*);
 template int launch_sweep<double, double>(const LevelDev<double>&, const double*, const double*, const double*,
-                                          double*, bool, const int*, void*, const SweepDots*, unsigned*);
+                                          double*, bool, const int*, void*, const SweepDots*, unsigned*,
+                                          const KernelEvents*);
 
 }  // namespace tbfg
