@@ -275,7 +275,7 @@ def main():
         copy_gbs = copy_peak(torch)
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                           "kernel": "k_sweep<float, float, 64, 8, 256, false, false, 4>: finest-level red/black "
+                           "kernel": "k_sweep<float, float, 32, 16, 256, false, false, 4>: finest-level red/black "
                                      "sweep from a non-zero start, out of place (16 B per cell)",
                            "avg_us": round(avg_s * 1e6, 2), "launches": launches,
                            "algorithmic_bytes_per_launch": bytes_per,
