@@ -129,7 +129,9 @@ __global__ void __launch_bounds__(NT)
     __shared__ T sP[(FIRST ? 1 : RING) * PS];
     __shared__ T sB[RING * PS];
     __shared__ T sQ[RING * QS];
-    __shared__ T sR[RING * QS];
+    // new red values: two slots (the black cells of step k read those of step
+    // k-1, and the slot is next written after the first barrier of step k+1)
+    __shared__ T sR[2 * QS];
     // face factors times 1/2 (exact), so a face coefficient is (beta + beta') * f
     __shared__ T sTx[2 * QX];    // x, tile columns -1..TX
     __shared__ T sTy[2 * QY];    // y, tile rows -1..TY
@@ -289,7 +291,7 @@ __global__ void __launch_bounds__(NT)
         const T* Pp = sP + SpP * PS;
         const T* Pm = sP + SmP * PS;
         const T* Q0 = sQ + S * QS;
-        T* R0 = sR + S * QS;
+        T* R0 = sR + (S & 1) * QS;
 
         // ---- red cell of plane k at this thread's column
         const int xr = colx(k);
@@ -370,7 +372,7 @@ __global__ void __launch_bounds__(NT)
             const T azp = sTz[kb - k0 + 1], azm = sTz[kZT + kb - k0 + 1];
             const T* B1 = sB + Sm * PS;
             const T* Q1 = sQ + Sm * QS;
-            const T* R1 = sR + Sm * QS;
+            const T* R1 = sR + (Sm & 1) * QS;
             const T bc = H.bf;  // beta(k-1) at this column
             const T aR = (B1[ipc + 1] + bc) * ipx;
             const T aL = (bc + B1[ipc - 1]) * imx;
