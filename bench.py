@@ -81,7 +81,12 @@ def batch_leg(torch, reps=3, workers=8):
     PCG + multigrid with 4..24 bubbles on odd ones), mixed precision, each solved
     once per pass; passes timed with 8 host threads on 8 streams and with one."""
     from tbfsolver_amd import batch as Bt
+    from tbfsolver_amd import tune as Tu
 
+    # tile shapes of the PCG+MG instances from the tuner's table for these sizes
+    # (python -m tbfsolver_amd.tune --config3 --out tbfsolver_amd/tuning/mi355x_config3_mixed.json)
+    table = os.path.join(ROOT, "tbfsolver_amd", "tuning", "mi355x_config3_mixed.json")
+    tuned = len(Tu.load(table)["entries"]) if os.path.exists(table) else 0
     t0 = time.perf_counter()
     specs = Bt.config3_instances(64, 1003)
     insts = [Bt.Instance(*x) for x in specs]
@@ -106,6 +111,8 @@ def batch_leg(torch, reps=3, workers=8):
             "ms_per_pass": round(conc * 1e3, 3), "streams": workers,
             "one_stream_solves_per_s": round(64 / seq, 2), "max_rel_error_vs_fp64_1e-12": max(errs),
             "error_sample": "every third instance", "setup_s": round(setup, 1),
+            "tile_shapes": "tuned per size (%d sizes, tbfsolver_amd/tuning/mi355x_config3_mixed.json)" % tuned
+            if tuned else "built-in order",
             "precision": "mixed (FP32 FFT + FP64 refinement; FP32 multigrid in FP64 flexible PCG to 1e-8)"}
 
 
@@ -293,6 +300,9 @@ def main():
 
     if not args.no_batch:
         out["config3"] = batch_leg(torch)
+        from tbfsolver_amd import tune as Tu
+
+        Tu.clear()
     if not args.no_serve:
         out["config4"] = serving_leg(torch)
 

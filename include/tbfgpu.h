@@ -217,6 +217,25 @@ int64_t tbfg_material_work_doubles(const tbfg_grid_desc* grid);
 int tbfg_material_props_dev(const tbfg_grid_desc* grid, double rho_l, double rho_g, double mu_l, double mu_g,
                             const double* c, double* cs, double* rho, double* mu, double* work, void* stream);
 
+/* ------------------------------------------------------------------- tuning table */
+
+/* No counterpart in the reference (its kernels are OpenMP loops).  The tile
+ * shape a kernel family launches with on a level of dims[3] cells in a
+ * precision of elem_bytes (4: the FP32 multigrid, 8: FP64), chosen by a tuner
+ * (tbfsolver_amd/tune.py) instead of the built-in preference order:
+ *   TBFG_TUNE_SWEEP       full sweep from a non-zero start   shape = {tx, ty, threads}
+ *   TBFG_TUNE_SWEEP_EDGE  first sweep / sweep with the dots  shape = {tx, ty, threads}
+ *   TBFG_TUNE_OP          FP64 operator product (level 0)    shape = {tx, threads, 0}, {-1, 0, 0} = columns
+ * A shape the level does not divide, or without a kernel, is ignored.  Sweeps
+ * read the table at every launch, the operator when a handle is created.
+ * shape {0, 0, 0} removes the entry; tbfg_tune_get returns {0, 0, 0} for none. */
+#define TBFG_TUNE_SWEEP 0
+#define TBFG_TUNE_SWEEP_EDGE 1
+#define TBFG_TUNE_OP 2
+int tbfg_tune_set(int32_t kind, const int32_t dims[3], int32_t elem_bytes, const int32_t shape[3]);
+int tbfg_tune_get(int32_t kind, const int32_t dims[3], int32_t elem_bytes, int32_t shape[3]);
+int tbfg_tune_clear(void);
+
 /* ------------------------------------------------------------- field files (host) */
 
 /* Field types of the reference (setIndexesBounds, field.f90:1746-1789): cell

@@ -96,6 +96,9 @@ SIGNATURES = {
     "tbfg_material_work_doubles": (C.c_int64, [C.POINTER(GridDesc)]),
     "tbfg_material_props_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, C.c_double, C.c_double, _P,
                                           _P, _P, _P, _P, _P]),
+    "tbfg_tune_set": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]),
+    "tbfg_tune_get": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]),
+    "tbfg_tune_clear": (C.c_int, []),
     "tbfg_field_count": (C.c_int, [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "tbfg_field_write": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int32), _D]),
     "tbfg_field_read": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int32), _D]),

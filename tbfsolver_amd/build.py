@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtbfgpu.so")
 
-SOURCES = ["common.cpp", "inputs.cpp", "fieldio.cpp", "pcg.cpp", "abi.cpp", "mg_kernels.hip", "sweep.hip", "fields.hip",
+SOURCES = ["common.cpp", "inputs.cpp", "fieldio.cpp", "tune.cpp", "pcg.cpp", "abi.cpp", "mg_kernels.hip", "sweep.hip", "fields.hip",
            "fps.hip", "vof.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

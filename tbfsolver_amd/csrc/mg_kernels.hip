@@ -7,6 +7,7 @@
 #include "mg_launch.hpp"
 #include "reduce.hpp"
 #include "smooth.hpp"
+#include "tune.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -1197,10 +1198,17 @@ OpPlan op_plan(const LevelDev<double>& L)
     OpPlan m;
     const char* env = std::getenv("TBFG_OP_TILE");
     std::vector<std::pair<int, int>> cand = {{128, 512}, {64, 512}, {32, 256}};
+    int tuned[3];
     if (env) {
         int a = 0, b = 0;
         if (std::sscanf(env, "%dx%d", &a, &b) != 2) return m;
         cand = {{a, b}};
+    } else if (tune_lookup(TBFG_TUNE_OP, L.nx, L.ny, L.nz, 8, tuned)) {
+        if (tuned[0] < 0) return m;  // columns
+        const std::pair<int, int> t{tuned[0], tuned[1]};
+        const bool known = t == std::pair<int, int>{128, 512} || t == std::pair<int, int>{64, 512} ||
+                           t == std::pair<int, int>{32, 256} || t == std::pair<int, int>{64, 256};
+        if (known) cand.insert(cand.begin(), t);
     }
     for (auto [tx, nt] : cand) {
         if (L.nx % tx == 0 && L.ny % (nt / tx) == 0 && L.nz >= 2) {
@@ -1222,14 +1230,17 @@ OpPlan op_plan(const LevelDev<double>& L)
     return m;
 }
 
-size_t reduction_slots(const LevelDev<double>& L)
+size_t reduction_slots(const LevelDev<double>& L, const OpPlan& plan)
 {
     const dim3 g3 = red_grid3(L, block2d(L.nx));
     const dim3 go = op_grid(L, op_block(L));
     const size_t chunked = (size_t)g3.x * g3.y * g3.z * 5;  // dots: 5 sums per block
     const size_t flat = (size_t)kRedBlocks * 5;
-    const size_t op = std::max((size_t)go.x * go.y * go.z, (size_t)op_plan(L).nblocks);
-    return std::max(std::max(chunked, flat), op);
+    const size_t op = std::max((size_t)go.x * go.y * go.z, (size_t)plan.nblocks);
+    // the sweep with the dot products on any tile shape: at most one workgroup
+    // per 32x8 tile and 6-plane chunk
+    const size_t sweep = (size_t)(L.nx / 32 + 1) * (L.ny / 8 + 1) * (L.nz / 6 + 1) * 5;
+    return std::max(std::max(chunked, flat), std::max(op, sweep));
 }
 
 int launch_op_dot(const LevelDev<double>& L, const OpPlan& m, const double* beta, const double* d, double* qv,

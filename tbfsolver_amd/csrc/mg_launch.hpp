@@ -4,8 +4,6 @@
 
 namespace tbfg {
 
-// doubles of reduction scratch the CG kernels of a level need
-size_t reduction_slots(const LevelDev<double>& L);
 
 // the operator product's launch: plane-marching tiles of tx x (nt / tx) cells
 // and kc planes, or (tx == 0: no tile divides the level) column segments.
@@ -15,6 +13,8 @@ struct OpPlan {
     unsigned nblocks = 0;
 };
 OpPlan op_plan(const LevelDev<double>& L);
+// doubles of reduction scratch the CG kernels of a level need (with its operator launch)
+size_t reduction_slots(const LevelDev<double>& L, const OpPlan& plan);
 
 int launch_apply_op(const LevelDev<double>& L, const double* beta, const double* p, double* q, void* stream);
 int launch_op_dot(const LevelDev<double>& L, const OpPlan& plan, const double* beta, const double* d, double* qv,

@@ -316,8 +316,8 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     TBFG_TRY(h->d.alloc(h->n0 * 8));
     TBFG_TRY(h->qv.alloc(h->n0 * 8));
     TBFG_TRY(h->st.alloc(sizeof(PcgState)));
-    TBFG_TRY(h->partials.alloc(reduction_slots(h->tabd[0]->view) * sizeof(double)));
     h->opp = op_plan(h->tabd[0]->view);
+    TBFG_TRY(h->partials.alloc(reduction_slots(h->tabd[0]->view, h->opp) * sizeof(double)));
     TBFG_TRY(h->counter.alloc(256));
     TBFG_HIP(hipMemset(h->counter.p, 0, 256));
     // dense bottom operator

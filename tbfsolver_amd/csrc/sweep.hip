@@ -20,6 +20,7 @@
 #include "mg_launch.hpp"
 #include "reduce.hpp"
 #include "smooth.hpp"
+#include "tune.hpp"
 
 #include <hip/hip_ext.h>
 
@@ -630,7 +631,14 @@ static bool pick_shape(const LevelDev<T>& L, SweepCfg& out, bool plain = false)
     const SweepCfg* tab = sizeof(T) == 4 ? (plain ? kShapesFPlain : kShapesF) : kShapesD;
     const int n = sizeof(T) == 4 ? (int)(sizeof kShapesF / sizeof kShapesF[0]) : (int)(sizeof kShapesD / sizeof kShapesD[0]);
     int want_tx = 0, want_ty = 0, want_nt = 0;
-    if (const char* e = std::getenv("TBFG_SWEEP_SHAPE")) std::sscanf(e, "%dx%dx%d", &want_tx, &want_ty, &want_nt);
+    int tuned[3];
+    if (const char* e = std::getenv("TBFG_SWEEP_SHAPE"))
+        std::sscanf(e, "%dx%dx%d", &want_tx, &want_ty, &want_nt);
+    else if (tune_lookup(plain ? TBFG_TUNE_SWEEP : TBFG_TUNE_SWEEP_EDGE, L.nx, L.ny, L.nz, (int)sizeof(T), tuned)) {
+        want_tx = tuned[0];
+        want_ty = tuned[1];
+        want_nt = tuned[2];
+    }
     for (int pass = 0; pass < 2; ++pass)
         for (int i = 0; i < n; ++i) {
             const SweepCfg c = tab[i];

@@ -162,6 +162,50 @@ def test_plane_marching_operator_matches_columns(torch_gpu, monkeypatch, precisi
     assert rel(out[0][1], out[1][1]) < 1e-9
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_tuned_tile_shapes(torch_gpu, precision):
+    """Tile shapes entered in the tuning table are the ones launched, and they
+    only regroup cells: the preconditioner agrees to round-off for every sweep
+    shape (a cell may be a tile cell in one shape and a recomputed ring cell in
+    another, whose products the compiler may fuse differently); an operator
+    tile changes the solve by its dot product's summation order only."""
+    from tbfsolver_amd import tune as Tu
+
+    n = (128, 96, 128)
+    print("case tuned shapes", precision, n, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 10, 1, 23)
+    dq, db = to_dev(q, torch_gpu), to_dev(beta, torch_gpu)
+    Tu.clear()
+    try:
+        zs = []
+        for shape in Tu.candidates("sweep", n, precision):
+            Tu.set_entry("sweep", n, precision, shape)
+            Tu.set_entry("sweep_edge", n, precision, shape)
+            s = T.PCG(g, nLevels=4, precision=precision, coarse_max_iter=0)
+            z = torch_gpu.zeros_like(dq)
+            s.precondition(db, dq, z)
+            zs.append(to_host(z))
+            s.close()
+        assert len(zs) >= 3
+        for z in zs[1:]:
+            assert rel(z, zs[0]) < (1e-6 if precision == "mixed" else 1e-13)
+        out = []
+        for shape in Tu.candidates("op", n, precision):
+            Tu.set_entry("op", n, precision, shape)
+            s = T.PCG(g, nLevels=4, tolPCG=1e-10, maxIterPCG=200, precision=precision, flexible=1,
+                      coarse_max_iter=0)
+            psi = torch_gpu.zeros_like(dq)
+            s.solvePCG(psi, db, dq)
+            assert s.converged
+            out.append(mean_free(to_host(psi)))
+            s.close()
+        assert len(out) >= 3
+        for p in out[1:]:
+            assert rel(p, out[0]) < 1e-9
+    finally:
+        Tu.clear()
+
+
 @pytest.mark.parametrize("precision,bound", [("fp64", 1e-10), ("mixed", 1e-9)])
 def test_dots_in_last_sweep_match_separate_kernel(torch_gpu, monkeypatch, precision, bound):
     """The PCG dot products formed by the finest level's last post-smoothing sweep
