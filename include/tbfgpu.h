@@ -113,7 +113,14 @@ typedef struct tbfg_pcg_params {
     int32_t flexible;        /* 0 Fletcher-Reeves, 1 Polak-Ribiere, -1 by precision  */
     double coarse_tol;       /* relative residual of the coarsest-level solve
                                 (0: 1e-10 in FP64, 1e-5 in mixed precision)          */
+    int32_t preconditioner;  /* TBFG_PRECOND_MG (the reference's multigrid) or
+                                TBFG_PRECOND_FFT: z = L^-1 r with L the constant-
+                                coefficient operator of the FFT solver (FP32 FFTs in
+                                mixed precision); x, z uniform, nx, nz 2^a 3^b      */
 } tbfg_pcg_params;
+
+#define TBFG_PRECOND_MG 0
+#define TBFG_PRECOND_FFT 1
 
 /* Fills *p with the reference's defaults for the channel case. */
 void tbfg_pcg_default_params(tbfg_pcg_params* p);

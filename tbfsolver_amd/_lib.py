@@ -54,7 +54,7 @@ class PcgParams(C.Structure):
     _fields_ = [("n_levels", C.c_int32), ("n_pre", C.c_int32), ("n_post", C.c_int32),
                 ("max_iter", C.c_int32), ("tol", C.c_double), ("mg_cycles", C.c_int32),
                 ("precision", C.c_int32), ("coarse_max_iter", C.c_int32), ("flexible", C.c_int32),
-                ("coarse_tol", C.c_double)]
+                ("coarse_tol", C.c_double), ("preconditioner", C.c_int32)]
 
 
 _P = C.c_void_p

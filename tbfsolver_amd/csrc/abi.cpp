@@ -195,6 +195,7 @@ void tbfg_pcg_default_params(tbfg_pcg_params* p)
     p->coarse_max_iter = -1;
     p->flexible = -1;
     p->coarse_tol = 0.0;
+    p->preconditioner = TBFG_PRECOND_MG;
 }
 
 int tbfg_pcg_create(const tbfg_grid_desc* grid, const tbfg_pcg_params* prm, tbfg_pcg** out)

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 
 #include "mg_launch.hpp"
@@ -48,6 +49,7 @@ tbfg_pcg::~tbfg_pcg()
     for (hipEvent_t e : hev)
         if (e) (void)hipEventDestroy(e);
     if (hstage) (void)hipHostFree(hstage);
+    if (fps) tbfg::fps_destroy(fps);
 }
 
 namespace tbfg {
@@ -205,6 +207,10 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         set_error("precision = %d", prm.precision);
         return TBFG_EINVAL;
     }
+    if (prm.preconditioner != TBFG_PRECOND_MG && prm.preconditioner != TBFG_PRECOND_FFT) {
+        set_error("preconditioner = %d", prm.preconditioner);
+        return TBFG_EINVAL;
+    }
     // multiGrid.f90:116-145: every axis divisible by 2^(nLevels-1)
     for (int a = 0; a < 3; ++a)
         if (g->n[a] % (1 << (prm.n_levels - 1)) != 0) {
@@ -312,6 +318,12 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     }
     if (h->mixed) TBFG_TRY(h->r32.alloc(h->n0 * 4));
     h->fuse_dots = !std::getenv("TBFG_UNFUSED_DOTS");
+    if (prm.preconditioner == TBFG_PRECOND_FFT) {
+        // FP32 transforms without refinement in mixed precision (a preconditioner)
+        tbfg_fps_opts fo{h->mixed ? TBFG_PREC_MIXED : TBFG_PREC_FP64, 0};
+        TBFG_TRY(fps_create(g, &fo, &h->fps));
+        TBFG_TRY(h->zfft.alloc(h->n0 * 8));
+    }
     TBFG_TRY(h->r.alloc(h->n0 * 8));
     TBFG_TRY(h->d.alloc(h->n0 * 8));
     TBFG_TRY(h->qv.alloc(h->n0 * 8));
@@ -373,7 +385,8 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     for (auto& t : h->tabf) fp += (int64_t)t->buf.bytes;
     fp += (int64_t)(h->r.bytes + h->r32.bytes + h->d.bytes + h->qv.bytes + h->st.bytes + h->partials.bytes + h->counter.bytes +
                     h->plan.bytes + h->G.bytes + h->dwork.bytes + h->vol.bytes + h->cx.bytes + h->cr.bytes +
-                    h->cd.bytes + h->cad.bytes + h->cvol.bytes);
+                    h->cd.bytes + h->cad.bytes + h->cvol.bytes + h->zfft.bytes);
+    if (h->fps) fp += fps_footprint(h->fps);
     h->footprint = fp;
     *out = h.release();
     return TBFG_OK;
@@ -492,10 +505,13 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
     return TBFG_OK;
 }
 
-template <class T>
+// T: the multigrid's precision; FFT: the preconditioner is the FFT solve of
+// the constant-coefficient operator (z in FP64) instead of the V-cycle
+template <class T, bool FFT = false>
 static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t* iters, double* res,
                    void* stream)
 {
+    using TZ = typename std::conditional<FFT, double, T>::type;
     hipStream_t s = (hipStream_t)stream;
     PcgState* st = h->st.as<PcgState>();
     const int* done = &st->done;
@@ -505,32 +521,43 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     h->hst->max_iter = h->prm.max_iter;
     h->hst->flexible = h->prm.flexible;
     TBFG_HIP(hipMemcpyAsync(st, h->hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
-    TBFG_TRY(setup_levels<T>(h, beta, stream));
+    if (!FFT) TBFG_TRY(setup_levels<T>(h, beta, stream));
     const T* beta0;
     if constexpr (sizeof(T) == 4)
         beta0 = h->beta[0].as<T>();
     else
         beta0 = beta;
-    T* z = nullptr;
+    TZ* z = nullptr;
     const LevelDev<double>& L0 = h->tabd[0]->view;
     const size_t n = h->n0;
     double* r = h->r.as<double>();
     double* d = h->d.as<double>();
     double* qv = h->qv.as<double>();
-    float* r32 = h->mixed ? h->r32.as<float>() : nullptr;
+    float* r32 = (h->mixed && !FFT) ? h->r32.as<float>() : nullptr;
     const T* r0;
     if constexpr (sizeof(T) == 4)
         r0 = r32;
     else
         r0 = r;
-    TBFG_TRY(launch_residual_init(L0, beta, q, psi, r, r32, st, red, stream));
+    // z = M r and the dot products of z (pcg.f90:118-139, 294-327)
     bool dots_done = false;
-    TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream, h->fuse_dots ? 1 : -1, &dots_done));
-    if (!dots_done) TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 1, st, red, stream));
+    auto precondition = [&](int init) -> int {
+        if constexpr (FFT) {
+            z = h->zfft.as<double>();
+            TBFG_TRY(fps_solve_dev(h->fps, r, z, stream));
+            TBFG_TRY(launch_pcg_dots<double>(n, r, z, qv, init, st, red, stream));
+        } else {
+            TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream, h->fuse_dots ? init : -1, &dots_done));
+            if (!dots_done) TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, init, st, red, stream));
+        }
+        return TBFG_OK;
+    };
+    TBFG_TRY(launch_residual_init(L0, beta, q, psi, r, r32, st, red, stream));
+    TBFG_TRY(precondition(1));
     // one iteration (pcg.f90:142-290): d = (z - m) + gamma d, qv = A d, alpha,
     // psi += alpha d, r -= alpha qv, the stop test, z = M r, the dot products
     auto iteration = [&](int init) -> int {
-        TBFG_TRY(launch_pcg_dir<T>(n, z, d, init, st, stream));
+        TBFG_TRY(launch_pcg_dir<TZ>(n, z, d, init, st, stream));
         {
             ProfScope ps(h, 2, stream);
             TBFG_TRY(launch_op_dot(L0, h->opp, beta, d, qv, st, red, stream));
@@ -539,8 +566,7 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
             ProfScope ps(h, 3, stream);
             TBFG_TRY(launch_pcg_update(n, psi, r, r32, d, qv, st, red, stream));
         }
-        TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream, h->fuse_dots ? 0 : -1, &dots_done));
-        if (!dots_done) TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, 0, st, red, stream));
+        TBFG_TRY(precondition(0));
         return TBFG_OK;
     };
     // copy of the device state after the work enqueued so far, and its event
@@ -584,6 +610,8 @@ int pcg_solve_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q,
         set_error("tbfg_pcg_solve_dev: NULL argument");
         return TBFG_EINVAL;
     }
+    if (h->fps)
+        return solve_t<double, true>(h, psi, beta, q, iters, res, stream);
     return h->mixed ? solve_t<float>(h, psi, beta, q, iters, res, stream)
                     : solve_t<double>(h, psi, beta, q, iters, res, stream);
 }
@@ -623,6 +651,7 @@ int pcg_precondition_dev(tbfg_pcg* h, const double* beta, const double* r, doubl
         set_error("tbfg_pcg_precondition_dev: NULL argument");
         return TBFG_EINVAL;
     }
+    if (h->fps) return fps_solve_dev(h->fps, r, z, stream);
     return h->mixed ? precond_t<float>(h, beta, r, z, stream) : precond_t<double>(h, beta, r, z, stream);
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <memory>
 
+#include "fps.hpp"
 #include "mg_launch.hpp"
 
 struct tbfg_pcg {
@@ -33,6 +34,8 @@ struct tbfg_pcg {
     tbfg::DevBuf G, dwork, vol;         // dense bottom operator and its set-up scratch
     tbfg::DevBuf cx, cr, cd, cad, cvol; // inner CG of the coarsest-level solve
     tbfg::DevBuf hpsi, hbeta, hq;       // device copies for the host-array entry point
+    tbfg_fps* fps = nullptr;            // FFT preconditioner (TBFG_PRECOND_FFT)
+    tbfg::DevBuf zfft;                  // its output z (FP64)
     tbfg::PcgState* hst = nullptr;      // pinned mirror of the device state (final, after a solve)
     tbfg::PcgState* hring = nullptr;    // pinned copies of the state after consecutive iterations
     hipEvent_t hev[2] = {nullptr, nullptr};

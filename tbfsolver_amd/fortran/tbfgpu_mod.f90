@@ -32,6 +32,7 @@ module tbfgpu_mod
         real(c_double) :: tol
         integer(c_int32_t) :: mg_cycles, precision, coarse_max_iter, flexible
         real(c_double) :: coarse_tol
+        integer(c_int32_t) :: preconditioner
     end type
 
     interface

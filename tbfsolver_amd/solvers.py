@@ -124,7 +124,8 @@ class PCG:
     `iterations` the number of direction updates."""
 
     def __init__(self, grid, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=100,
-                 precision="fp64", mg_cycles=1, coarse_max_iter=-1, coarse_tol=0.0, flexible=None):
+                 precision="fp64", mg_cycles=1, coarse_max_iter=-1, coarse_tol=0.0, flexible=None,
+                 preconditioner="mg"):
         # pcgCTOR + multiGridCTOR
         self.grid = grid
         p = default_params()
@@ -134,6 +135,9 @@ class PCG:
         p.coarse_max_iter = int(coarse_max_iter)
         p.coarse_tol = float(coarse_tol)
         p.flexible = -1 if flexible is None else int(flexible)
+        if preconditioner not in ("mg", "fft"):
+            raise ValueError("preconditioner must be 'mg' or 'fft'")
+        p.preconditioner = 1 if preconditioner == "fft" else 0
         self.params = p
         h = C.c_void_p()
         check(lib().tbfg_pcg_create(C.byref(grid), C.byref(p), C.byref(h)))

@@ -385,3 +385,52 @@ def test_config1_bubbles_128(torch_gpu):
         print("  ", precision, "iterations", s.iterations, "oracle", it, flush=True)
         assert s.converged
         assert rel(mean_free(to_host(psi)), mean_free(ref)) < bound
+
+
+@pytest.mark.parametrize("precision,max_it", [("fp64", 2), ("mixed", 4)])
+def test_fft_preconditioner_constant_density_is_exact(torch_gpu, precision, max_it):
+    """With constant density the variable-coefficient operator is beta times the
+    FFT solver's operator (same 3-point stencils and metrics), so the FFT
+    preconditioner is its inverse up to a scale: PCG stops after one or two
+    directions (a few in FP32)."""
+    n = (64, 48, 32)
+    print("case fft precond constant", precision, n, flush=True)
+    g, go = grids(n, 1)
+    ux, uy, uz = F.make_velocity(g, 13)
+    q = O.divergence(go, ux, uy, uz)
+    beta = np.full(n, 2.5e-3)
+    s = T.PCG(g, nLevels=3, tolPCG=1e-10, maxIterPCG=50, precision=precision, flexible=1, preconditioner="fft")
+    psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+    s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+    assert s.converged and s.iterations <= max_it, s.iterations
+    ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=3, tol=1e-12, flexible=1), beta, q)
+    assert rel(mean_free(to_host(psi)), mean_free(ref)) < 1e-8
+    s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fft_preconditioner_bubbles(torch_gpu, precision):
+    """Variable density (ratio 20): the FFT-preconditioned PCG converges to the
+    oracle's pressure (its iteration count depends on how much of the volume the
+    bubbles fill; here fewer than the multigrid's)."""
+    n = (64, 48, 64)
+    print("case fft precond bubbles", precision, n, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 6, 1, 29)
+    dq, db = to_dev(q, torch_gpu), to_dev(beta, torch_gpu)
+    s = T.PCG(g, nLevels=4, tolPCG=1e-10, maxIterPCG=400, precision=precision, flexible=1, preconditioner="fft")
+    psi = torch_gpu.zeros_like(dq)
+    s.solvePCG(psi, db, dq)
+    assert s.converged, (s.iterations, s.res_)
+    mg = T.PCG(g, nLevels=4, tolPCG=1e-10, maxIterPCG=400, precision=precision, flexible=1)
+    psi_mg = torch_gpu.zeros_like(dq)
+    mg.solvePCG(psi_mg, db, dq)
+    assert mg.converged
+    print("iterations fft %d mg %d" % (s.iterations, mg.iterations), flush=True)
+    ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=4, tol=1e-12, flexible=1), beta, q)
+    assert rel(mean_free(to_host(psi)), mean_free(ref)) < 1e-6
+    z = torch_gpu.zeros_like(dq)
+    s.precondition(db, dq, z)  # the entry point applies the FFT solve
+    torch_gpu.cuda.synchronize()
+    assert np.all(np.isfinite(to_host(z)))
+    s.close()
+    mg.close()
