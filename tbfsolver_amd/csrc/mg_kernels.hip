@@ -175,10 +175,27 @@ struct OpTile {
     static constexpr int ON = (OS + NT - 1) / NT;
 };
 
-template <int TX, int NT, int PD>
+// With DIR (an iteration's direction, operator product and the previous
+// iteration's pressure update in one pass): the staged plane is the new
+// direction d = (z - m) + gamma d_old (pcg.f90:266-290; d_old from the other
+// buffer of a pair, so no workgroup overwrites what another one reads), made
+// from z and d_old as the plane is stashed, ring included; the owner of a cell
+// writes d and psi += alpha_prev d_old (pcg.f90:228-237 of the iteration
+// before, deferred to here).  init: the first direction d = z - m, no psi.
+template <class TZ>
+struct OpDir {
+    const TZ* z;
+    const double* dold;
+    double* dnew;
+    double* psi;
+    int init;
+};
+
+template <int TX, int NT, int PD, class TZ, bool DIR>
 __global__ void __launch_bounds__(NT) k_op_march(LevelDev<double> L, const double* __restrict__ beta,
                                                   const double* __restrict__ d, double* __restrict__ qv,
-                                                  PcgState* st, Reducer red, int kc, int tiles_x, int ntiles)
+                                                  PcgState* st, Reducer red, int kc, int tiles_x, int ntiles,
+                                                  OpDir<TZ> od)
 {
     using Sh = OpTile<TX, NT>;
     constexpr int TY = Sh::TY, OX = Sh::OX, OS = Sh::OS, ON = Sh::ON;
@@ -210,23 +227,44 @@ __global__ void __launch_bounds__(NT) k_op_march(LevelDev<double> L, const doubl
         off[m] = nx * (jj < 0 ? 0 : (jj >= ny ? ny - 1 : jj)) + (ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii));
     }
     const size_t plane = (size_t)nx * ny;
+    const size_t own = (size_t)nx * gy + gx;
+    // the iteration's scalars (written by the kernels before this one)
+    const double zmean = DIR ? st->zmean : 0.0, gamma = DIR ? st->gamma : 0.0, alpha = DIR ? st->alpha : 0.0;
+    const bool init = DIR && od.init;
     auto modz = [&](int k) { return ((k % nz) + nz) % nz; };
     auto slot = [&](int k) { return (k - k0 + 4) & 3; };
-    auto fetch = [&](double(&rd)[ON], double(&rb)[ON], int kp) {
+    // a register set holds one plane: d (or z and d_old) and beta; with DIR
+    // also psi and d_old of this thread's own cell on the plane before (o2)
+    auto fetch = [&](double(&rd)[ON], TZ(&rz)[ON], double(&rb)[ON], double(&o2)[2], int kp) {
         const size_t o = plane * modz(kp);
 #pragma unroll
         for (int m = 0; m < ON; ++m) {
-            rd[m] = d[o + off[m]];
+            if (DIR) {
+                rz[m] = od.z[o + off[m]];
+                rd[m] = init ? 0.0 : od.dold[o + off[m]];
+            } else {
+                rd[m] = d[o + off[m]];
+            }
             rb[m] = beta[o + off[m]];
         }
+        if (DIR) {
+            const size_t oo = plane * modz(kp - 1) + own;
+            o2[0] = init ? 0.0 : od.psi[oo];
+            o2[1] = init ? 0.0 : od.dold[oo];
+        }
     };
-    auto stash = [&](const double(&rd)[ON], const double(&rb)[ON], int k) {
+    auto stash = [&](const double(&rd)[ON], const TZ(&rz)[ON], const double(&rb)[ON], int k) {
         double* D = sD + slot(k) * OS;
         double* B = sB + slot(k) * OS;
 #pragma unroll
         for (int m = 0; m < ON; ++m) {
             const int t = min((int)threadIdx.x + m * NT, OS - 1);
-            D[t] = rd[m];
+            double dv = rd[m];
+            if (DIR) {  // k_pcg_dir's arithmetic
+                const double zc = (double)rz[m] - zmean;
+                dv = init ? zc : zc + gamma * rd[m];
+            }
+            D[t] = dv;
             B[t] = rb[m];
         }
     };
@@ -236,22 +274,25 @@ __global__ void __launch_bounds__(NT) k_op_march(LevelDev<double> L, const doubl
         const double ipy = L.a[1].ip[gy], imy = L.a[1].im[gy];
         const int ic = (ty + 1) * OX + tx + 1;
         {
-            double rd[ON], rb[ON];
-            fetch(rd, rb, k0 - 1);
-            stash(rd, rb, k0 - 1);
-            fetch(rd, rb, k0);
-            stash(rd, rb, k0);
+            double rd[ON], rb[ON], o2[2];
+            TZ rz[ON];
+            fetch(rd, rz, rb, o2, k0 - 1);
+            stash(rd, rz, rb, k0 - 1);
+            fetch(rd, rz, rb, o2, k0);
+            stash(rd, rz, rb, k0);
         }
-        double rds[PD][ON], rbs[PD][ON];
+        double rds[PD][ON], rbs[PD][ON], o2s[PD][2];
+        TZ rzs[PD][ON];
 #pragma unroll
         for (int u = 0; u < PD; ++u) {
-            fetch(rds[u], rbs[u], k0 + 1 + u);
+            fetch(rds[u], rzs[u], rbs[u], o2s[u], k0 + 1 + u);
             __builtin_amdgcn_sched_barrier(0);
         }
-        double* out = qv + (size_t)nx * gy + gx;
-        auto step = [&](int k, double(&rd)[ON], double(&rb)[ON]) {
-            stash(rd, rb, k + 1);
-            fetch(rd, rb, k + 1 + PD);  // (past the chunk end: loaded, not used)
+        double* out = qv + own;
+        auto step = [&](int k, double(&rd)[ON], TZ(&rz)[ON], double(&rb)[ON], double(&o2)[2]) {
+            stash(rd, rz, rb, k + 1);
+            const double opsi = o2[0], odold = o2[1];  // the own cell on plane k
+            fetch(rd, rz, rb, o2, k + 1 + PD);  // (past the chunk end: loaded, not used)
             __syncthreads();
             if (k < k1) {
                 const double* D0 = sD + slot(k) * OS;
@@ -273,14 +314,31 @@ __global__ void __launch_bounds__(NT) k_op_march(LevelDev<double> L, const doubl
                                  aB * (dc - D0[ic - OX]) + aF * (Dp[ic] - dc) - aK * (dc - Dm[ic]);
                 out[plane * k] = v;
                 acc[0] += dc * v;
+                if (DIR) {
+                    od.dnew[plane * k + own] = dc;
+                    if (!init) od.psi[plane * k + own] = opsi + alpha * odold;  // k_pcg_update's arithmetic
+                }
             }
         };
         for (int k = k0; k < k1; k += PD) {
 #pragma unroll
-            for (int u = 0; u < PD; ++u) step(k + u, rds[u], rbs[u]);
+            for (int u = 0; u < PD; ++u) step(k + u, rds[u], rzs[u], rbs[u], o2s[u]);
         }
     }
     block_partial<1>(acc, red);
+}
+
+// psi += alpha d of the last iteration that ran (its direction is in buffer
+// iter & 1 of the pair); nothing after a non-finite value or before any iteration
+__global__ void k_psi_last(size_t ncell, double* __restrict__ psi, const double* __restrict__ d0,
+                           const double* __restrict__ d1, const PcgState* st)
+{
+    const int it = st->iter;
+    if (it == 0 || st->status == TBFG_ENUMERIC) return;
+    const double alpha = st->alpha;
+    const double* d = (it & 1) ? d1 : d0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += stride) psi[t] = psi[t] + alpha * d[t];
 }
 
 __global__ void k_fin_alpha(unsigned nblocks, PcgState* st, Reducer red)
@@ -360,7 +418,7 @@ __global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __r
     double v[1] = {0.0};
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += stride) {
-        psi[t] = psi[t] + alpha * d[t];
+        if (psi) psi[t] = psi[t] + alpha * d[t];  // (NULL: deferred to the next operator pass)
         const double rn = r[t] - alpha * qv[t];
         r[t] = rn;
         if (r32) r32[t] = (float)rn;
@@ -1249,8 +1307,8 @@ int launch_op_dot(const LevelDev<double>& L, const OpPlan& m, const double* beta
     if (m.tx) {
 #define TBFG_OPM(TX_, NT_)                                                                                          \
     if (m.tx == TX_ && m.nt == NT_)                                                                                 \
-        hipLaunchKernelGGL((k_op_march<TX_, NT_, 4>), dim3(m.nblocks), dim3(NT_), 0, S(stream), L, beta, d, qv, st, \
-                           red, m.kc, m.tiles_x, m.ntiles);
+        hipLaunchKernelGGL((k_op_march<TX_, NT_, 4, double, false>), dim3(m.nblocks), dim3(NT_), 0, S(stream), L,   \
+                           beta, d, qv, st, red, m.kc, m.tiles_x, m.ntiles, OpDir<double>{});
         TBFG_OPM(128, 512) else TBFG_OPM(64, 512) else TBFG_OPM(32, 256) else TBFG_OPM(64, 256) else {
             set_error("no operator tile %dx%d", m.tx, m.nt);
             return TBFG_EINVAL;
@@ -1265,6 +1323,36 @@ int launch_op_dot(const LevelDev<double>& L, const OpPlan& m, const double* beta
     const unsigned nb = grd.x * grd.y * grd.z;
     hipLaunchKernelGGL(k_op_dot<kOpPlanes>, dim3(nb), blk, 0, S(stream), L, beta, d, qv, st, red);
     hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, S(stream), nb, st, red);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+template <class TZ>
+int launch_op_dir(const LevelDev<double>& L, const OpPlan& m, const double* beta, const TZ* z, const double* dold,
+                  double* dnew, double* psi, int init, double* qv, PcgState* st, const Reducer& red, void* stream)
+{
+    const OpDir<TZ> od{z, dold, dnew, psi, init};
+#define TBFG_OPD(TX_, NT_)                                                                                         \
+    if (m.tx == TX_ && m.nt == NT_)                                                                                \
+        hipLaunchKernelGGL((k_op_march<TX_, NT_, 4, TZ, true>), dim3(m.nblocks), dim3(NT_), 0, S(stream), L, beta, \
+                           nullptr, qv, st, red, m.kc, m.tiles_x, m.ntiles, od);
+    TBFG_OPD(128, 512) else TBFG_OPD(64, 512) else TBFG_OPD(32, 256) else TBFG_OPD(64, 256) else {
+        set_error("launch_op_dir: no operator tile %dx%d", m.tx, m.nt);
+        return TBFG_EINVAL;
+    }
+#undef TBFG_OPD
+    hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, S(stream), m.nblocks, st, red);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+template int launch_op_dir<float>(const LevelDev<double>&, const OpPlan&, const double*, const float*, const double*,
+                                  double*, double*, int, double*, PcgState*, const Reducer&, void*);
+template int launch_op_dir<double>(const LevelDev<double>&, const OpPlan&, const double*, const double*,
+                                   const double*, double*, double*, int, double*, PcgState*, const Reducer&, void*);
+
+int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, const PcgState* st, void* stream)
+{
+    hipLaunchKernelGGL(k_psi_last, dim3(red_blocks1(n)), dim3(256), 0, S(stream), n, psi, d0, d1, st);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }

@@ -19,6 +19,13 @@ size_t reduction_slots(const LevelDev<double>& L, const OpPlan& plan);
 int launch_apply_op(const LevelDev<double>& L, const double* beta, const double* p, double* q, void* stream);
 int launch_op_dot(const LevelDev<double>& L, const OpPlan& plan, const double* beta, const double* d, double* qv,
                   PcgState* st, const Reducer& red, void* stream);
+// the direction d = (z - m) + gamma d_old into dnew, qv = A d, d.qv and psi +=
+// alpha_prev d_old in one pass (plane-marching tiles: plan.tx != 0)
+template <class TZ>
+int launch_op_dir(const LevelDev<double>& L, const OpPlan& plan, const double* beta, const TZ* z, const double* dold,
+                  double* dnew, double* psi, int init, double* qv, PcgState* st, const Reducer& red, void* stream);
+// psi += alpha d of the last iteration that ran (d in d0 or d1 by its parity)
+int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, const PcgState* st, void* stream);
 int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
                          double* r, float* r32, PcgState* st, const Reducer& red, void* stream);
 int launch_pcg_update(size_t n, double* psi, double* r, float* r32, const double* d, const double* qv,

@@ -125,8 +125,12 @@ int pcg_profile_end(tbfg_pcg* h, int64_t* launches, double* total_ms, double* by
     switch (h->prof_kernel) {
         case 0: bytes = 4.0 * bp * n; break;                               // full sweep: p, beta, q in; p out
         case 1: bytes = 3.0 * bp * n + bp * n / 8; break;                  // p, beta, q in; coarse q out
-        case 2: bytes = 24.0 * n; break;                                   // d, beta in; qv out
-        case 3: bytes = 48.0 * n + (h->mixed ? 4.0 * n : 0.0); break;      // psi, r, d, qv in; psi, r (+ FP32 r) out
+        case 2:  // d, beta in; qv out (fused: z, d_old, beta, psi in; d, qv, psi out)
+            bytes = h->fuse_op ? (48.0 + (h->mixed && !h->fps ? 4.0 : 8.0)) * n : 24.0 * n;
+            break;
+        case 3:  // psi, r, d, qv in; psi, r (+ FP32 r) out (fused: r, qv in; r (+ FP32 r) out)
+            bytes = (h->fuse_op ? 24.0 * n : 48.0 * n) + (h->mixed && !h->fps ? 4.0 * n : 0.0);
+            break;
     }
     if (launches) *launches = h->prof_launches;
     if (total_ms) *total_ms = h->prof_ms;
@@ -329,6 +333,8 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     TBFG_TRY(h->qv.alloc(h->n0 * 8));
     TBFG_TRY(h->st.alloc(sizeof(PcgState)));
     h->opp = op_plan(h->tabd[0]->view);
+    h->fuse_op = h->opp.tx != 0 && !std::getenv("TBFG_UNFUSED_OP");
+    if (h->fuse_op) TBFG_TRY(h->d2.alloc(h->n0 * 8));
     TBFG_TRY(h->partials.alloc(reduction_slots(h->tabd[0]->view, h->opp) * sizeof(double)));
     TBFG_TRY(h->counter.alloc(256));
     TBFG_HIP(hipMemset(h->counter.p, 0, 256));
@@ -385,7 +391,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     for (auto& t : h->tabf) fp += (int64_t)t->buf.bytes;
     fp += (int64_t)(h->r.bytes + h->r32.bytes + h->d.bytes + h->qv.bytes + h->st.bytes + h->partials.bytes + h->counter.bytes +
                     h->plan.bytes + h->G.bytes + h->dwork.bytes + h->vol.bytes + h->cx.bytes + h->cr.bytes +
-                    h->cd.bytes + h->cad.bytes + h->cvol.bytes + h->zfft.bytes);
+                    h->cd.bytes + h->cad.bytes + h->cvol.bytes + h->zfft.bytes + h->d2.bytes);
     if (h->fps) fp += fps_footprint(h->fps);
     h->footprint = fp;
     *out = h.release();
@@ -555,14 +561,29 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     TBFG_TRY(launch_residual_init(L0, beta, q, psi, r, r32, st, red, stream));
     TBFG_TRY(precondition(1));
     // one iteration (pcg.f90:142-290): d = (z - m) + gamma d, qv = A d, alpha,
-    // psi += alpha d, r -= alpha qv, the stop test, z = M r, the dot products
+    // psi += alpha d, r -= alpha qv, the stop test, z = M r, the dot products.
+    // Fused form: iteration j writes its direction to dbuf[j & 1] from
+    // dbuf[(j - 1) & 1] in the operator pass, which also adds the previous
+    // iteration's alpha d to psi; the last one is added after the loop.
+    double* dbuf[2] = {d, h->fuse_op ? h->d2.as<double>() : d};
+    int j = 0;
     auto iteration = [&](int init) -> int {
-        TBFG_TRY(launch_pcg_dir<TZ>(n, z, d, init, st, stream));
-        {
-            ProfScope ps(h, 2, stream);
-            TBFG_TRY(launch_op_dot(L0, h->opp, beta, d, qv, st, red, stream));
-        }
-        {
+        ++j;
+        if (h->fuse_op) {
+            double* dj = dbuf[j & 1];
+            {
+                ProfScope ps(h, 2, stream);
+                TBFG_TRY(launch_op_dir<TZ>(L0, h->opp, beta, z, dbuf[(j - 1) & 1], dj, psi, init, qv, st, red,
+                                           stream));
+            }
+            ProfScope ps(h, 3, stream);
+            TBFG_TRY(launch_pcg_update(n, nullptr, r, r32, dj, qv, st, red, stream));
+        } else {
+            TBFG_TRY(launch_pcg_dir<TZ>(n, z, d, init, st, stream));
+            {
+                ProfScope ps(h, 2, stream);
+                TBFG_TRY(launch_op_dot(L0, h->opp, beta, d, qv, st, red, stream));
+            }
             ProfScope ps(h, 3, stream);
             TBFG_TRY(launch_pcg_update(n, psi, r, r32, d, qv, st, red, stream));
         }
@@ -587,6 +608,7 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
         TBFG_TRY(iteration(0));
         TBFG_TRY(snapshot(slot));
     }
+    if (h->fuse_op) TBFG_TRY(launch_psi_last(n, psi, dbuf[0], dbuf[1], st, stream));
     TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     TBFG_HIP(hipStreamSynchronize(s));
     if (h->prof_kernel >= 0) prof_harvest(h);

@@ -27,7 +27,8 @@ struct tbfg_pcg {
     bool fuse_dots = true;              // PCG dot products formed by the finest level's last sweep
     size_t tail_lds = 0;                // dynamic LDS of the LDS-resident tail kernel (0: HBM form)
     tbfg::OpPlan opp;                   // launch of the finest level's operator product
-    tbfg::DevBuf r, d, qv;              // CG vectors (FP64)
+    bool fuse_op = false;               // direction + operator + deferred psi update in one pass (d2: the pair)
+    tbfg::DevBuf r, d, d2, qv;          // CG vectors (FP64)
     tbfg::DevBuf r32;                   // FP32 copy of r (mixed precision: the multigrid's right-hand side)
     tbfg::DevBuf st, partials, counter; // PcgState + reduction scratch
     tbfg::DevBuf plan;                  // TailPlan<T>

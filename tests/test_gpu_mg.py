@@ -434,3 +434,28 @@ def test_fft_preconditioner_bubbles(torch_gpu, precision):
     assert np.all(np.isfinite(to_host(z)))
     s.close()
     mg.close()
+
+
+@pytest.mark.parametrize("precision,precond", [("fp64", "mg"), ("mixed", "mg"), ("mixed", "fft")])
+def test_fused_direction_operator_matches_separate_kernels(torch_gpu, monkeypatch, precision, precond):
+    """The direction update and the deferred pressure update inside the
+    operator pass (with the last alpha d added after the loop) give the solve
+    of the separate direction / operator / update kernels (TBFG_UNFUSED_OP)
+    bit for bit: the same arithmetic on every cell, in the same order."""
+    n = (128, 96, 64)
+    print("case fused direction+operator", precision, precond, n, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 12, 1, 37)
+    dq, db = to_dev(q, torch_gpu), to_dev(beta, torch_gpu)
+    out = []
+    for unfused in (False, True):
+        if unfused:
+            monkeypatch.setenv("TBFG_UNFUSED_OP", "1")
+        s = T.PCG(g, nLevels=4, tolPCG=1e-9, maxIterPCG=300, precision=precision, flexible=1,
+                  coarse_max_iter=0, preconditioner=precond)
+        psi = torch_gpu.zeros_like(dq)
+        s.solvePCG(psi, db, dq)
+        assert s.converged
+        out.append((s.iterations, s.res_, to_host(psi)))
+        s.close()
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
