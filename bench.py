@@ -284,6 +284,33 @@ def main():
             log("fft-preconditioned mixed PCG: %s" % ex)
         fftp.close()
 
+    # FAST_MODE on the same grid: the FFT x/z + tridiagonal y solve of the
+    # constant-density right-hand side s = q (FP64, and FP32 with 2 FP64 refinement sweeps)
+    if not args.no_fp64:
+        fast = {}
+        p_ref = None
+        for prec in ("fp64", "mixed"):
+            fs = T.FastPoissonSolver(g, precision=prec)
+            pf = torch.zeros_like(drho)
+            fs.solveFPS(pf, q)
+            torch.cuda.synchronize()
+            reps = 20
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fs.solveFPS(pf, q)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            fast[prec + "_ms"] = round(ms, 4)
+            if p_ref is None:
+                p_ref = pf.clone()
+            else:
+                fast["mixed_rel_l2_vs_fp64"] = float(torch.linalg.norm(pf - p_ref) / torch.linalg.norm(p_ref))
+            fs.close()
+        fast["solves_per_s_mixed"] = round(1e3 / fast["mixed_ms"], 2)
+        out["fast_solver"] = fast
+        log("FFT solve (FAST_MODE) on the same grid: fp64 %.3f ms, mixed %.3f ms (rel diff %.1e)"
+            % (fast["fp64_ms"], fast["mixed_ms"], fast["mixed_rel_l2_vs_fp64"]))
+
     # roofline of the dominant kernel (finest-level red/black sweep)
     launches, kms, bytes_per = prof
     if launches > 0:

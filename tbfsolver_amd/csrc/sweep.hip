@@ -24,6 +24,7 @@
 
 #include <hip/hip_ext.h>
 
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 
@@ -673,15 +674,19 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
                               const KernelEvents* kev)
 {
     constexpr int pdm = sweep_pd_max<T, DOTS>();
-    static int per_cu = -1, ncu = 0;
-    if (per_cu < 0) {
-        int dev = 0;
+    // resident workgroups per CU and CU count, queried once (threads may race
+    // to fill it; they store the same value)
+    static std::atomic<int64_t> occ{-1};
+    int64_t o = occ.load(std::memory_order_acquire);
+    if (o < 0) {
+        int dev = 0, ncu_q = 0, nb = 0;
         TBFG_HIP(hipGetDevice(&dev));
-        TBFG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        int nb = 0;
+        TBFG_HIP(hipDeviceGetAttribute(&ncu_q, hipDeviceAttributeMultiprocessorCount, dev));
         TBFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, pdm>, NT, 0));
-        per_cu = nb < 1 ? 1 : nb;
+        o = ((int64_t)(nb < 1 ? 1 : nb) << 32) | (uint32_t)ncu_q;
+        occ.store(o, std::memory_order_release);
     }
+    const int per_cu = (int)(o >> 32), ncu = (int)(o & 0xffffffff);
     const int tiles_x = L.nx / TX;
     const int tiles_y = L.ny / TY;
     const int ntiles = tiles_x * tiles_y;
