@@ -516,10 +516,10 @@ __global__ void k_fft_residual(OpTables o, const double* __restrict__ s, const d
                                double* __restrict__ r, double* __restrict__ out_mean, double inv_vg, Reducer red)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;  // a block covers blockDim.y rows
     const int k = blockIdx.z;
     double v[1] = {0.0};
-    if (i < o.nx) {
+    if (i < o.nx && j < o.ny) {
         const int nx = o.nx, ny = o.ny, nz = o.nz;
         const size_t c = i + (size_t)nx * (j + (size_t)ny * k);
         const double pc = p[c];
@@ -776,10 +776,13 @@ int fps_solve_dev(tbfg_fps* h, const double* s, double* p, void* stream)
     TBFG_TRY((spectral_solve<float, double>(h, s, p, 0, st)));
     Reducer red{h->partials.as<double>()};
     for (int it = 0; it < h->refine; ++it) {
-        const dim3 blk(row_threads(h->nx));
-        hipLaunchKernelGGL(k_fft_residual, dim3(row_blocks(h->nx), h->ny, h->nz), blk, 0, st, h->ot, s, p,
-                           h->r.as<double>(), h->mean.as<double>(), h->inv_vg, red);
-        hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(256), 0, st, row_blocks(h->nx) * h->ny * h->nz, h->inv_vg,
+        // four rows per block: a quarter of the partial sums for the one-block finalize
+        constexpr unsigned kRows = 4;
+        const dim3 blk(row_threads(h->nx), kRows);
+        const dim3 grd(row_blocks(h->nx), ceil_div(h->ny, kRows), h->nz);
+        hipLaunchKernelGGL(k_fft_residual, grd, blk, 0, st, h->ot, s, p, h->r.as<double>(), h->mean.as<double>(),
+                           h->inv_vg, red);
+        hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(1024), 0, st, grd.x * grd.y * grd.z, h->inv_vg,
                            h->mean.as<double>(), red);
         TBFG_TRY((spectral_solve<float, double>(h, h->r.as<double>(), p, 1, st)));
     }
