@@ -504,3 +504,45 @@ def test_config2_full_size_properties(torch_gpu):
     F.computeSource(g, ux, uy, uz, d)
     torch_gpu.cuda.synchronize()
     assert float(torch_gpu.linalg.norm(d)) <= 1.05e-8 * div0
+
+
+def test_fused_operator_edge_cases(torch_gpu, monkeypatch):
+    """Edge cases of the deferred pressure update: stopping at the iteration
+    limit (the last alpha d still added), a warm start that already meets the
+    tolerance (no direction at all: psi unchanged), and a zero right-hand
+    side; each equal bit for bit to the separate kernels."""
+    n = (128, 96, 64)
+    print("case fused operator edges", n, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 8, 1, 43)
+    dq, db = to_dev(q, torch_gpu), to_dev(beta, torch_gpu)
+
+    def run(unfused, max_it, q_dev, psi0):
+        if unfused:
+            monkeypatch.setenv("TBFG_UNFUSED_OP", "1")
+        else:
+            monkeypatch.delenv("TBFG_UNFUSED_OP", raising=False)
+        s = T.PCG(g, nLevels=4, tolPCG=1e-8, maxIterPCG=max_it, precision="mixed", flexible=1, coarse_max_iter=0)
+        psi = psi0.clone()
+        s.solvePCG(psi, db, q_dev)
+        out = (s.iterations, s.converged, to_host(psi))
+        s.close()
+        return out
+
+    zero = torch_gpu.zeros_like(dq)
+    a = run(False, 3, dq, zero)
+    b = run(True, 3, dq, zero)
+    assert a[0] == b[0] == 3 and not a[1] and not b[1]
+    assert np.array_equal(a[2], b[2])
+    # warm start at the converged pressure: stops before the first direction
+    conv = run(False, 200, dq, zero)
+    assert conv[1]
+    warm = to_dev(conv[2], torch_gpu)
+    for unfused in (False, True):
+        it, ok, psi = run(unfused, 200, dq, warm)
+        assert ok and it <= 1
+        if it == 0:
+            assert np.array_equal(psi, conv[2])
+    # zero right-hand side from a zero start
+    c = run(False, 50, zero, zero)
+    d = run(True, 50, zero, zero)
+    assert np.array_equal(c[2], d[2]) and c[0] == d[0]
