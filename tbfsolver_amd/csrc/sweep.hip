@@ -627,8 +627,10 @@ template <class T>
 static bool pick_shape(const LevelDev<T>& L, SweepCfg& out, bool plain = false)
 {
     // nz even (the colour of a plane is its index's parity), nz >= 16 (a
-    // chunk's plane steps wrap around z at most once)
+    // chunk's plane steps wrap around z at most once); the level's arrays
+    // addressable by 32-bit buffer offsets (< 2 GiB)
     if (L.nz < 16 || L.nz % 2 != 0 || L.ny % 2 != 0) return false;
+    if ((size_t)L.nx * L.ny * L.nz * sizeof(T) >= ((size_t)1 << 31)) return false;
     const SweepCfg* tab = sizeof(T) == 4 ? (plain ? kShapesFPlain : kShapesF) : kShapesD;
     const int n = sizeof(T) == 4 ? (int)(sizeof kShapesF / sizeof kShapesF[0]) : (int)(sizeof kShapesD / sizeof kShapesD[0]);
     int want_tx = 0, want_ty = 0, want_nt = 0;
