@@ -1018,6 +1018,231 @@ __device__ void tail_coarse_pcg(const TailPlan<T>& P)
     __syncthreads();
 }
 
+// ---- the coarsest-level solve by multi-workgroup kernels (tail_coarse_pcg's
+// algorithm, for a coarsest level too large for one workgroup): one thread per
+// cell, block partial sums, a one-block finalize per reduction.  Every kernel
+// but the last returns at once when the inner solve has stopped.
+
+// sum V q; the finalize also stops the inner solve when the outer one has
+template <class T>
+__global__ void k_in_sum_vq(int n, const T* __restrict__ vol, const T* __restrict__ q, const PcgState* st,
+                            Reducer red)
+{
+    double v[1] = {0.0};
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!st->done && t < n) v[0] = (double)vol[t] * (double)q[t];
+    block_partial<1>(v, red);
+}
+__global__ void k_in_fin_shift(unsigned nb, int n, InnerState* is, const PcgState* st, Reducer red)
+{
+    double tot[1];
+    final_sum<1>(red, nb, tot);
+    if (threadIdx.x == 0) {
+        is->shift = tot[0] / (double)n;
+        is->done = st->done ? 1 : 0;
+        is->it = 0;
+        is->rlast = 0.0;
+    }
+}
+// r = q - shift / V (the right-hand side projected as the Lagrange row does),
+// x = 0, q = r (the right-hand side of the inner V-cycle); sum V r^2
+template <class T>
+__global__ void k_in_init(int n, const T* __restrict__ vol, T* __restrict__ q, T* __restrict__ r, T* __restrict__ x,
+                          const InnerState* is, Reducer red)
+{
+    double v[1] = {0.0};
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!is->done && t < n) {
+        const T rv = q[t] - (T)(is->shift / (double)vol[t]);
+        r[t] = rv;
+        x[t] = (T)0;
+        q[t] = rv;
+        v[0] = (double)vol[t] * (double)rv * (double)rv;
+    }
+    block_partial<1>(v, red);
+}
+__global__ void k_in_fin_rr0(unsigned nb, InnerState* is, Reducer red)
+{
+    if (is->done) return;
+    double tot[1];
+    final_sum<1>(red, nb, tot);
+    if (threadIdx.x == 0) {
+        is->rr0 = tot[0];
+        is->rlast = tot[0];
+        if (!(tot[0] > 0.0)) is->done = 1;  // zero right-hand side: x = 0
+    }
+}
+// d = z, sum V r z (the first direction)
+template <class T>
+__global__ void k_in_first_dir(int n, const T* __restrict__ vol, const T* __restrict__ r, const T* __restrict__ z,
+                               T* __restrict__ d, const InnerState* is, Reducer red)
+{
+    if (is->done) return;
+    double v[1] = {0.0};
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        d[t] = z[t];
+        v[0] = (double)vol[t] * (double)r[t] * (double)z[t];
+    }
+    block_partial<1>(v, red);
+}
+__global__ void k_in_fin_delta(unsigned nb, InnerState* is, Reducer red)
+{
+    if (is->done) return;
+    double tot[1];
+    final_sum<1>(red, nb, tot);
+    if (threadIdx.x == 0) is->delta = tot[0];
+}
+// ad = A d, sum V d ad
+template <class T>
+__global__ void k_in_op(LevelDev<T> L, const T* __restrict__ beta, const T* __restrict__ vol, const T* __restrict__ d,
+                        T* __restrict__ ad, const InnerState* is, Reducer red)
+{
+    if (is->done) return;
+    double v[1] = {0.0};
+    const int n = L.nx * L.ny * L.nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        const int i = t % L.nx, rr = t / L.nx, j = rr % L.ny, k = rr / L.ny;
+        size_t c, e, w, nn, s, f, b;
+        neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, nn, s, f, b);
+        Coef<T> a = coefs<T>(L, beta, i, j, k, c, e, w, nn, s, f, b);
+        const T av = apply_cell<T>(a, d, c, e, w, nn, s, f, b);
+        ad[t] = av;
+        v[0] = (double)vol[t] * (double)d[t] * (double)av;
+    }
+    block_partial<1>(v, red);
+}
+__global__ void k_in_fin_alpha(unsigned nb, InnerState* is, Reducer red)
+{
+    if (is->done) return;
+    double tot[1];
+    final_sum<1>(red, nb, tot);
+    if (threadIdx.x == 0) {
+        if (!(tot[0] != 0.0))
+            is->done = 1;
+        else
+            is->alpha = is->delta / tot[0];
+    }
+}
+// x += alpha d, r -= alpha ad, q = r; sum V r^2
+template <class T>
+__global__ void k_in_update(int n, const T* __restrict__ vol, T* __restrict__ x, T* __restrict__ r, T* __restrict__ q,
+                            const T* __restrict__ d, const T* __restrict__ ad, const InnerState* is, Reducer red)
+{
+    if (is->done) return;
+    double v[1] = {0.0};
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        const T alpha = (T)is->alpha;
+        x[t] = x[t] + alpha * d[t];
+        const T rv = r[t] - alpha * ad[t];
+        r[t] = rv;
+        q[t] = rv;
+        v[0] = (double)vol[t] * (double)rv * (double)rv;
+    }
+    block_partial<1>(v, red);
+}
+__global__ void k_in_fin_update(unsigned nb, double tol2, InnerState* is, Reducer red)
+{
+    if (is->done) return;
+    double tot[1];
+    final_sum<1>(red, nb, tot);
+    if (threadIdx.x == 0) {
+        is->rlast = tot[0];
+        is->it += 1;
+        if (tot[0] <= tol2 * is->rr0) is->done = 1;
+    }
+}
+// sum V r z, sum V ad z of the new preconditioned residual z
+template <class T>
+__global__ void k_in_dots(int n, const T* __restrict__ vol, const T* __restrict__ r, const T* __restrict__ ad,
+                          const T* __restrict__ z, const InnerState* is, Reducer red)
+{
+    if (is->done) return;
+    double v[2] = {0.0, 0.0};
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        const double zz = (double)z[t], vv = (double)vol[t];
+        v[0] = vv * (double)r[t] * zz;
+        v[1] = vv * (double)ad[t] * zz;
+    }
+    block_partial<2>(v, red);
+}
+__global__ void k_in_fin_gamma(unsigned nb, InnerState* is, Reducer red)
+{
+    if (is->done) return;
+    double tot[2];
+    final_sum<2>(red, nb, tot);
+    if (threadIdx.x == 0) {
+        is->gamma = -is->alpha * tot[1] / is->delta;  // Polak-Ribiere
+        is->delta = tot[0];
+    }
+}
+template <class T>
+__global__ void k_in_dir(int n, const T* __restrict__ z, T* __restrict__ d, const InnerState* is)
+{
+    if (is->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) d[t] = z[t] + (T)is->gamma * d[t];
+}
+// the solution into the level's p, and the diagnostics of tbfg_pcg_coarse_stats
+template <class T>
+__global__ void k_in_final(int n, const T* __restrict__ x, T* __restrict__ p, const InnerState* is, PcgState* st)
+{
+    if (st->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = x[t];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && is->rr0 > 0.0) {  // a zero right-hand side counts no call
+        st->coarse_calls += 1;
+        st->coarse_iters += is->it;
+        if (is->it > st->coarse_iters_max) st->coarse_iters_max = is->it;
+        const double rel = sqrt(is->rlast / is->rr0);
+        if (rel > st->coarse_res_max) st->coarse_res_max = rel;
+    }
+}
+
+template <class T>
+int launch_inner(InnerOp op, const LevelDev<T>& L, const InnerArgs<T>& a, InnerState* is, PcgState* st,
+                 const Reducer& red, void* stream)
+{
+    const int n = L.nx * L.ny * L.nz;
+    const unsigned nb = ceil_div((size_t)n, 256);
+    hipStream_t s = (hipStream_t)stream;
+    switch (op) {
+    case InnerOp::Start:
+        hipLaunchKernelGGL(k_in_sum_vq<T>, dim3(nb), dim3(256), 0, s, n, a.vol, a.q, st, red);
+        hipLaunchKernelGGL(k_in_fin_shift, dim3(1), dim3(256), 0, s, nb, n, is, st, red);
+        hipLaunchKernelGGL(k_in_init<T>, dim3(nb), dim3(256), 0, s, n, a.vol, a.q, a.r, a.x, is, red);
+        hipLaunchKernelGGL(k_in_fin_rr0, dim3(1), dim3(256), 0, s, nb, is, red);
+        break;
+    case InnerOp::FirstDir:
+        hipLaunchKernelGGL(k_in_first_dir<T>, dim3(nb), dim3(256), 0, s, n, a.vol, a.r, a.z, a.d, is, red);
+        hipLaunchKernelGGL(k_in_fin_delta, dim3(1), dim3(256), 0, s, nb, is, red);
+        break;
+    case InnerOp::OpUpdate:
+        hipLaunchKernelGGL(k_in_op<T>, dim3(nb), dim3(256), 0, s, L, a.beta, a.vol, a.d, a.ad, is, red);
+        hipLaunchKernelGGL(k_in_fin_alpha, dim3(1), dim3(256), 0, s, nb, is, red);
+        hipLaunchKernelGGL(k_in_update<T>, dim3(nb), dim3(256), 0, s, n, a.vol, a.x, a.r, a.q, a.d, a.ad, is, red);
+        hipLaunchKernelGGL(k_in_fin_update, dim3(1), dim3(256), 0, s, nb, a.tol2, is, red);
+        break;
+    case InnerOp::NextDir:
+        hipLaunchKernelGGL(k_in_dots<T>, dim3(nb), dim3(256), 0, s, n, a.vol, a.r, a.ad, a.z, is, red);
+        hipLaunchKernelGGL(k_in_fin_gamma, dim3(1), dim3(256), 0, s, nb, is, red);
+        hipLaunchKernelGGL(k_in_dir<T>, dim3(nb), dim3(256), 0, s, n, a.z, a.d, is);
+        break;
+    case InnerOp::Finish:
+        hipLaunchKernelGGL(k_in_final<T>, dim3(nb), dim3(256), 0, s, n, a.x, a.p, is, st);
+        break;
+    }
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+template int launch_inner<float>(InnerOp, const LevelDev<float>&, const InnerArgs<float>&, InnerState*, PcgState*,
+                                 const Reducer&, void*);
+template int launch_inner<double>(InnerOp, const LevelDev<double>&, const InnerArgs<double>&, InnerState*,
+                                  PcgState*, const Reducer&, void*);
+
 // V-cycle over tail levels top..bot (multiGrid.f90:186-232) with the coarsest
 // level solved by tail_coarse_pcg (or the dense solve when it is the bottom;
 // COARSE_CG false: always, and the inner CG is not compiled in).
@@ -1034,7 +1259,7 @@ __device__ void tail_vcycle(const TailPlan<T>& P, int top, int bot, bool reset_t
         tail_dense<T>(P, P.lv[bot]);
     for (int l = bot - 1; l >= top; --l) {
         tail_prolong<T>(P.lv[l], P.lv[l + 1]);
-        tail_smooth<T>(P.lv[l], P.n_post, false);
+        tail_smooth<T>(P.lv[l], P.n_post, false, P.rev_post != 0);
     }
 }
 

@@ -39,6 +39,20 @@ struct PcgState {
     double coarse_res_max;    // worst final relative residual
 };
 
+// The coarsest-level solve run by multi-workgroup kernels (levels too large for
+// the tail): the scalars of its flexible CG.  done: stop (converged, a zero
+// right-hand side or the outer solve already stopped).
+struct InnerState {
+    double shift;  // (sum V q) / N, removed from the right-hand side
+    double rr0;    // sum V r^2 at the start
+    double delta;  // sum V r z
+    double alpha;
+    double gamma;
+    double rlast;  // sum V r^2 after the last update
+    int32_t done;
+    int32_t it;
+};
+
 // Partial sums of a grid reduction: NV doubles per block of the producing
 // kernel, added in block order by a one-block finalize kernel.
 struct Reducer {
@@ -60,6 +74,7 @@ struct TailPlan {
     int bottom;        // tail index of the dense bottom level
     int n_pre, n_post;
     int cmaxit;        // inner PCG iterations on the coarsest user level
+    int rev_post;      // post-smoothing black then red (the tail is the inner V-cycle's bottom part)
     T ctol;            // its relative residual target
     T *cx, *cr, *cd, *cad;  // inner PCG vectors (coarsest user level)
     const T* cvol;     // cell volumes of that level

@@ -67,6 +67,23 @@ template <class T, class TQ>
 int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
                  const int* done, void* stream, const SweepDots* dots = nullptr, unsigned* nblocks_out = nullptr,
                  const KernelEvents* kev = nullptr);
+// the coarsest-level solve by multi-workgroup kernels, in phases driven by the
+// host: Start (project q, r = q, x = 0), FirstDir (d = z), OpUpdate (ad = A d,
+// alpha, x and r updated, q = r, stop test), NextDir (gamma, d = z + gamma d),
+// Finish (p = x, diagnostics)
+enum class InnerOp { Start, FirstDir, OpUpdate, NextDir, Finish };
+template <class T>
+struct InnerArgs {
+    const T* beta;
+    const T* vol;
+    T *q, *r, *x, *d, *ad;  // q: the level's right-hand side, also the inner V-cycle's
+    const T* z;              // the inner V-cycle's output
+    T* p;                    // receives x
+    double tol2;             // (relative residual target)^2
+};
+template <class T>
+int launch_inner(InnerOp op, const LevelDev<T>& L, const InnerArgs<T>& a, InnerState* is, PcgState* st,
+                 const Reducer& red, void* stream);
 int launch_fin_dots(unsigned nblocks, size_t ncell, int init, PcgState* st, const Reducer& red, void* stream);
 template <class T, class TQ>
 int launch_resid_restrict(const LevelDev<T>& L, const T* beta, const TQ* q, const T* p, T* qc, const int* done,

@@ -14,6 +14,7 @@
 // approximates its exact solve run inside one single-workgroup kernel.
 #include "pcg.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -49,6 +50,7 @@ tbfg_pcg::~tbfg_pcg()
     for (hipEvent_t e : hev)
         if (e) (void)hipEventDestroy(e);
     if (hstage) (void)hipHostFree(hstage);
+    if (hin) (void)hipHostFree(hin);
     if (fps) tbfg::fps_destroy(fps);
 }
 
@@ -153,7 +155,10 @@ static int build_plan(tbfg_pcg* h)
     std::memset(&P, 0, sizeof P);
     P.nlev = h->ntot - h->t0;
     // with coarse_max_iter == 0 the V-cycle runs straight to the dense bottom
-    P.user_last = (h->prm.coarse_max_iter > 0 ? h->nuser - 1 : h->ntot - 1) - h->t0;
+    // and so does it when the coarsest user level is solved outside the tail
+    // (the tail is then the bottom part of that solve's inner V-cycle)
+    P.user_last = (h->prm.coarse_max_iter > 0 && h->kin < 0 ? h->nuser - 1 : h->ntot - 1) - h->t0;
+    P.rev_post = h->kin >= 0 ? 1 : 0;
     P.bottom = (h->ntot - 1) - h->t0;
     P.n_pre = h->prm.n_pre;
     P.n_post = h->prm.n_post;
@@ -183,7 +188,7 @@ static int build_plan(tbfg_pcg* h)
     // the LDS-resident tail when its levels fit (the inner coarsest-level CG
     // keeps its vectors in HBM and uses the other form)
     h->tail_lds = 0;
-    if (h->prm.coarse_max_iter == 0 && !std::getenv("TBFG_TAIL_HBM") && lds <= tail_lds_capacity<T>())
+    if ((h->prm.coarse_max_iter == 0 || h->kin >= 0) && !std::getenv("TBFG_TAIL_HBM") && lds <= tail_lds_capacity<T>())
         h->tail_lds = lds;
     return upload(h->plan, &P, sizeof P);
 }
@@ -264,10 +269,21 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     const size_t tail_lim = tail_cells_limit();
     // first level of the single-workgroup tail: the first small level, and at
     // the latest the reference's coarsest level when that one is solved by the
-    // inner CG (which runs inside the tail)
-    const int t_cap = prm.coarse_max_iter > 0 ? h->nuser - 1 : ntot - 1;
+    // inner CG (which then runs inside the tail).  A coarsest user level too
+    // large for the tail has its inner CG run as multi-workgroup kernels, and
+    // the tail starts at the first small level under it.
+    auto level_cells = [&](int l) {
+        size_t cells = 1;
+        for (int a = 0; a < 3; ++a) cells *= (size_t)(g->n[a] >> l);
+        return cells;
+    };
+    h->kin = -1;
+    if (prm.coarse_max_iter > 0 && ntot > h->nuser && level_cells(h->nuser - 1) > tail_lim &&
+        !std::getenv("TBFG_INNER_IN_TAIL"))
+        h->kin = h->nuser - 1;
+    const int t_cap = prm.coarse_max_iter > 0 && h->kin < 0 ? h->nuser - 1 : ntot - 1;
     int t0 = t_cap;
-    for (int l = 1; l < t_cap; ++l) {
+    for (int l = h->kin >= 0 ? h->kin + 1 : 1; l < t_cap; ++l) {
         size_t cells = 1;
         for (int a = 0; a < 3; ++a) cells *= (size_t)(g->n[a] >> l);
         if (cells <= tail_lim) {
@@ -306,7 +322,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         TBFG_TRY(h->p[l].alloc(n * es));
         if (l > 0) TBFG_TRY(h->q[l].alloc(n * es));
         if (l > 0 || h->mixed) TBFG_TRY(h->beta[l].alloc(n * es));
-        if (l < t0) {
+        if (l < (h->kin >= 0 ? h->kin : t0)) {
             h->fused[l] = h->mixed ? sweep_supported<float>(h->tabf[l]->view) : sweep_supported<double>(h->tabd[l]->view);
             // below ~1M cells a plane march of a few steps per workgroup is
             // latency bound and the two half-sweep launches are quicker
@@ -360,6 +376,10 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         TBFG_TRY(h->cr.alloc(nu * es));
         TBFG_TRY(h->cd.alloc(nu * es));
         TBFG_TRY(h->cad.alloc(nu * es));
+        if (h->kin >= 0) {
+            TBFG_TRY(h->instate.alloc(sizeof(InnerState)));
+            TBFG_HIP(hipHostMalloc((void**)&h->hin, 2 * sizeof(int32_t), hipHostMallocDefault));
+        }
         std::vector<double> v(nu);
         for (int k = 1; k <= U.n[2]; ++k)
             for (int j = 1; j <= U.n[1]; ++j)
@@ -391,7 +411,8 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     for (auto& t : h->tabf) fp += (int64_t)t->buf.bytes;
     fp += (int64_t)(h->r.bytes + h->r32.bytes + h->d.bytes + h->qv.bytes + h->st.bytes + h->partials.bytes + h->counter.bytes +
                     h->plan.bytes + h->G.bytes + h->dwork.bytes + h->vol.bytes + h->cx.bytes + h->cr.bytes +
-                    h->cd.bytes + h->cad.bytes + h->cvol.bytes + h->zfft.bytes + h->d2.bytes);
+                    h->cd.bytes + h->cad.bytes + h->cvol.bytes + h->zfft.bytes + h->d2.bytes +
+                    h->instate.bytes);
     if (h->fps) fp += fps_footprint(h->fps);
     h->footprint = fp;
     *out = h.release();
@@ -452,6 +473,95 @@ static int smooth(tbfg_pcg* h, int l, const T* b, const T* q, T*& p, T*& alt, in
     return TBFG_OK;
 }
 
+// The inner V-cycle of the coarsest-level solve on level kin (the tail's
+// tail_inner_vcycle, for a level too large for the tail): from zero, red/black
+// pre-smoothing and black/red post-smoothing in place on the levels above the
+// tail, the tail (post-smoothing reversed too) below.  q[kin] in, p[kin] out.
+template <class T>
+static int inner_vcycle(tbfg_pcg* h, const int* done, void* stream)
+{
+    const int t0 = h->t0;
+    for (int l = h->kin; l < t0; ++l) {
+        const LevelDev<T>& L = geo<T>(h, l);
+        const T* b = h->beta[l].as<T>();
+        const T* q = h->q[l].as<T>();
+        T* p = h->p[l].as<T>();
+        if (h->prm.n_pre == 0)
+            TBFG_HIP(hipMemsetAsync(p, 0, h->hl[l].cells() * sizeof(T), (hipStream_t)stream));
+        for (int s = 0; s < h->prm.n_pre; ++s) {
+            TBFG_TRY(launch_rbgs<T, T>(L, b, q, p, RED, s == 0, done, stream));
+            TBFG_TRY(launch_rbgs<T, T>(L, b, q, p, BLACK, s == 0, done, stream));
+        }
+        TBFG_TRY(launch_resid_restrict<T, T>(L, b, q, p, h->q[l + 1].as<T>(), done, stream));
+    }
+    TBFG_TRY(launch_tail<T>(h->plan.as<TailPlan<T>>(), done, h->tail_lds, stream));
+    for (int l = t0 - 1; l >= h->kin; --l) {
+        const LevelDev<T>& L = geo<T>(h, l);
+        const T* b = h->beta[l].as<T>();
+        const T* q = h->q[l].as<T>();
+        T* p = h->p[l].as<T>();
+        TBFG_TRY(launch_prolong_add<T>(L, h->p[l + 1].as<T>(), p, done, stream));
+        for (int s = 0; s < h->prm.n_post; ++s) {
+            TBFG_TRY(launch_rbgs<T, T>(L, b, q, p, BLACK, false, done, stream));
+            TBFG_TRY(launch_rbgs<T, T>(L, b, q, p, RED, false, done, stream));
+        }
+    }
+    return TBFG_OK;
+}
+
+// The coarsest-level solve (poissMat.f90:289-312) as the tail's
+// tail_coarse_pcg computes it, by multi-workgroup kernels: flexible CG in the
+// V-weighted inner product preconditioned by inner_vcycle, to the relative
+// residual coarse_tol or coarse_max_iter iterations.  q[kin] in, p[kin] out.
+// The host queues as many iterations as the previous solve took, then reads
+// the stop flag and queues two more at a time until it is set; iterations
+// queued after the stop return at once.
+template <class T>
+static int inner_solve(tbfg_pcg* h, void* stream)
+{
+    const int U = h->kin;
+    const LevelDev<T>& L = geo<T>(h, U);
+    InnerState* is = h->instate.as<InnerState>();
+    PcgState* st = h->st.as<PcgState>();
+    const int* idone = &is->done;
+    Reducer red{h->partials.as<double>()};
+    InnerArgs<T> a;
+    a.beta = h->beta[U].as<T>();
+    a.vol = h->cvol.as<T>();
+    a.q = h->q[U].as<T>();
+    a.r = h->cr.as<T>();
+    a.x = h->cx.as<T>();
+    a.d = h->cd.as<T>();
+    a.ad = h->cad.as<T>();
+    a.z = h->p[U].as<T>();
+    a.p = h->p[U].as<T>();
+    const double ctol = (double)(T)h->prm.coarse_tol;
+    a.tol2 = ctol * ctol;
+    TBFG_TRY(launch_inner<T>(InnerOp::Start, L, a, is, st, red, stream));
+    TBFG_TRY(inner_vcycle<T>(h, idone, stream));
+    TBFG_TRY(launch_inner<T>(InnerOp::FirstDir, L, a, is, st, red, stream));
+    const int maxit = h->prm.coarse_max_iter;
+    int queued = 0, chunk = std::max(1, h->inner_guess);
+    while (queued < maxit) {
+        const int m = std::min(chunk, maxit - queued);
+        for (int it = 0; it < m; ++it) {
+            TBFG_TRY(launch_inner<T>(InnerOp::OpUpdate, L, a, is, st, red, stream));
+            TBFG_TRY(inner_vcycle<T>(h, idone, stream));
+            TBFG_TRY(launch_inner<T>(InnerOp::NextDir, L, a, is, st, red, stream));
+        }
+        queued += m;
+        if (queued >= maxit) break;
+        TBFG_HIP(hipMemcpyAsync(h->hin, &is->done, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, (hipStream_t)stream));
+        TBFG_HIP(hipStreamSynchronize((hipStream_t)stream));
+        if (h->hin[0]) {
+            if (h->hin[1] > 0) h->inner_guess = h->hin[1];
+            break;
+        }
+        chunk = 2;
+    }
+    return launch_inner<T>(InnerOp::Finish, L, a, is, st, red, stream);
+}
+
 // One preconditioner application z = M r (multiGrid.f90:165-232, fixed cycles).
 // r0 is the finest right-hand side in the multigrid precision; on return *z
 // points at the level-0 buffer that holds the result.  dots_init >= 0: the
@@ -471,7 +581,8 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
         sd.red = Reducer{h->partials.as<double>()};
         sd.capacity = h->partials.bytes / sizeof(double);
     }
-    const int t0 = h->t0;
+    // levels smoothed here; below them the tail, or the multi-workgroup solve of level kin
+    const int t0 = h->kin >= 0 ? h->kin : h->t0;
     T* cur[kMaxLevels];
     T* alt[kMaxLevels];
     for (int l = 0; l < t0; ++l) {
@@ -491,7 +602,10 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
                 TBFG_TRY(launch_resid_restrict<T, T>(geo<T>(h, l), b, q, cur[l], h->q[l + 1].as<T>(), done, stream));
             }
         }
-        TBFG_TRY(launch_tail<T>(h->plan.as<TailPlan<T>>(), done, h->tail_lds, stream));
+        if (h->kin >= 0)
+            TBFG_TRY(inner_solve<T>(h, stream));
+        else
+            TBFG_TRY(launch_tail<T>(h->plan.as<TailPlan<T>>(), done, h->tail_lds, stream));
         for (int l = t0 - 1; l >= 0; --l) {
             const T* b = (l == 0) ? beta0 : h->beta[l].as<T>();
             const T* q = (l == 0) ? r0 : h->q[l].as<T>();

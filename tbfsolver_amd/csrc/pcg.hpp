@@ -13,6 +13,7 @@ struct tbfg_pcg {
     int nuser = 0;   // reference levels (nLevels)
     int ntot = 0;    // reference levels + sub-levels of the coarsest-solve hierarchy
     int t0 = 0;      // first level run by the tail kernel
+    int kin = -1;    // coarsest user level when its solve runs as multi-workgroup kernels (too large for the tail)
     size_t n0 = 0;   // finest cells
 
     std::vector<tbfg::HostLevel> hl;
@@ -34,6 +35,9 @@ struct tbfg_pcg {
     tbfg::DevBuf plan;                  // TailPlan<T>
     tbfg::DevBuf G, dwork, vol;         // dense bottom operator and its set-up scratch
     tbfg::DevBuf cx, cr, cd, cad, cvol; // inner CG of the coarsest-level solve
+    tbfg::DevBuf instate;               // its InnerState (multi-workgroup form)
+    int32_t* hin = nullptr;             // pinned copy of InnerState::done, ::it (the host's stop test)
+    int inner_guess = 8;                // iterations launched before the first stop test (last solve's count)
     tbfg::DevBuf hpsi, hbeta, hq;       // device copies for the host-array entry point
     tbfg_fps* fps = nullptr;            // FFT preconditioner (TBFG_PRECOND_FFT)
     tbfg::DevBuf zfft;                  // its output z (FP64)

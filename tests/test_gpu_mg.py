@@ -546,3 +546,68 @@ def test_fused_operator_edge_cases(torch_gpu, monkeypatch):
     c = run(False, 50, zero, zero)
     d = run(True, 50, zero, zero)
     assert np.array_equal(c[2], d[2]) and c[0] == d[0]
+
+
+@pytest.mark.parametrize("n,stretch,nlev,nb,tail_cells", [((64, 64, 64), 0, 4, 18, 256), ((64, 48, 64), 1, 3, 20, 1024)])
+def test_coarse_solve_kernels_match_oracle(torch_gpu, monkeypatch, n, stretch, nlev, nb, tail_cells):
+    """A coarsest level larger than the single-workgroup tail has its inner CG
+    run as multi-workgroup kernels (the tail limit is lowered here so that
+    small grids take that path): the solve agrees with the oracle's exact
+    coarsest solve as the in-tail form does, with the same outer and inner
+    iteration counts as the in-tail form within one."""
+    print("case pcg coarse kernels", n, stretch, nlev, tail_cells, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, nb, stretch, 21)
+    monkeypatch.setenv("TBFG_TAIL_CELLS", str(tail_cells))
+    for flexible in (0, 1):
+        ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=nlev, tol=1e-12, max_iter=200, coarse_mode=0,
+                                                    flexible=flexible), beta, q)
+        runs = {}
+        for form in ("kernels", "tail"):
+            if form == "tail":
+                monkeypatch.setenv("TBFG_INNER_IN_TAIL", "1")
+            else:
+                monkeypatch.delenv("TBFG_INNER_IN_TAIL", raising=False)
+            s = T.PCG(g, nLevels=nlev, tolPCG=1e-12, maxIterPCG=200, flexible=flexible)
+            psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+            s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+            cs = s.coarse_stats()
+            print("  flexible", flexible, form, "iterations", s.iterations, "oracle", it, cs, flush=True)
+            assert rc == 0 and s.converged
+            assert abs(s.iterations - it) <= 1, (s.iterations, it)
+            out = to_host(psi)
+            assert rel(mean_free(out), mean_free(ref)) < 1e-10
+            assert cs["calls"] >= s.iterations and cs["iters_max"] < 40 and cs["res_max"] <= 1e-10
+            runs[form] = (s.iterations, cs)
+            s.close()
+        (ik, ck), (it_, ct) = runs["kernels"], runs["tail"]
+        assert abs(ik - it_) <= 1
+        assert abs(ck["iters_max"] - ct["iters_max"]) <= 1
+
+
+def test_coarse_solve_kernels_native_size(torch_gpu, monkeypatch):
+    """A coarsest level of 32x24x32 (over the tail's 4096 cells) in FP64 and in
+    mixed precision: the multi-workgroup inner CG against the in-tail one."""
+    print("case pcg coarse kernels 32x24x32", flush=True)
+    g, go, rho, _, q, beta = bubble_case((64, 48, 64), 16, 1, 29)
+    for precision, tol, ctol in (("fp64", 1e-12, 1e-10), ("mixed", 1e-8, 1e-5)):
+        runs = {}
+        for form in ("kernels", "tail"):
+            if form == "tail":
+                monkeypatch.setenv("TBFG_INNER_IN_TAIL", "1")
+            else:
+                monkeypatch.delenv("TBFG_INNER_IN_TAIL", raising=False)
+            s = T.PCG(g, nLevels=2, tolPCG=tol, maxIterPCG=200, precision=precision, flexible=1,
+                      coarse_max_iter=40, coarse_tol=ctol)
+            psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+            s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+            cs = s.coarse_stats()
+            print("  ", precision, form, "iterations", s.iterations, cs, flush=True)
+            assert s.converged and cs["calls"] >= s.iterations
+            out = to_host(psi)
+            true_res = np.linalg.norm(q - O.apply_op(go, beta, out)) / np.linalg.norm(q)
+            assert true_res < 2 * tol
+            runs[form] = (s.iterations, cs, out)
+            s.close()
+        (ik, ck, ok), (it_, ct, ot) = runs["kernels"], runs["tail"]
+        assert abs(ik - it_) <= 1 and abs(ck["iters_max"] - ct["iters_max"]) <= 1
+        assert rel(mean_free(ok), mean_free(ot)) < 1e3 * tol
