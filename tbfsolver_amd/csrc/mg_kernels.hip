@@ -22,18 +22,44 @@ __device__ __forceinline__ int wrap_dn(int i, int n) { return i > 0 ? i - 1 : n 
 __device__ __forceinline__ int clamp_up(int j, int n) { return j + 1 < n ? j + 1 : j; }
 __device__ __forceinline__ int clamp_dn(int j) { return j > 0 ? j - 1 : j; }
 
-__device__ __forceinline__ void neighbours(int i, int j, int k, int nx, int ny, int nz, size_t& c, size_t& e,
-                                           size_t& w, size_t& n, size_t& s, size_t& f, size_t& b)
+// (I: the index type, size_t for arrays in HBM, int for the LDS-resident tail)
+template <class I>
+__device__ __forceinline__ void neighbours(int i, int j, int k, int nx, int ny, int nz, I& c, I& e, I& w, I& n, I& s,
+                                           I& f, I& b)
 {
-    const size_t row = (size_t)nx * ((size_t)j + (size_t)ny * k);
+    const I row = (I)nx * ((I)j + (I)ny * k);
     c = row + i;
     e = row + wrap_up(i, nx);
     w = row + wrap_dn(i, nx);
-    n = (size_t)nx * ((size_t)clamp_up(j, ny) + (size_t)ny * k) + i;
-    s = (size_t)nx * ((size_t)clamp_dn(j) + (size_t)ny * k) + i;
-    f = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * wrap_up(k, nz));
-    b = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * wrap_dn(k, nz));
+    n = (I)nx * ((I)clamp_up(j, ny) + (I)ny * k) + i;
+    s = (I)nx * ((I)clamp_dn(j) + (I)ny * k) + i;
+    f = (I)i + (I)nx * ((I)j + (I)ny * wrap_up(k, nz));
+    b = (I)i + (I)nx * ((I)j + (I)ny * wrap_dn(k, nz));
 }
+
+// A level whose tables live in LDS (the LDS-resident tail): the same fields as
+// LevelDev with pointers typed as LDS addresses, so their loads are LDS reads
+// with 32-bit addresses rather than generic (flat) loads.
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+template <class T>
+struct AxisLds {
+    lds_ptr<const T> ip, im, rlo, rhi, plo, phi;
+};
+template <class T>
+struct LevelLds {
+    int nx, ny, nz;
+    AxisLds<T> a[3];
+};
+// index type of a level's arrays
+template <class LV>
+struct IdxOf {
+    using type = size_t;
+};
+template <class T>
+struct IdxOf<LevelLds<T>> {
+    using type = int;
+};
 
 // Block coordinates of a 3-D block grid launched as a 1-D grid: blocks go
 // round-robin to the 8 XCDs, each with its own L2, so consecutive blocks are
@@ -62,9 +88,9 @@ struct Coef {
     T aR, aL, aT, aB, aF, aK;
 };
 
-template <class T, class TB>
-__device__ __forceinline__ Coef<T> coefs(const LevelDev<T>& L, const TB* __restrict__ beta, int i, int j, int k,
-                                         size_t c, size_t e, size_t w, size_t n, size_t s, size_t f, size_t b)
+template <class T, class LV, class PB, class I>
+__device__ __forceinline__ Coef<T> coefs(const LV& L, PB __restrict__ beta, int i, int j, int k, I c, I e, I w, I n,
+                                         I s, I f, I b)
 {
     const T bc = (T)beta[c];
     Coef<T> a;
@@ -78,9 +104,8 @@ __device__ __forceinline__ Coef<T> coefs(const LevelDev<T>& L, const TB* __restr
 }
 
 // (A p)_c, pcg.f90:171-176
-template <class T, class TP>
-__device__ __forceinline__ T apply_cell(const Coef<T>& a, const TP* __restrict__ p, size_t c, size_t e, size_t w,
-                                        size_t n, size_t s, size_t f, size_t b)
+template <class T, class PP, class I>
+__device__ __forceinline__ T apply_cell(const Coef<T>& a, PP __restrict__ p, I c, I e, I w, I n, I s, I f, I b)
 {
     const T pc = (T)p[c];
     return a.aR * ((T)p[e] - pc) - a.aL * (pc - (T)p[w]) + a.aT * ((T)p[n] - pc) - a.aB * (pc - (T)p[s]) +
@@ -235,7 +260,10 @@ __global__ void __launch_bounds__(NT) k_op_march(LevelDev<double> L, const doubl
     auto slot = [&](int k) { return (k - k0 + 4) & 3; };
     // a register set holds one plane: d (or z and d_old) and beta; with DIR
     // also psi and d_old of this thread's own cell on the plane before (o2)
+    // (plane k1 is the last one a chunk reads; the prefetch of the last steps
+    // loads it again instead of the planes after it: a cache hit, not HBM traffic)
     auto fetch = [&](double(&rd)[ON], TZ(&rz)[ON], double(&rb)[ON], double(&o2)[2], int kp) {
+        kp = min(kp, k1);
         const size_t o = plane * modz(kp);
 #pragma unroll
         for (int m = 0; m < ON; ++m) {
@@ -292,7 +320,7 @@ __global__ void __launch_bounds__(NT) k_op_march(LevelDev<double> L, const doubl
         auto step = [&](int k, double(&rd)[ON], TZ(&rz)[ON], double(&rb)[ON], double(&o2)[2]) {
             stash(rd, rz, rb, k + 1);
             const double opsi = o2[0], odold = o2[1];  // the own cell on plane k
-            fetch(rd, rz, rb, o2, k + 1 + PD);  // (past the chunk end: loaded, not used)
+            fetch(rd, rz, rb, o2, k + 1 + PD);  // (past the chunk end: plane k1 again, not used)
             __syncthreads();
             if (k < k1) {
                 const double* D0 = sD + slot(k) * OS;
@@ -517,13 +545,13 @@ __global__ void k_pcg_dir(size_t ncell, const TZ* __restrict__ z, double* __rest
 // one red or black half-sweep of iterateGS (rbgs.f90:205-259) on a level.
 // FIRST: the level starts from p = 0 (rbgs.f90:196-198 / pcg.f90:118), so the
 // red pass reads no p and the black pass uses 0 as its own wall ghost.
-template <class T, class TQ, int COLOUR, bool FIRST>
-__device__ __forceinline__ void rbgs_cell(const LevelDev<T>& L, const T* __restrict__ beta,
-                                          const TQ* __restrict__ q, T* __restrict__ p, int h, int j, int k)
+template <class T, class TQ, int COLOUR, bool FIRST, class LV, class PB, class PQ, class PP>
+__device__ __forceinline__ void rbgs_cell(const LV& L, PB __restrict__ beta, PQ __restrict__ q, PP __restrict__ p,
+                                          int h, int j, int k)
 {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int i = ((COLOUR + j + k + 1) & 1) + 2 * h;
-    size_t c, e, w, n, s, f, b;
+    typename IdxOf<LV>::type c, e, w, n, s, f, b;
     neighbours(i, j, k, nx, ny, nz, c, e, w, n, s, f, b);
     Coef<T> a = coefs<T>(L, beta, i, j, k, c, e, w, n, s, f, b);
     const T den = a.aR + a.aL + a.aT + a.aB + a.aF + a.aK;
@@ -557,10 +585,9 @@ __global__ void k_rbgs(LevelDev<T> L, const T* __restrict__ beta, const TQ* __re
 // one thread per coarse cell, qc = sum_{children} wx wy wz (q - A p).  The
 // plane loop stays rolled: all eight children in flight need more registers
 // than a 1024-thread workgroup has, and the spill costs more than the overlap.
-template <class T, class TQ>
-__device__ __forceinline__ T resid_restrict_cell(const LevelDev<T>& L, const T* __restrict__ beta,
-                                                 const TQ* __restrict__ q, const T* __restrict__ p, int I, int J,
-                                                 int K)
+template <class T, class TQ, class LV, class PB, class PQ, class PP>
+__device__ __forceinline__ T resid_restrict_cell(const LV& L, PB __restrict__ beta, PQ __restrict__ q,
+                                                 PP __restrict__ p, int I, int J, int K)
 {
     T acc = 0;
 #pragma unroll 1
@@ -575,7 +602,7 @@ __device__ __forceinline__ T resid_restrict_cell(const LevelDev<T>& L, const T* 
             for (int di = 0; di < 2; ++di) {
                 const int i = 2 * I + di;
                 const T wx = di ? L.a[0].rhi[I] : L.a[0].rlo[I];
-                size_t c, e, w, n, s, f, b;
+                typename IdxOf<LV>::type c, e, w, n, s, f, b;
                 neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
                 Coef<T> a = coefs<T>(L, beta, i, j, k, c, e, w, n, s, f, b);
                 const T res = (T)q[c] - apply_cell<T>(a, p, c, e, w, n, s, f, b);
@@ -704,10 +731,11 @@ __global__ void k_restrict(LevelDev<T> L, const TF* __restrict__ fine, T* __rest
 }
 
 // prolongation with coarse ghosts and correction p += P pc (multiGrid.f90:222-223).
-template <class T>
-__device__ __forceinline__ void prolong_cell(const LevelDev<T>& L, int ncx, int ncy, int ncz, const T* __restrict__ pc,
-                                             T* __restrict__ p, int i, int j, int k)
+template <class T, class LV, class PC, class PP>
+__device__ __forceinline__ void prolong_cell(const LV& L, int ncx, int ncy, int ncz, PC __restrict__ pc,
+                                             PP __restrict__ p, int i, int j, int k)
 {
+    using I = typename IdxOf<LV>::type;
     int il = ((i + 1) >> 1) - 1, jl = ((j + 1) >> 1) - 1, kl = ((k + 1) >> 1) - 1;
     int ih = il + 1, jh = jl + 1, kh = kl + 1;
     if (il < 0) il = ncx - 1;
@@ -719,12 +747,12 @@ __device__ __forceinline__ void prolong_cell(const LevelDev<T>& L, int ncx, int 
     const T wxl = L.a[0].plo[i], wxh = L.a[0].phi[i];
     const T wyl = L.a[1].plo[j], wyh = L.a[1].phi[j];
     const T wzl = L.a[2].plo[k], wzh = L.a[2].phi[k];
-    const size_t sxy = (size_t)ncx * ncy;
-    const size_t r_ll = (size_t)ncx * jl + sxy * kl, r_hl = (size_t)ncx * jh + sxy * kl;
-    const size_t r_lh = (size_t)ncx * jl + sxy * kh, r_hh = (size_t)ncx * jh + sxy * kh;
+    const I sxy = (I)ncx * ncy;
+    const I r_ll = (I)ncx * jl + sxy * kl, r_hl = (I)ncx * jh + sxy * kl;
+    const I r_lh = (I)ncx * jl + sxy * kh, r_hh = (I)ncx * jh + sxy * kh;
     const T v = wzl * (wyl * (wxl * pc[r_ll + il] + wxh * pc[r_ll + ih]) + wyh * (wxl * pc[r_hl + il] + wxh * pc[r_hl + ih])) +
                 wzh * (wyl * (wxl * pc[r_lh + il] + wxh * pc[r_lh + ih]) + wyh * (wxl * pc[r_hh + il] + wxh * pc[r_hh + ih]));
-    const size_t c = i + (size_t)L.nx * (j + (size_t)L.ny * k);
+    const I c = i + (I)L.nx * (j + (I)L.ny * k);
     p[c] = p[c] + v;
 }
 
@@ -801,11 +829,49 @@ __global__ void k_convert(size_t n, const TS* __restrict__ src, TD* __restrict__
 
 // The levels below ~32K cells are latency bound: every phase is a few
 // microseconds of work, less than a kernel launch.  One workgroup runs all of
-// them back to back with workgroup barriers; their arrays stay in L2.
+// them back to back with workgroup barriers; their arrays stay in L2, or in
+// LDS (LDS = true: the level's pointers point into the workgroup's LDS and are
+// used as LDS addresses).
+template <class T, bool LDS>
+struct TailView;
 template <class T>
-__device__ void tail_smooth(const TailLevel<T>& lv, int sweeps, bool first, bool reverse = false)
+struct TailView<T, false> {
+    LevelDev<T> geo;
+    T* p;
+    T* q;
+    const T* beta;
+    __device__ explicit TailView(const TailLevel<T>& lv) : geo(lv.geo), p(lv.p), q(lv.q), beta(lv.beta) {}
+};
+template <class T>
+struct TailView<T, true> {
+    LevelLds<T> geo;
+    lds_ptr<T> p;
+    lds_ptr<T> q;
+    lds_ptr<const T> beta;
+    __device__ explicit TailView(const TailLevel<T>& lv)
+    {
+        geo.nx = lv.geo.nx;
+        geo.ny = lv.geo.ny;
+        geo.nz = lv.geo.nz;
+        for (int a = 0; a < 3; ++a) {
+            geo.a[a].ip = (lds_ptr<const T>)lv.geo.a[a].ip;
+            geo.a[a].im = (lds_ptr<const T>)lv.geo.a[a].im;
+            geo.a[a].rlo = (lds_ptr<const T>)lv.geo.a[a].rlo;
+            geo.a[a].rhi = (lds_ptr<const T>)lv.geo.a[a].rhi;
+            geo.a[a].plo = (lds_ptr<const T>)lv.geo.a[a].plo;
+            geo.a[a].phi = (lds_ptr<const T>)lv.geo.a[a].phi;
+        }
+        p = (lds_ptr<T>)lv.p;
+        q = (lds_ptr<T>)lv.q;
+        beta = (lds_ptr<const T>)lv.beta;
+    }
+};
+
+template <class T, bool LDS = false>
+__device__ void tail_smooth(const TailLevel<T>& lvl, int sweeps, bool first, bool reverse = false)
 {
-    const LevelDev<T>& L = lv.geo;
+    const TailView<T, LDS> lv(lvl);
+    const auto& L = lv.geo;
     const int half = L.nx >> 1;
     const int rows = L.ny * L.nz;
     const int work = half * rows;
@@ -839,10 +905,11 @@ __device__ void tail_smooth(const TailLevel<T>& lv, int sweeps, bool first, bool
     }
 }
 
-template <class T>
-__device__ void tail_resid_restrict(const TailLevel<T>& lv, const TailLevel<T>& lc)
+template <class T, bool LDS = false>
+__device__ void tail_resid_restrict(const TailLevel<T>& lvl, const TailLevel<T>& lcl)
 {
-    const LevelDev<T>& L = lv.geo;
+    const TailView<T, LDS> lv(lvl), lc(lcl);
+    const auto& L = lv.geo;
     const int ncx = L.nx >> 1, ncy = L.ny >> 1, ncz = L.nz >> 1;
     const int cells = ncx * ncy * ncz;
     for (int t = threadIdx.x; t < cells; t += blockDim.x) {
@@ -853,10 +920,11 @@ __device__ void tail_resid_restrict(const TailLevel<T>& lv, const TailLevel<T>& 
     __syncthreads();
 }
 
-template <class T>
-__device__ void tail_prolong(const TailLevel<T>& lv, const TailLevel<T>& lc)
+template <class T, bool LDS = false>
+__device__ void tail_prolong(const TailLevel<T>& lvl, const TailLevel<T>& lcl)
 {
-    const LevelDev<T>& L = lv.geo;
+    const TailView<T, LDS> lv(lvl), lc(lcl);
+    const auto& L = lv.geo;
     const int cells = L.nx * L.ny * L.nz;
     for (int t = threadIdx.x; t < cells; t += blockDim.x) {
         const int i = t % L.nx, r = t / L.nx;
@@ -866,15 +934,28 @@ __device__ void tail_prolong(const TailLevel<T>& lv, const TailLevel<T>& lc)
     __syncthreads();
 }
 
-template <class T>
-__device__ void tail_dense(const TailPlan<T>& P, const TailLevel<T>& lv)
+// x = G q on the bottom level: S threads per row (S a power of two, at most
+// 64, as many as the workgroup allows) each sum every S-th product of the row,
+// then a butterfly over the S lanes; a row's loads are coalesced and in flight
+// together instead of one dependent load per column.
+template <class T, bool LDS = false>
+__device__ void tail_dense(const TailPlan<T>& P, const TailLevel<T>& lvl)
 {
+    const TailView<T, LDS> lv(lvl);
     const int nb = lv.geo.nx * lv.geo.ny * lv.geo.nz;
-    for (int r = threadIdx.x; r < nb; r += blockDim.x) {
+    int S = 1;
+    while (S < 64 && nb * (S * 2) <= (int)blockDim.x) S *= 2;
+    const int rows = (int)blockDim.x / S;
+    const int l = (int)threadIdx.x & (S - 1);
+    for (int r0 = 0; r0 < nb; r0 += rows) {
+        const int r = r0 + (int)threadIdx.x / S;
         double acc = 0.0;
-        const double* g = P.G + (size_t)r * nb;
-        for (int c = 0; c < nb; ++c) acc += g[c] * (double)lv.q[c];
-        lv.p[r] = (T)acc;
+        if (r < nb) {
+            const double* g = P.G + (size_t)r * nb;
+            for (int c = l; c < nb; c += S) acc += g[c] * (double)lv.q[c];
+        }
+        for (int off = S >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (r < nb && l == 0) lv.p[r] = (T)acc;
     }
     __syncthreads();
 }
@@ -1246,20 +1327,25 @@ template int launch_inner<double>(InnerOp, const LevelDev<double>&, const InnerA
 // V-cycle over tail levels top..bot (multiGrid.f90:186-232) with the coarsest
 // level solved by tail_coarse_pcg (or the dense solve when it is the bottom;
 // COARSE_CG false: always, and the inner CG is not compiled in).
-template <class T, bool COARSE_CG>
+template <class T, bool COARSE_CG, bool LDS = false>
 __device__ void tail_vcycle(const TailPlan<T>& P, int top, int bot, bool reset_top)
 {
+    static_assert(!(COARSE_CG && LDS), "the inner coarsest-level CG keeps its vectors in HBM");
     for (int l = top; l < bot; ++l) {
-        tail_smooth<T>(P.lv[l], P.n_pre, (l > top) || reset_top);
-        tail_resid_restrict<T>(P.lv[l], P.lv[l + 1]);
+        tail_smooth<T, LDS>(P.lv[l], P.n_pre, (l > top) || reset_top);
+        tail_resid_restrict<T, LDS>(P.lv[l], P.lv[l + 1]);
     }
-    if (COARSE_CG && P.bottom > P.user_last)
-        tail_coarse_pcg<T>(P);
-    else
-        tail_dense<T>(P, P.lv[bot]);
+    if constexpr (COARSE_CG) {
+        if (P.bottom > P.user_last)
+            tail_coarse_pcg<T>(P);
+        else
+            tail_dense<T>(P, P.lv[bot]);
+    } else {
+        tail_dense<T, LDS>(P, P.lv[bot]);
+    }
     for (int l = bot - 1; l >= top; --l) {
-        tail_prolong<T>(P.lv[l], P.lv[l + 1]);
-        tail_smooth<T>(P.lv[l], P.n_post, false, P.rev_post != 0);
+        tail_prolong<T, LDS>(P.lv[l], P.lv[l + 1]);
+        tail_smooth<T, LDS>(P.lv[l], P.n_post, false, P.rev_post != 0);
     }
 }
 
@@ -1292,6 +1378,13 @@ __global__ void __launch_bounds__(1024) k_tail_lds(const TailPlan<T>* __restrict
 {
     extern __shared__ __align__(16) unsigned char tail_smem[];
     __shared__ TailPlan<T> Ps;
+    // the arrays staged from HBM: source, destination (elements from the LDS
+    // base), and the first element of each in the concatenated list
+    constexpr int kSegs = 2 * kMaxTail + 1;
+    __shared__ const T* seg_src[kSegs];
+    __shared__ int seg_dst[kSegs];
+    __shared__ int seg_first[kSegs + 1];
+    __shared__ int nseg;
     if (done && *done) return;
     {
         static_assert(sizeof(TailPlan<T>) % 4 == 0, "plan copied as 32-bit words");
@@ -1321,46 +1414,77 @@ __global__ void __launch_bounds__(1024) k_tail_lds(const TailPlan<T>* __restrict
             }
             base += (tail_table_len(Ps, m) + 3) & ~3;
         }
+        // beta and the tables of every level, q of the first
+        const T* b0 = reinterpret_cast<const T*>(tail_smem);
+        int ns = 0, first = 0;
+        auto add = [&](const T* src, const T* dst, int n) {
+            seg_src[ns] = src;
+            seg_dst[ns] = (int)(dst - b0);
+            seg_first[ns] = first;
+            first += n;
+            ++ns;
+        };
+        for (int m = 0; m < Ps.nlev; ++m) {
+            add(plan->lv[m].beta, Ps.lv[m].beta, Ps.lv[m].geo.nx * Ps.lv[m].geo.ny * Ps.lv[m].geo.nz);
+            add(plan->lv[m].geo.a[0].ip, Ps.lv[m].geo.a[0].ip, tail_table_len(Ps, m));
+        }
+        add(plan->lv[0].q, Ps.lv[0].q, Ps.lv[0].geo.nx * Ps.lv[0].geo.ny * Ps.lv[0].geo.nz);
+        seg_first[ns] = first;
+        nseg = ns;
     }
     __syncthreads();
-    for (int m = 0; m < Ps.nlev; ++m) {
-        const int n = Ps.lv[m].geo.nx * Ps.lv[m].geo.ny * Ps.lv[m].geo.nz;
-        T* b = const_cast<T*>(Ps.lv[m].beta);
-        const T* gb = plan->lv[m].beta;
-        for (int t = threadIdx.x; t < n; t += blockDim.x) b[t] = gb[t];
-        T* tb = const_cast<T*>(Ps.lv[m].geo.a[0].ip);
-        const T* gt = plan->lv[m].geo.a[0].ip;
-        const int nt = tail_table_len(Ps, m);
-        for (int t = threadIdx.x; t < nt; t += blockDim.x) tb[t] = gt[t];
-        if (m == 0) {
-            const T* gq = plan->lv[0].q;
-            for (int t = threadIdx.x; t < n; t += blockDim.x) Ps.lv[0].q[t] = gq[t];
+    {
+        // every thread issues the loads of up to kStage elements of the list
+        // before it stores any, so the HBM/L2 latency is paid once per round
+        // instead of once per array
+        constexpr int kStage = sizeof(T) == 4 ? 12 : 6;
+        const lds_ptr<T> sm = (lds_ptr<T>)reinterpret_cast<T*>(tail_smem);
+        const int total = seg_first[nseg];
+        for (int e0 = 0; e0 < total; e0 += kStage * (int)blockDim.x) {
+            T v[kStage];
+            int dst[kStage];
+            int sg = 0;
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int e = e0 + u * (int)blockDim.x + (int)threadIdx.x;
+                dst[u] = -1;
+                if (e < total) {
+                    while (e >= seg_first[sg + 1]) ++sg;  // e grows with u
+                    const int o = e - seg_first[sg];
+                    v[u] = seg_src[sg][o];
+                    dst[u] = seg_dst[sg] + o;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kStage; ++u)
+                if (dst[u] >= 0) sm[dst[u]] = v[u];
         }
     }
     __syncthreads();
-    tail_vcycle<T, false>(Ps, 0, Ps.user_last, true);
+    tail_vcycle<T, false, true>(Ps, 0, Ps.user_last, true);
     const int n0 = Ps.lv[0].geo.nx * Ps.lv[0].geo.ny * Ps.lv[0].geo.nz;
     T* gp = plan->lv[0].p;
-    for (int t = threadIdx.x; t < n0; t += blockDim.x) gp[t] = Ps.lv[0].p[t];
+    const lds_ptr<T> p0 = (lds_ptr<T>)Ps.lv[0].p;
+    for (int t = threadIdx.x; t < n0; t += blockDim.x) gp[t] = p0[t];
 }
 
 // Dense bottom operator: G = (M + c 1 1^T)^-1 (I - 1 1^T / n) (-V), with
 // M = -V A the V-scaled symmetric matrix of poissMat.f90:746-763.  x = G q is the
 // Lagrange-constrained exact solve (poissMat.f90:246-259,467-503, sum x = 0).
 // Gauss-Jordan on the symmetric positive definite M + c 1 1^T, one workgroup.
-template <class T>
-__global__ void __launch_bounds__(1024) k_dense_setup(LevelDev<double> L, const T* __restrict__ beta_t,
-                                                     const double* __restrict__ vol, double* __restrict__ W,
-                                                     double* __restrict__ G)
+// A, B: the n x n work matrices, in HBM (W) or in LDS (bottoms of at most
+// kDenseLdsCells cells); int indices (n <= 512).
+template <class T, class PA>
+__device__ void dense_setup_body(const LevelDev<double>& L, const T* __restrict__ beta_t,
+                                 const double* __restrict__ vol, double* __restrict__ bet, PA A, PA B,
+                                 double* __restrict__ G)
 {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int nb = nx * ny * nz;
-    double* A = W;                        // n x n: M + c 1 1^T, reduced to the identity
-    double* B = W + (size_t)nb * nb;      // n x n: identity, becomes the inverse
-    double* bet = W + 2 * (size_t)nb * nb;
+    const int nn = nb * nb;
     __shared__ double s_trace;
     for (int t = threadIdx.x; t < nb; t += blockDim.x) bet[t] = (double)beta_t[t];
-    for (size_t t = threadIdx.x; t < (size_t)nb * nb; t += blockDim.x) {
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) {
         A[t] = 0.0;
         B[t] = ((t / nb) == (t % nb)) ? 1.0 : 0.0;
     }
@@ -1368,56 +1492,80 @@ __global__ void __launch_bounds__(1024) k_dense_setup(LevelDev<double> L, const 
     // rows of M: for each face with neighbour nb != c: M[c][nb] -= V a_f, M[c][c] += V a_f
     for (int c = threadIdx.x; c < nb; c += blockDim.x) {
         const int i = c % nx, r = c / nx, j = r % ny, k = r / ny;
-        size_t cc, e, w, n, s, f, b;
+        int cc, e, w, n, s, f, b;
         neighbours(i, j, k, nx, ny, nz, cc, e, w, n, s, f, b);
         Coef<double> a = coefs<double>(L, bet, i, j, k, cc, e, w, n, s, f, b);
         const double v = vol[c];
-        const size_t nbr[6] = {e, w, n, s, f, b};
+        const int nbr[6] = {e, w, n, s, f, b};
         const double af[6] = {a.aR, a.aL, a.aT, a.aB, a.aF, a.aK};
         for (int q = 0; q < 6; ++q) {
             if (nbr[q] == cc) continue;
-            A[(size_t)c * nb + nbr[q]] -= v * af[q];
-            A[(size_t)c * nb + c] += v * af[q];
+            A[c * nb + nbr[q]] -= v * af[q];
+            A[c * nb + c] += v * af[q];
         }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double tr = 0.0;
-        for (int c = 0; c < nb; ++c) tr += A[(size_t)c * nb + c];
+        for (int c = 0; c < nb; ++c) tr += A[c * nb + c];
         s_trace = (tr > 0.0 ? tr : 1.0) / ((double)nb * nb);
     }
     __syncthreads();
     const double cshift = s_trace;
-    for (size_t t = threadIdx.x; t < (size_t)nb * nb; t += blockDim.x) A[t] += cshift;
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) A[t] += cshift;
     __syncthreads();
     // Gauss-Jordan without pivoting (SPD)
     for (int piv = 0; piv < nb; ++piv) {
-        const double inv = 1.0 / A[(size_t)piv * nb + piv];
+        const double inv = 1.0 / A[piv * nb + piv];
         __syncthreads();
         for (int c = threadIdx.x; c < nb; c += blockDim.x) {
-            A[(size_t)piv * nb + c] *= inv;
-            B[(size_t)piv * nb + c] *= inv;
+            A[piv * nb + c] *= inv;
+            B[piv * nb + c] *= inv;
         }
         __syncthreads();
-        for (size_t t = threadIdx.x; t < (size_t)nb * nb; t += blockDim.x) {
-            const int r = (int)(t / nb), c = (int)(t % nb);
+        for (int t = threadIdx.x; t < nn; t += blockDim.x) {
+            const int r = t / nb, c = t - r * nb;
             if (r == piv) continue;
-            const double fct = A[(size_t)r * nb + piv];
-            if (c != piv) A[t] -= fct * A[(size_t)piv * nb + c];
-            B[t] -= fct * B[(size_t)piv * nb + c];
+            const double fct = A[r * nb + piv];
+            if (c != piv) A[t] -= fct * A[piv * nb + c];
+            B[t] -= fct * B[piv * nb + c];
         }
         __syncthreads();
         for (int r = threadIdx.x; r < nb; r += blockDim.x)
-            if (r != piv) A[(size_t)r * nb + piv] = 0.0;
+            if (r != piv) A[r * nb + piv] = 0.0;
         __syncthreads();
     }
     // G[i][j] = -V_j (B[i][j] - rowmean_i(B))
     for (int r = threadIdx.x; r < nb; r += blockDim.x) {
         double s = 0.0;
-        for (int c = 0; c < nb; ++c) s += B[(size_t)r * nb + c];
+        for (int c = 0; c < nb; ++c) s += B[r * nb + c];
         s /= (double)nb;
-        for (int c = 0; c < nb; ++c) G[(size_t)r * nb + c] = -vol[c] * (B[(size_t)r * nb + c] - s);
+        for (int c = 0; c < nb; ++c) G[(size_t)r * nb + c] = -vol[c] * (B[r * nb + c] - s);
     }
+}
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_dense_setup(LevelDev<double> L, const T* __restrict__ beta_t,
+                                                     const double* __restrict__ vol, double* __restrict__ W,
+                                                     double* __restrict__ G)
+{
+    const int nb = L.nx * L.ny * L.nz;
+    double* A = W;                        // n x n: M + c 1 1^T, reduced to the identity
+    double* B = W + (size_t)nb * nb;      // n x n: identity, becomes the inverse
+    dense_setup_body<T>(L, beta_t, vol, W + 2 * (size_t)nb * nb, A, B, G);
+}
+
+// the same with A and B in LDS (dynamic, 16 n^2 bytes)
+constexpr int kDenseLdsCells = 63;
+template <class T>
+__global__ void __launch_bounds__(1024) k_dense_setup_lds(LevelDev<double> L, const T* __restrict__ beta_t,
+                                                         const double* __restrict__ vol, double* __restrict__ W,
+                                                         double* __restrict__ G)
+{
+    extern __shared__ double dense_smem[];
+    const int nb = L.nx * L.ny * L.nz;
+    const lds_ptr<double> A = (lds_ptr<double>)dense_smem;
+    dense_setup_body<T>(L, beta_t, vol, W + 2 * (size_t)nb * nb, A, A + nb * nb, G);
 }
 
 // ------------------------------------------------------------ host launches
@@ -1706,7 +1854,12 @@ template <class T>
 int launch_dense_setup(const LevelDev<double>& L, const T* beta, const double* vol, double* work, double* G,
                        void* stream)
 {
-    hipLaunchKernelGGL(k_dense_setup<T>, dim3(1), dim3(1024), 0, S(stream), L, beta, vol, work, G);
+    const int nb = L.nx * L.ny * L.nz;
+    if (nb <= kDenseLdsCells)
+        hipLaunchKernelGGL(k_dense_setup_lds<T>, dim3(1), dim3(1024), (size_t)nb * nb * 2 * sizeof(double), S(stream),
+                           L, beta, vol, work, G);
+    else
+        hipLaunchKernelGGL(k_dense_setup<T>, dim3(1), dim3(1024), 0, S(stream), L, beta, vol, work, G);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }

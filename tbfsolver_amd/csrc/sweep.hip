@@ -207,8 +207,15 @@ __global__ void __launch_bounds__(NT)
     const bool wall_s = gy == 0, wall_n = gy == ny - 1;
     const size_t pair_off = (size_t)nx * gy + tx0 + 2 * h;
 
-    // planes of p and beta at kp, of q at kp - 1, into registers
+    // planes of p and beta at kp, of q at kp - 1, into registers.  The last
+    // planes a chunk needs are p and beta at k1 + 1 and q at k1; a fetch for a
+    // later plane step (the prefetch of the last steps, the red-only steps
+    // that round the march up to a group of 4) loads that plane again, a
+    // cache hit, where a load of the next planes would cost HBM traffic for
+    // values no cell uses.
+    const int klast = k1 + 1;
     auto fetch_pb = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], int kp) {
+        kp = min(kp, klast);
         const unsigned so = (unsigned)(plane_sz * modz(kp) * sizeof(T));
 #pragma unroll
         for (int m = 0; m < Sh::NP; ++m) {
@@ -220,6 +227,7 @@ __global__ void __launch_bounds__(NT)
     // black plane of the step that stashes these registers)
     auto fetch = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], T(&rq)[Sh::NQ], double2& dr, double2& dq, int kp) {
         fetch_pb(rp, rb, kp);
+        kp = min(kp, klast);
         const unsigned so = (unsigned)(plane_sz * modz(kp - 1) * sizeof(TQ));
 #pragma unroll
         for (int m = 0; m < Sh::NQ; ++m) rq[m] = (T)buf_load<TQ>(rs_q, offq[m], so);
