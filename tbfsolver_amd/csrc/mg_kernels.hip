@@ -42,6 +42,11 @@ __device__ __forceinline__ void neighbours(int i, int j, int k, int nx, int ny, 
 // with 32-bit addresses rather than generic (flat) loads.
 template <class T>
 using lds_ptr = __attribute__((address_space(3))) T*;
+// A pointer typed as a global address: its loads count only against the
+// vector-memory counter, so waiting for an LDS read does not also wait for them
+// (a generic load counts against both).
+template <class T>
+using glb_ptr = __attribute__((address_space(1))) T*;
 template <class T>
 struct AxisLds {
     lds_ptr<const T> ip, im, rlo, rhi, plo, phi;
@@ -585,31 +590,37 @@ __global__ void k_rbgs(LevelDev<T> L, const T* __restrict__ beta, const TQ* __re
 // one thread per coarse cell, qc = sum_{children} wx wy wz (q - A p).  The
 // plane loop stays rolled: all eight children in flight need more registers
 // than a 1024-thread workgroup has, and the spill costs more than the overlap.
+// resid_restrict_plane: the four children in fine plane 2K + dk, added to acc.
+template <class T, class TQ, class LV, class PB, class PQ, class PP>
+__device__ __forceinline__ T resid_restrict_plane(const LV& L, PB __restrict__ beta, PQ __restrict__ q,
+                                                  PP __restrict__ p, int I, int J, int K, int dk, T acc)
+{
+    const int k = 2 * K + dk;
+    const T wz = dk ? L.a[2].rhi[K] : L.a[2].rlo[K];
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+        const int j = 2 * J + dj;
+        const T wy = dj ? L.a[1].rhi[J] : L.a[1].rlo[J];
+#pragma unroll
+        for (int di = 0; di < 2; ++di) {
+            const int i = 2 * I + di;
+            const T wx = di ? L.a[0].rhi[I] : L.a[0].rlo[I];
+            typename IdxOf<LV>::type c, e, w, n, s, f, b;
+            neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
+            Coef<T> a = coefs<T>(L, beta, i, j, k, c, e, w, n, s, f, b);
+            const T res = (T)q[c] - apply_cell<T>(a, p, c, e, w, n, s, f, b);
+            acc += (wx * wy * wz) * res;
+        }
+    }
+    return acc;
+}
 template <class T, class TQ, class LV, class PB, class PQ, class PP>
 __device__ __forceinline__ T resid_restrict_cell(const LV& L, PB __restrict__ beta, PQ __restrict__ q,
                                                  PP __restrict__ p, int I, int J, int K)
 {
     T acc = 0;
 #pragma unroll 1
-    for (int dk = 0; dk < 2; ++dk) {
-        const int k = 2 * K + dk;
-        const T wz = dk ? L.a[2].rhi[K] : L.a[2].rlo[K];
-#pragma unroll
-        for (int dj = 0; dj < 2; ++dj) {
-            const int j = 2 * J + dj;
-            const T wy = dj ? L.a[1].rhi[J] : L.a[1].rlo[J];
-#pragma unroll
-            for (int di = 0; di < 2; ++di) {
-                const int i = 2 * I + di;
-                const T wx = di ? L.a[0].rhi[I] : L.a[0].rlo[I];
-                typename IdxOf<LV>::type c, e, w, n, s, f, b;
-                neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, n, s, f, b);
-                Coef<T> a = coefs<T>(L, beta, i, j, k, c, e, w, n, s, f, b);
-                const T res = (T)q[c] - apply_cell<T>(a, p, c, e, w, n, s, f, b);
-                acc += (wx * wy * wz) * res;
-            }
-        }
-    }
+    for (int dk = 0; dk < 2; ++dk) acc = resid_restrict_plane<T, TQ>(L, beta, q, p, I, J, K, dk, acc);
     return acc;
 }
 
@@ -912,10 +923,16 @@ __device__ void tail_resid_restrict(const TailLevel<T>& lvl, const TailLevel<T>&
     const auto& L = lv.geo;
     const int ncx = L.nx >> 1, ncy = L.ny >> 1, ncz = L.nz >> 1;
     const int cells = ncx * ncy * ncz;
-    for (int t = threadIdx.x; t < cells; t += blockDim.x) {
+    // two neighbouring lanes per coarse cell, one per fine plane (the chains
+    // of dependent loads are half as long); the cell's value is the sum of the
+    // two plane sums, the same on both lanes
+    for (int t2 = threadIdx.x; t2 < 2 * cells; t2 += blockDim.x) {
+        const int t = t2 >> 1, dk = t2 & 1;
         const int I = t % ncx, r = t / ncx;
         const int J = r % ncy, K = r / ncy;
-        lc.q[t] = resid_restrict_cell<T, T>(L, lv.beta, lv.q, lv.p, I, J, K);
+        const T part = resid_restrict_plane<T, T>(L, lv.beta, lv.q, lv.p, I, J, K, dk, (T)0);
+        const T other = __shfl_xor(part, 1, 64);
+        if (dk == 0) lc.q[t] = part + other;
     }
     __syncthreads();
 }
@@ -951,7 +968,7 @@ __device__ void tail_dense(const TailPlan<T>& P, const TailLevel<T>& lvl)
         const int r = r0 + (int)threadIdx.x / S;
         double acc = 0.0;
         if (r < nb) {
-            const double* g = P.G + (size_t)r * nb;
+            const glb_ptr<const double> g = (glb_ptr<const double>)(P.G + (size_t)r * nb);
             for (int c = l; c < nb; c += S) acc += g[c] * (double)lv.q[c];
         }
         for (int off = S >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
@@ -1357,17 +1374,6 @@ __global__ void __launch_bounds__(1024) k_tail(const TailPlan<T>* __restrict__ p
     tail_vcycle<T, true>(P, 0, P.user_last, true);
 }
 
-// Entries of a level's table block (LevelTables::build: ip, im, plo, phi of n
-// and rlo, rhi of the next level's n per axis).
-template <class T>
-__device__ __forceinline__ int tail_table_len(const TailPlan<T>& P, int m)
-{
-    const LevelDev<T>& g = P.lv[m].geo;
-    const bool c = m + 1 < P.nlev;
-    const LevelDev<T>& h = P.lv[c ? m + 1 : m].geo;
-    return 4 * (g.nx + g.ny + g.nz) + (c ? 2 * (h.nx + h.ny + h.nz) : 0);
-}
-
 // The same V-cycle with p, q, beta and the metric tables of every tail level
 // held in LDS (dynamic shared memory sized by the host): every phase then
 // costs LDS latency instead of an L2 round trip.  beta, the tables and q of the
@@ -1378,13 +1384,6 @@ __global__ void __launch_bounds__(1024) k_tail_lds(const TailPlan<T>* __restrict
 {
     extern __shared__ __align__(16) unsigned char tail_smem[];
     __shared__ TailPlan<T> Ps;
-    // the arrays staged from HBM: source, destination (elements from the LDS
-    // base), and the first element of each in the concatenated list
-    constexpr int kSegs = 2 * kMaxTail + 1;
-    __shared__ const T* seg_src[kSegs];
-    __shared__ int seg_dst[kSegs];
-    __shared__ int seg_first[kSegs + 1];
-    __shared__ int nseg;
     if (done && *done) return;
     {
         static_assert(sizeof(TailPlan<T>) % 4 == 0, "plan copied as 32-bit words");
@@ -1393,53 +1392,33 @@ __global__ void __launch_bounds__(1024) k_tail_lds(const TailPlan<T>* __restrict
         for (unsigned t = threadIdx.x; t < sizeof(TailPlan<T>) / 4; t += blockDim.x) dst[t] = src[t];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        T* base = reinterpret_cast<T*>(tail_smem);
-        for (int m = 0; m < Ps.nlev; ++m) {
-            const int n = Ps.lv[m].geo.nx * Ps.lv[m].geo.ny * Ps.lv[m].geo.nz;
-            Ps.lv[m].p = base;
-            Ps.lv[m].q = base + n;
-            Ps.lv[m].beta = base + 2 * n;
-            base += 3 * n;
-            // the table block, every pointer moved by the same offset
-            const T* g0 = Ps.lv[m].geo.a[0].ip;
-            for (int a = 0; a < 3; ++a) {
-                AxisDev<T>& ax = Ps.lv[m].geo.a[a];
-                ax.ip = base + (ax.ip - g0);
-                ax.im = base + (ax.im - g0);
-                ax.rlo = base + (ax.rlo - g0);
-                ax.rhi = base + (ax.rhi - g0);
-                ax.plo = base + (ax.plo - g0);
-                ax.phi = base + (ax.phi - g0);
-            }
-            base += (tail_table_len(Ps, m) + 3) & ~3;
+    T* const base = reinterpret_cast<T*>(tail_smem);
+    // level m's pointers into LDS (thread m), the table pointers moved by the
+    // offset of the block
+    if ((int)threadIdx.x < Ps.nlev) {
+        TailLevel<T>& lv = Ps.lv[threadIdx.x];
+        const T* g0 = lv.geo.a[0].ip;
+        T* tb = base + lv.lds_tab;
+        for (int a = 0; a < 3; ++a) {
+            AxisDev<T>& ax = lv.geo.a[a];
+            ax.ip = tb + (ax.ip - g0);
+            ax.im = tb + (ax.im - g0);
+            ax.rlo = tb + (ax.rlo - g0);
+            ax.rhi = tb + (ax.rhi - g0);
+            ax.plo = tb + (ax.plo - g0);
+            ax.phi = tb + (ax.phi - g0);
         }
-        // beta and the tables of every level, q of the first
-        const T* b0 = reinterpret_cast<const T*>(tail_smem);
-        int ns = 0, first = 0;
-        auto add = [&](const T* src, const T* dst, int n) {
-            seg_src[ns] = src;
-            seg_dst[ns] = (int)(dst - b0);
-            seg_first[ns] = first;
-            first += n;
-            ++ns;
-        };
-        for (int m = 0; m < Ps.nlev; ++m) {
-            add(plan->lv[m].beta, Ps.lv[m].beta, Ps.lv[m].geo.nx * Ps.lv[m].geo.ny * Ps.lv[m].geo.nz);
-            add(plan->lv[m].geo.a[0].ip, Ps.lv[m].geo.a[0].ip, tail_table_len(Ps, m));
-        }
-        add(plan->lv[0].q, Ps.lv[0].q, Ps.lv[0].geo.nx * Ps.lv[0].geo.ny * Ps.lv[0].geo.nz);
-        seg_first[ns] = first;
-        nseg = ns;
+        lv.p = base + lv.lds_p;
+        lv.q = base + lv.lds_q;
+        lv.beta = base + lv.lds_beta;
     }
-    __syncthreads();
     {
         // every thread issues the loads of up to kStage elements of the list
         // before it stores any, so the HBM/L2 latency is paid once per round
         // instead of once per array
         constexpr int kStage = sizeof(T) == 4 ? 12 : 6;
-        const lds_ptr<T> sm = (lds_ptr<T>)reinterpret_cast<T*>(tail_smem);
-        const int total = seg_first[nseg];
+        const lds_ptr<T> sm = (lds_ptr<T>)base;
+        const int total = Ps.seg_first[Ps.nseg];
         for (int e0 = 0; e0 < total; e0 += kStage * (int)blockDim.x) {
             T v[kStage];
             int dst[kStage];
@@ -1449,10 +1428,10 @@ __global__ void __launch_bounds__(1024) k_tail_lds(const TailPlan<T>* __restrict
                 const int e = e0 + u * (int)blockDim.x + (int)threadIdx.x;
                 dst[u] = -1;
                 if (e < total) {
-                    while (e >= seg_first[sg + 1]) ++sg;  // e grows with u
-                    const int o = e - seg_first[sg];
-                    v[u] = seg_src[sg][o];
-                    dst[u] = seg_dst[sg] + o;
+                    while (e >= Ps.seg_first[sg + 1]) ++sg;  // e grows with u
+                    const int o = e - Ps.seg_first[sg];
+                    v[u] = ((glb_ptr<const T>)Ps.seg_src[sg])[o];
+                    dst[u] = Ps.seg_dst[sg] + o;
                 }
             }
 #pragma unroll
@@ -1463,7 +1442,7 @@ __global__ void __launch_bounds__(1024) k_tail_lds(const TailPlan<T>* __restrict
     __syncthreads();
     tail_vcycle<T, false, true>(Ps, 0, Ps.user_last, true);
     const int n0 = Ps.lv[0].geo.nx * Ps.lv[0].geo.ny * Ps.lv[0].geo.nz;
-    T* gp = plan->lv[0].p;
+    const glb_ptr<T> gp = (glb_ptr<T>)plan->lv[0].p;
     const lds_ptr<T> p0 = (lds_ptr<T>)Ps.lv[0].p;
     for (int t = threadIdx.x; t < n0; t += blockDim.x) gp[t] = p0[t];
 }

@@ -65,7 +65,12 @@ struct TailLevel {
     T* p;
     T* q;
     const T* beta;
+    // the level's place in the LDS-resident tail (elements from the dynamic
+    // LDS base): p, q, beta, the table block
+    int lds_p, lds_q, lds_beta, lds_tab;
 };
+
+constexpr int kTailSegs = 2 * kMaxTail + 1;
 
 template <class T>
 struct TailPlan {
@@ -81,6 +86,13 @@ struct TailPlan {
     const double* G;   // bottom dense operator, row-major: x = G q
     PcgState* st;      // receives the coarse-solve diagnostics (may be null)
     TailLevel<T> lv[kMaxTail];
+    // the arrays the LDS-resident tail stages from HBM (beta and the tables of
+    // every level, q of the first): source, destination (elements from the
+    // LDS base), and the first element of each in the concatenated list
+    int nseg;
+    const T* seg_src[kTailSegs];
+    int seg_dst[kTailSegs];
+    int seg_first[kTailSegs + 1];
 };
 
 }  // namespace tbfg

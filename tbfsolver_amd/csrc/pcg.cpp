@@ -171,20 +171,39 @@ static int build_plan(tbfg_pcg* h)
     P.cvol = h->cvol.as<T>();
     P.G = h->G.as<double>();
     P.st = h->st.as<PcgState>();
+    // LDS layout of the LDS-resident tail: per level p, q, beta, then its
+    // table block (LevelTables::build), padded to 4 entries
     size_t lds = 0;
+    int first = 0;
+    auto add_seg = [&](const T* src, size_t dst, size_t n) {
+        P.seg_src[P.nseg] = src;
+        P.seg_dst[P.nseg] = (int)dst;
+        P.seg_first[P.nseg] = first;
+        first += (int)n;
+        ++P.nseg;
+    };
     for (int m = 0; m < P.nlev; ++m) {
         const int l = h->t0 + m;
+        const size_t n = h->hl[l].cells();
         P.lv[m].geo = geo<T>(h, l);
         P.lv[m].p = h->p[l].as<T>();
         P.lv[m].q = h->q[l].as<T>();
         P.lv[m].beta = h->beta[l].as<T>();
-        lds += 3 * h->hl[l].cells() * sizeof(T);
-        // the level's table block (LevelTables::build), padded to 4 entries
+        const size_t e0 = lds / sizeof(T);
+        P.lv[m].lds_p = (int)e0;
+        P.lv[m].lds_q = (int)(e0 + n);
+        P.lv[m].lds_beta = (int)(e0 + 2 * n);
+        P.lv[m].lds_tab = (int)(e0 + 3 * n);
+        lds += 3 * n * sizeof(T);
         size_t nt = 0;
         for (int a = 0; a < 3; ++a)
             nt += 4 * (size_t)h->hl[l].n[a] + (m + 1 < P.nlev ? 2 * (size_t)h->hl[l + 1].n[a] : 0);
         lds += ((nt + 3) & ~(size_t)3) * sizeof(T);
+        add_seg(P.lv[m].beta, P.lv[m].lds_beta, n);
+        add_seg(P.lv[m].geo.a[0].ip, P.lv[m].lds_tab, nt);
     }
+    add_seg(P.lv[0].q, P.lv[0].lds_q, h->hl[h->t0].cells());
+    P.seg_first[P.nseg] = first;
     // the LDS-resident tail when its levels fit (the inner coarsest-level CG
     // keeps its vectors in HBM and uses the other form)
     h->tail_lds = 0;
