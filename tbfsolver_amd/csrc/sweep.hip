@@ -705,6 +705,7 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
     int chunks = (target + ntiles - 1) / ntiles;
     int kc = (L.nz + chunks - 1) / chunks;
     // kc + 2 plane steps run in groups of 4: kc = 4m + 2, from 6 to kSweepMaxChunk
+    if (const char* e = std::getenv("TBFG_SWEEP_KC")) kc = std::atoi(e);  // A/B switch (planes per chunk)
     kc = ((kc + 2 + 3) / 4) * 4 - 2;
     if (kc < 6) kc = 6;
     if (kc > kSweepMaxChunk) kc = kSweepMaxChunk;
