@@ -117,6 +117,13 @@ typedef struct tbfg_pcg_params {
                                 TBFG_PRECOND_FFT: z = L^-1 r with L the constant-
                                 coefficient operator of the FFT solver (FP32 FFTs in
                                 mixed precision); x, z uniform, nx, nz 2^a 3^b      */
+    int32_t mg_max_iter;     /* maxIterMG: most V-cycles per application (mg_tol > 0) */
+    double mg_tol;           /* tolMG (multiGrid.f90:165-182, 236-279): > 0 each
+                                application runs V-cycles until the finest level's
+                                ||q - A p|| / ||q|| after its last smoothing is at
+                                most mg_tol (at least one cycle, at most
+                                mg_max_iter; channel_18/specs/pcg_solver: 0.4, 100);
+                                0: mg_cycles cycles.  Must be < 1.                  */
 } tbfg_pcg_params;
 
 #define TBFG_PRECOND_MG 0
@@ -170,6 +177,14 @@ int tbfg_pcg_profile_begin(tbfg_pcg* h, int32_t kernel);
 /* Diagnostics of the handle's last solve: coarsest-level solves made, their
  * inner iterations (sum and maximum) and the worst final relative residual. */
 int tbfg_pcg_coarse_stats(const tbfg_pcg* h, int32_t* calls, int32_t* iters, int32_t* iters_max, double* res_max);
+/* V-cycles of the last solve's preconditioner applications: the sum and the
+ * largest per application (the reference's multiGrid iter_, multiGrid.f90:236-279).
+ * A solve of n updates applies the preconditioner n times (the first, then
+ * after every update but the last, whose stop test comes first); the
+ * reference applies it once more, after the final update, before its test
+ * (pcg.f90:122-136).  After tbfg_pcg_precondition_dev with mg_tol > 0: that
+ * application's cycles. */
+int tbfg_pcg_mg_stats(const tbfg_pcg* h, int32_t* cycles_total, int32_t* cycles_max);
 int tbfg_pcg_profile_end(tbfg_pcg* h, int64_t* launches, double* total_ms, double* bytes_per_launch);
 
 int tbfg_pcg_destroy(tbfg_pcg* h);

@@ -24,7 +24,7 @@ class PcgParams(C.Structure):
     _fields_ = [("n_levels", C.c_int32), ("n_pre", C.c_int32), ("n_post", C.c_int32),
                 ("max_iter", C.c_int32), ("tol", C.c_double), ("mg_cycles", C.c_int32),
                 ("flexible", C.c_int32), ("coarse_mode", C.c_int32),
-                ("coarse_cycles", C.c_int32), ("pad", C.c_int32)]
+                ("coarse_cycles", C.c_int32), ("mg_max_iter", C.c_int32), ("mg_tol", C.c_double)]
 
 
 def lib():
@@ -57,12 +57,21 @@ def desc(n, L=(2 * np.pi, 2.0, np.pi), stretch=(0, 0, 0), gamma=1.4):
 
 
 def params(n_levels=4, n_pre=2, n_post=2, max_iter=200, tol=1e-8, mg_cycles=1, flexible=0,
-           coarse_mode=0, coarse_cycles=4):
+           coarse_mode=0, coarse_cycles=4, mg_tol=0.0, mg_max_iter=100):
     p = PcgParams()
     p.n_levels, p.n_pre, p.n_post, p.max_iter = n_levels, n_pre, n_post, max_iter
     p.tol, p.mg_cycles, p.flexible = tol, mg_cycles, flexible
     p.coarse_mode, p.coarse_cycles = coarse_mode, coarse_cycles
+    p.mg_tol, p.mg_max_iter = mg_tol, mg_max_iter
     return p
+
+
+def last_mg_cycles(with_last=False):
+    """(total, largest per application[, last application]) V-cycles of the
+    last pcg_solve / mg_apply."""
+    a, b, c = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+    lib().or_last_mg_cycles(C.byref(a), C.byref(b), C.byref(c))
+    return (int(a.value), int(b.value), int(c.value)) if with_last else (int(a.value), int(b.value))
 
 
 def shape(g, lev=0):

@@ -337,10 +337,18 @@ static int coarse_exact_lv(const lev_t* L, const double* beta, const double* q, 
                 dMd += d[t] * Md[t];
             }
             if (dMd <= 0.0) break;
-            double a = rr / dMd, rr2 = 0.0;
+            double a = rr / dMd, rr2 = 0.0, rm = 0.0;
             for (size_t t = 0; t < n; ++t) {
                 x[t] += a * d[t];
                 r[t] -= a * Md[t];
+                rm += r[t];
+            }
+            /* r stays in range(M) = {sum = 0}; rounding would otherwise leave a
+             * component along the null vector that CG cannot remove, x would
+             * drift along it and lose digits when its mean is taken out */
+            rm /= (double)n;
+            for (size_t t = 0; t < n; ++t) {
+                r[t] -= rm;
                 rr2 += r[t] * r[t];
             }
             if (rr2 <= 1e-30 * bb) break;
@@ -515,14 +523,49 @@ static void vcycle(mg_t* m, int l, int last, int reset)
     rbgs_sweeps(&m->L[l], m->beta[l], m->q[l], m->p[l], prm->n_post, 1);
 }
 
-/* multiGrid.f90:165-182 with a fixed number of V-cycles; the finest level starts
- * from zero (pcg.f90:118,128). */
+static int32_t g_mg_cycles_total = 0, g_mg_cycles_max = 0, g_mg_cycles_last = 0;
+
+void or_last_mg_cycles(int32_t* total, int32_t* max_per_apply, int32_t* last)
+{
+    *total = g_mg_cycles_total;
+    *max_per_apply = g_mg_cycles_max;
+    *last = g_mg_cycles_last;
+}
+
+/* multiGrid.f90:165-182 (solveMG); the finest level starts from zero
+ * (pcg.f90:118,128).  mg_tol = 0: a fixed number of V-cycles.  mg_tol > 0:
+ * continueIterating (multiGrid.f90:236-279) before every cycle -- stop after
+ * mg_max_iter cycles, else continue while the finest level's residual
+ * ||q - A p|| / (||q|| + tiny) exceeds tolMG.  That residual is the one
+ * solveRBGS leaves after the finest level's last smoothing (rbgs.f90:168-181,
+ * computeResiduals rbgs.f90:276-333); before the first cycle p = 0.  The
+ * right-hand side is PCG's r, assigned on the interior only (pcg.f90:117,349),
+ * so its halo adds nothing to the norm. */
 static void mg_apply_built(mg_t* m, const double* r, double* z)
 {
-    size_t n = lev_cells(&m->L[0]);
+    const lev_t* L = &m->L[0];
+    size_t n = lev_cells(L);
     memcpy(m->q[0], r, sizeof(double) * n);
     memset(m->p[0], 0, sizeof(double) * n);
-    for (int c = 0; c < m->prm->mg_cycles; ++c) vcycle(m, 0, m->nlev - 1, 0);
+    int cycles = 0;
+    if (m->prm->mg_tol > 0.0) {
+        const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
+        const double tiny = 2.2250738585072014e-308;
+        const double qn = sqrt(sum_planes(nx, ny, nz, m->q[0], NULL, 2));
+        double res = qn / (qn + tiny);
+        while (cycles != m->prm->mg_max_iter && res > m->prm->mg_tol) {
+            vcycle(m, 0, m->nlev - 1, 0);
+            ++cycles;
+            residual(L, m->beta[0], m->q[0], m->p[0], m->res[0]);
+            res = sqrt(sum_planes(nx, ny, nz, m->res[0], NULL, 2)) / (qn + tiny);
+        }
+    } else {
+        for (int c = 0; c < m->prm->mg_cycles; ++c) vcycle(m, 0, m->nlev - 1, 0);
+        cycles = m->prm->mg_cycles;
+    }
+    g_mg_cycles_total += cycles;
+    g_mg_cycles_last = cycles;
+    if (cycles > g_mg_cycles_max) g_mg_cycles_max = cycles;
     memcpy(z, m->p[0], sizeof(double) * n);
 }
 
@@ -530,6 +573,7 @@ int or_mg_apply(const or_grid_desc* g, const or_pcg_params* prm, const double* b
 {
     mg_t m;
     if (mg_build(&m, g, prm, beta)) return -1;
+    g_mg_cycles_total = g_mg_cycles_max = 0;
     mg_apply_built(&m, r, z);
     mg_free(&m);
     return 0;
@@ -544,6 +588,7 @@ int or_pcg_solve(const or_grid_desc* g, const or_pcg_params* prm, const double* 
 {
     mg_t m;
     if (mg_build(&m, g, prm, beta)) return -1;
+    g_mg_cycles_total = g_mg_cycles_max = 0;
     const lev_t* L = &m.L[0];
     const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
     const size_t n = lev_cells(L);

@@ -40,7 +40,9 @@ typedef struct {
     int32_t flexible;      /* 0: Fletcher-Reeves (pcg.f90:130-133), 1: Polak-Ribiere */
     int32_t coarse_mode;   /* 0: exact coarsest solve, 1: K inner cycles on a sub-hierarchy */
     int32_t coarse_cycles; /* K for coarse_mode 1                                 */
-    int32_t pad;
+    int32_t mg_max_iter;   /* maxIterMG (used when mg_tol > 0)                    */
+    double  mg_tol;        /* tolMG: > 0 runs solveMG's loop (V-cycles while the
+                              finest residual exceeds it), 0 mg_cycles fixed      */
 } or_pcg_params;
 
 /* grid coordinates with `ghost` extra layers: xc has n+2*ghost entries for
@@ -77,6 +79,11 @@ int  or_coarse_exact(const or_grid_desc* g, int32_t lev, const double* beta, con
 int or_pcg_solve(const or_grid_desc* g, const or_pcg_params* prm, const double* beta,
                  const double* q, double* psi, int32_t* iters, double* rel_res);
 /* one preconditioner application z = M r. */
+/* V-cycles of the preconditioner applications of the last or_pcg_solve /
+ * or_mg_apply call in this process: total, largest per application, and
+ * those of the last application (the one after the final update, which the
+ * reference makes before its stop test, pcg.f90:122-136). */
+void or_last_mg_cycles(int32_t* total, int32_t* max_per_apply, int32_t* last);
 int or_mg_apply(const or_grid_desc* g, const or_pcg_params* prm, const double* beta,
                 const double* r, double* z);
 

@@ -54,7 +54,8 @@ class PcgParams(C.Structure):
     _fields_ = [("n_levels", C.c_int32), ("n_pre", C.c_int32), ("n_post", C.c_int32),
                 ("max_iter", C.c_int32), ("tol", C.c_double), ("mg_cycles", C.c_int32),
                 ("precision", C.c_int32), ("coarse_max_iter", C.c_int32), ("flexible", C.c_int32),
-                ("coarse_tol", C.c_double), ("preconditioner", C.c_int32)]
+                ("coarse_tol", C.c_double), ("preconditioner", C.c_int32), ("mg_max_iter", C.c_int32),
+                ("mg_tol", C.c_double)]
 
 
 _P = C.c_void_p
@@ -83,6 +84,7 @@ SIGNATURES = {
     "tbfg_pcg_profile_begin": (C.c_int, [_P, C.c_int32]),
     "tbfg_pcg_coarse_stats": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                         C.POINTER(C.c_double)]),
+    "tbfg_pcg_mg_stats": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "tbfg_pcg_profile_end": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tbfg_apply_operator_dev": (C.c_int, [C.POINTER(GridDesc), _P, _P, _P, _P]),
     "tbfg_divergence_dev": (C.c_int, [C.POINTER(GridDesc), _P, _P, _P, _P, _P]),

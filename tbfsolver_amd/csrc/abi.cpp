@@ -196,6 +196,8 @@ void tbfg_pcg_default_params(tbfg_pcg_params* p)
     p->flexible = -1;
     p->coarse_tol = 0.0;
     p->preconditioner = TBFG_PRECOND_MG;
+    p->mg_max_iter = 100;
+    p->mg_tol = 0.0;
 }
 
 int tbfg_pcg_create(const tbfg_grid_desc* grid, const tbfg_pcg_params* prm, tbfg_pcg** out)
@@ -269,6 +271,17 @@ int tbfg_pcg_coarse_stats(const tbfg_pcg* h, int32_t* calls, int32_t* iters, int
     if (iters) *iters = h->hst->coarse_iters;
     if (iters_max) *iters_max = h->hst->coarse_iters_max;
     if (res_max) *res_max = h->hst->coarse_res_max;
+    return TBFG_OK;
+}
+
+int tbfg_pcg_mg_stats(const tbfg_pcg* h, int32_t* cycles_total, int32_t* cycles_max)
+{
+    if (!h || !h->hst) {
+        set_error("tbfg_pcg_mg_stats: no solve yet");
+        return TBFG_EINVAL;
+    }
+    if (cycles_total) *cycles_total = h->hst->mg_cyc_total;
+    if (cycles_max) *cycles_max = h->hst->mg_cyc_max;
     return TBFG_OK;
 }
 

@@ -1116,6 +1116,64 @@ __device__ void tail_coarse_pcg(const TailPlan<T>& P)
     __syncthreads();
 }
 
+// ---- solveMG's stop test (multiGrid.f90:236-279): after V-cycle `cyc` of a
+// preconditioner application, ||q - A p|| / (||q|| + tiny) on the finest level
+// (computeResiduals, rbgs.f90:276-333, in the multigrid's precision, summed in
+// FP64); the application stops at tolMG or after max_cycles cycles.
+template <class T, class TQ>
+__global__ void k_mg_res(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q,
+                         const T* __restrict__ p, const PcgState* st, Reducer red)
+{
+    double v[2] = {0.0, 0.0};
+    if (!st->done) {
+        const int n = L.nx * L.ny * L.nz;
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+            const int i = t % L.nx, rr = t / L.nx, j = rr % L.ny, k = rr / L.ny;
+            size_t c, e, w, nn, s, f, b;
+            neighbours(i, j, k, L.nx, L.ny, L.nz, c, e, w, nn, s, f, b);
+            Coef<T> a = coefs<T>(L, beta, i, j, k, c, e, w, nn, s, f, b);
+            const T qc = (T)q[c];
+            const T r = qc - apply_cell<T>(a, p, c, e, w, nn, s, f, b);
+            v[0] += (double)r * (double)r;
+            v[1] += (double)qc * (double)qc;
+        }
+    }
+    block_partial<2>(v, red);
+}
+__global__ void k_fin_mgres(unsigned nb, double tol, int cyc, int max_cycles, PcgState* st, Reducer red)
+{
+    double tot[2];
+    final_sum<2>(red, nb, tot);
+    if (threadIdx.x == 0) {
+        if (st->done) {
+            st->mg_stop = 1;
+            return;
+        }
+        const double res = sqrt(tot[0]) / (sqrt(tot[1]) + 2.2250738585072014e-308);
+        const int cycles = cyc + 1;
+        st->mg_cyc_total += 1;
+        if (cycles > st->mg_cyc_max) st->mg_cyc_max = cycles;
+        st->mg_stop = (cycles >= max_cycles || !(res > tol)) ? 1 : 0;
+    }
+}
+
+template <class T>
+int launch_mg_res(const LevelDev<T>& L, const T* beta, const T* q, const T* p, double tol, int cyc, int max_cycles,
+                  PcgState* st, const Reducer& red, void* stream)
+{
+    const size_t n = (size_t)L.nx * L.ny * L.nz;
+    // grid-stride: at most 2048 blocks (the reduction scratch holds 5 sums for that many)
+    const unsigned nb = std::min<unsigned>(ceil_div(n, 256), 2048u);
+    hipLaunchKernelGGL((k_mg_res<T, T>), dim3(nb), dim3(256), 0, (hipStream_t)stream, L, beta, q, p, st, red);
+    hipLaunchKernelGGL(k_fin_mgres, dim3(1), dim3(256), 0, (hipStream_t)stream, nb, tol, cyc, max_cycles, st, red);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+template int launch_mg_res<float>(const LevelDev<float>&, const float*, const float*, const float*, double, int, int,
+                                  PcgState*, const Reducer&, void*);
+template int launch_mg_res<double>(const LevelDev<double>&, const double*, const double*, const double*, double, int,
+                                   int, PcgState*, const Reducer&, void*);
+
 // ---- the coarsest-level solve by multi-workgroup kernels (tail_coarse_pcg's
 // algorithm, for a coarsest level too large for one workgroup): one thread per
 // cell, block partial sums, a one-block finalize per reduction.  Every kernel

@@ -37,6 +37,11 @@ struct PcgState {
     int32_t coarse_iters_max;
     int32_t pad2;
     double coarse_res_max;    // worst final relative residual
+    // V-cycles per preconditioner application under a residual target (tolMG)
+    int32_t mg_stop;       // 1: no further cycle in this application
+    int32_t mg_cyc_total;  // cycles of this solve
+    int32_t mg_cyc_max;    // most cycles in one application
+    int32_t pad3;
 };
 
 // The coarsest-level solve run by multi-workgroup kernels (levels too large for

@@ -67,6 +67,11 @@ template <class T, class TQ>
 int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
                  const int* done, void* stream, const SweepDots* dots = nullptr, unsigned* nblocks_out = nullptr,
                  const KernelEvents* kev = nullptr);
+// solveMG's stop test after V-cycle cyc: sets PcgState::mg_stop
+template <class T>
+int launch_mg_res(const LevelDev<T>& L, const T* beta, const T* q, const T* p, double tol, int cyc, int max_cycles,
+                  PcgState* st, const Reducer& red, void* stream);
+
 // the coarsest-level solve by multi-workgroup kernels, in phases driven by the
 // host: Start (project q, r = q, x = 0), FirstDir (d = z), OpUpdate (ad = A d,
 // alpha, x and r updated, q = r, stop test), NextDir (gamma, d = z + gamma d),

@@ -51,6 +51,7 @@ tbfg_pcg::~tbfg_pcg()
         if (e) (void)hipEventDestroy(e);
     if (hstage) (void)hipHostFree(hstage);
     if (hin) (void)hipHostFree(hin);
+    if (hmg) (void)hipHostFree(hmg);
     if (fps) tbfg::fps_destroy(fps);
 }
 
@@ -226,7 +227,8 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         return TBFG_EINVAL;
     }
     if (prm.n_pre < 0 || prm.n_post < 0 || prm.n_pre + prm.n_post < 1 || prm.mg_cycles < 1 || prm.max_iter < 0 ||
-        !(prm.tol >= 0.0) || prm.coarse_max_iter < -1 || !(prm.coarse_tol >= 0.0)) {
+        !(prm.tol >= 0.0) || prm.coarse_max_iter < -1 || !(prm.coarse_tol >= 0.0) || !(prm.mg_tol >= 0.0) ||
+        !(prm.mg_tol < 1.0) || (prm.mg_tol > 0.0 && prm.mg_max_iter < 1)) {
         set_error("invalid PCG/MG parameters (pre %d post %d cycles %d max_iter %d tol %g coarse %d/%g)", prm.n_pre,
                   prm.n_post, prm.mg_cycles, prm.max_iter, prm.tol, prm.coarse_max_iter, prm.coarse_tol);
         return TBFG_EINVAL;
@@ -419,6 +421,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     TBFG_HIP(hipHostMalloc((void**)&h->hst, sizeof(PcgState), hipHostMallocDefault));
     std::memset(h->hst, 0, sizeof(PcgState));
     TBFG_HIP(hipHostMalloc((void**)&h->hring, 2 * sizeof(PcgState), hipHostMallocDefault));
+    TBFG_HIP(hipHostMalloc((void**)&h->hmg, sizeof(int32_t), hipHostMallocDefault));
     for (auto& e : h->hev) TBFG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // footprint: every device buffer of the handle
     int64_t fp = 0;
@@ -591,7 +594,13 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
                   int dots_init = -1, bool* dots_done = nullptr)
 {
     if (dots_done) *dots_done = false;
-    const bool want_dots = dots_init >= 0 && h->fused[0] == 1 && h->prm.n_post > 0 && h->t0 > 0;
+    // solveMG's loop (mg_tol > 0): after every cycle the finest residual is
+    // tested on the device and the host reads the decision before it queues
+    // another cycle; the last cycle is not known in advance, so the dot
+    // products are not folded into a sweep
+    const bool tol_mode = h->prm.mg_tol > 0.0;
+    const int ncycles = tol_mode ? h->prm.mg_max_iter : h->prm.mg_cycles;
+    const bool want_dots = !tol_mode && dots_init >= 0 && h->fused[0] == 1 && h->prm.n_post > 0 && h->t0 > 0;
     SweepDots sd;
     if (want_dots) {
         sd.r = h->r.as<double>();
@@ -608,7 +617,7 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
         cur[l] = h->p[l].as<T>();
         alt[l] = h->palt[l].as<T>();
     }
-    for (int cyc = 0; cyc < h->prm.mg_cycles; ++cyc) {
+    for (int cyc = 0; cyc < ncycles; ++cyc) {
         for (int l = 0; l < t0; ++l) {
             const bool first = (l > 0) || (cyc == 0);
             const T* b = (l == 0) ? beta0 : h->beta[l].as<T>();
@@ -630,13 +639,24 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
             const T* q = (l == 0) ? r0 : h->q[l].as<T>();
             const T* pc = (l + 1 < t0) ? cur[l + 1] : h->p[l + 1].as<T>();
             TBFG_TRY(launch_prolong_add<T>(geo<T>(h, l), pc, cur[l], done, stream));
-            const bool fuse = want_dots && l == 0 && cyc == h->prm.mg_cycles - 1;
+            const bool fuse = want_dots && l == 0 && cyc == ncycles - 1;
             unsigned nb = 0;
             TBFG_TRY(smooth<T>(h, l, b, q, cur[l], alt[l], h->prm.n_post, false, done, stream, fuse ? &sd : nullptr,
                                fuse ? &nb : nullptr));
             if (fuse) {
                 TBFG_TRY(launch_fin_dots(nb, h->n0, dots_init, h->st.as<PcgState>(), sd.red, stream));
                 if (dots_done) *dots_done = true;
+            }
+        }
+        if (tol_mode) {
+            PcgState* st = h->st.as<PcgState>();
+            TBFG_TRY(launch_mg_res<T>(geo<T>(h, 0), beta0, r0, cur[0], h->prm.mg_tol, cyc, ncycles, st,
+                                      Reducer{h->partials.as<double>()}, stream));
+            if (cyc + 1 < ncycles) {
+                TBFG_HIP(hipMemcpyAsync(h->hmg, &st->mg_stop, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                        (hipStream_t)stream));
+                TBFG_HIP(hipStreamSynchronize((hipStream_t)stream));
+                if (*h->hmg) break;
             }
         }
     }
@@ -745,6 +765,15 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     TBFG_HIP(hipStreamSynchronize(s));
     if (h->prof_kernel >= 0) prof_harvest(h);
+    if (FFT) {
+        h->hst->mg_cyc_total = h->hst->mg_cyc_max = 0;
+    } else if (!(h->prm.mg_tol > 0.0)) {
+        // fixed cycles: the first application and one after every update but
+        // the last (whose stop test comes before it; the reference applies
+        // the preconditioner once more before testing, pcg.f90:122-136)
+        h->hst->mg_cyc_total = h->hst->iter * h->prm.mg_cycles;
+        h->hst->mg_cyc_max = h->hst->iter > 0 ? h->prm.mg_cycles : 0;
+    }
     if (iters) *iters = h->hst->iter;
     if (res) *res = h->hst->res;
     if (h->hst->status == TBFG_ENUMERIC) {
@@ -797,6 +826,10 @@ static int precond_t(tbfg_pcg* h, const double* beta, const double* r, double* z
     // z with its arithmetic mean removed, as the reference's sweeps leave it
     TBFG_TRY(launch_pcg_dots<T>(h->n0, r, zt, nullptr, 1, st, red, stream));
     TBFG_TRY(launch_pcg_dir<T>(h->n0, zt, z, 1, st, stream));
+    if (h->prm.mg_tol > 0.0) {  // the cycles taken, for tbfg_pcg_mg_stats (the loop synchronised already)
+        TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        TBFG_HIP(hipStreamSynchronize(s));
+    }
     return TBFG_OK;
 }
 

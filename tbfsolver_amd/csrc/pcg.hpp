@@ -38,6 +38,7 @@ struct tbfg_pcg {
     tbfg::DevBuf instate;               // its InnerState (multi-workgroup form)
     int32_t* hin = nullptr;             // pinned copy of InnerState::done, ::it (the host's stop test)
     int inner_guess = 8;                // iterations launched before the first stop test (last solve's count)
+    int32_t* hmg = nullptr;             // pinned copy of PcgState::mg_stop (solveMG's loop, mg_tol > 0)
     tbfg::DevBuf hpsi, hbeta, hq;       // device copies for the host-array entry point
     tbfg_fps* fps = nullptr;            // FFT preconditioner (TBFG_PRECOND_FFT)
     tbfg::DevBuf zfft;                  // its output z (FP64)

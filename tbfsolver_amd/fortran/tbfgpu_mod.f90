@@ -32,7 +32,8 @@ module tbfgpu_mod
         real(c_double) :: tol
         integer(c_int32_t) :: mg_cycles, precision, coarse_max_iter, flexible
         real(c_double) :: coarse_tol
-        integer(c_int32_t) :: preconditioner
+        integer(c_int32_t) :: preconditioner, mg_max_iter
+        real(c_double) :: mg_tol
     end type
 
     interface
@@ -156,8 +157,12 @@ contains
     ! flexible selects the direction update: .false. Fletcher-Reeves as
     ! pcg.f90:241-255 (the FP64 default), .true. Polak-Ribiere, which keeps
     ! converging where the V-cycle preconditioner (red-black order in both pre-
-    ! and post-smoothing) is not symmetric enough for Fletcher-Reeves.
-    subroutine tbfg_pcgCTOR(h, grid, nLevels, nPreSweep, nPostSweep, tolPCG, maxIterPCG, mixed, ierr, flexible)
+    ! and post-smoothing) is not symmetric enough for Fletcher-Reeves.  The
+    ! optional tolMG and maxIterMG select solveMG's loop (V-cycles per
+    ! application until the finest residual is at most tolMG); without them
+    ! one V-cycle per application.
+    subroutine tbfg_pcgCTOR(h, grid, nLevels, nPreSweep, nPostSweep, tolPCG, maxIterPCG, mixed, ierr, flexible, &
+                            tolMG, maxIterMG)
         type(c_ptr), intent(out) :: h
         type(tbfg_grid_desc), intent(in) :: grid
         integer, intent(in) :: nLevels, nPreSweep, nPostSweep, maxIterPCG
@@ -165,6 +170,8 @@ contains
         logical, intent(in) :: mixed
         integer, intent(out) :: ierr
         logical, intent(in), optional :: flexible
+        real(c_double), intent(in), optional :: tolMG
+        integer, intent(in), optional :: maxIterMG
         type(tbfg_pcg_params) :: prm
         call tbfg_pcg_default_params(prm)
         prm%n_levels = nLevels
@@ -177,6 +184,8 @@ contains
             prm%flexible = 0
             if (flexible) prm%flexible = 1
         end if
+        if (present(tolMG)) prm%mg_tol = tolMG
+        if (present(maxIterMG)) prm%mg_max_iter = maxIterMG
         ierr = tbfg_pcg_create(grid, prm, h)
     end subroutine
 

@@ -125,7 +125,7 @@ class PCG:
 
     def __init__(self, grid, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=100,
                  precision="fp64", mg_cycles=1, coarse_max_iter=-1, coarse_tol=0.0, flexible=None,
-                 preconditioner="mg"):
+                 preconditioner="mg", tolMG=0.0, maxIterMG=100):
         # pcgCTOR + multiGridCTOR
         self.grid = grid
         p = default_params()
@@ -138,6 +138,9 @@ class PCG:
         if preconditioner not in ("mg", "fft"):
             raise ValueError("preconditioner must be 'mg' or 'fft'")
         p.preconditioner = 1 if preconditioner == "fft" else 0
+        # solveMG's loop (multiGrid.f90:165-182): V-cycles per application until
+        # the finest residual is at most tolMG; 0 keeps mg_cycles fixed
+        p.mg_tol, p.mg_max_iter = float(tolMG), int(maxIterMG)
         self.params = p
         h = C.c_void_p()
         check(lib().tbfg_pcg_create(C.byref(grid), C.byref(p), C.byref(h)))
@@ -153,7 +156,8 @@ class PCG:
         prm = read_parameter_file(path)
         args = dict(nLevels=int(prm.get("nLevels", 4)), nPreSweep=int(prm.get("nPreSweep", 2)),
                     nPostSweep=int(prm.get("nPostSweep", 2)), tolPCG=float(prm.get("tolPCG", 1e-8)),
-                    maxIterPCG=int(prm.get("maxIterPCG", 100)))
+                    maxIterPCG=int(prm.get("maxIterPCG", 100)), tolMG=float(prm.get("tolMG", 0.0)),
+                    maxIterMG=int(prm.get("maxIterMG", 100)))
         args.update(kw)
         return cls(grid, **args)
 
@@ -198,6 +202,12 @@ class PCG:
         a, b, c, d = C.c_int32(0), C.c_int32(0), C.c_int32(0), C.c_double(0.0)
         check(lib().tbfg_pcg_coarse_stats(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         return {"calls": a.value, "iters": b.value, "iters_max": c.value, "res_max": d.value}
+
+    def mg_stats(self):
+        """V-cycles of the last solve's preconditioner applications: (total, most in one)."""
+        a, b = C.c_int32(0), C.c_int32(0)
+        check(lib().tbfg_pcg_mg_stats(self._h, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
 
     def profile_begin(self, kernel):
         check(lib().tbfg_pcg_profile_begin(self._h, self.KERNELS.get(kernel, kernel)))

@@ -344,6 +344,40 @@ def test_pcg_max_iter_and_warm_start():
     assert it2 < it3
 
 
+@pytest.mark.parametrize("stretch", [0, 1])
+def test_mg_tolerance_loop(stretch):
+    """solveMG's loop (mg_tol > 0): the cycle count is the first c at which the
+    fixed c-cycle application's residual ||r - A z|| / ||r|| is at most tolMG,
+    capped at maxIterMG, and the result is that application's, bit for bit;
+    before any cycle the residual is 1, so there is always at least one."""
+    g = grid((32, 16, 32), stretch_y=stretch)
+    shape = O.shape(g)
+    beta = np.exp(0.8 * rng.standard_normal(shape))
+    r = rng.standard_normal(shape)
+    r -= np.sum(vol(g) * r) / vol(g).sum()  # in the range of A (sum V r = 0)
+    fixed, res = {}, {}
+    for c in range(1, 7):
+        fixed[c] = O.mg_apply(g, O.params(n_levels=3, mg_cycles=c), beta, r)
+        res[c] = np.linalg.norm(r - O.apply_op(g, beta, fixed[c])) / np.linalg.norm(r)
+    assert all(res[c + 1] < res[c] for c in range(1, 6))
+    for tol in (0.9, res[2] * 1.0001, res[3] * 0.9999, res[5] * 1.0001):
+        want = next(c for c in range(1, 7) if res[c] <= tol)
+        z = O.mg_apply(g, O.params(n_levels=3, mg_tol=tol, mg_max_iter=100), beta, r)
+        assert O.last_mg_cycles() == (want, want), (tol, want)
+        assert np.array_equal(z, fixed[want])
+    for cap in (1, 2, 4):
+        z = O.mg_apply(g, O.params(n_levels=3, mg_tol=1e-300, mg_max_iter=cap), beta, r)
+        assert O.last_mg_cycles() == (cap, cap) and np.array_equal(z, fixed[cap])
+    # in a solve: every application counted, the converged answer unchanged
+    q = r
+    psi, it, rres, rc = O.pcg_solve(g, O.params(n_levels=3, tol=1e-10, max_iter=200, mg_tol=0.05,
+                                                flexible=1), beta, q)
+    total, most, last = O.last_mg_cycles(with_last=True)
+    assert rc == 0 and it + 1 <= total <= (it + 1) * most and most >= 2 and 1 <= last <= most
+    psi1, _, _, rc1 = O.pcg_solve(g, O.params(n_levels=3, tol=1e-10, max_iter=200, flexible=1), beta, q)
+    assert np.linalg.norm((psi - psi.mean()) - (psi1 - psi1.mean())) <= 1e-8 * np.linalg.norm(psi1 - psi1.mean())
+
+
 # ---------------------------------------------- projection and divergence (a33)
 
 def test_divergence_manufactured():
