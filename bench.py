@@ -262,6 +262,21 @@ def main():
         totr, msr, itr, _ = timed_solves(torch, ref_alg, torch.zeros_like(drho), beta, q, 2, 1)
         out["fp64_reference_algorithm"] = {"solves_per_s": round(2 / totr, 3), "iterations": int(np.median(itr)),
                                            "pcg": "Fletcher-Reeves, coarsest level to 1e-10"}
+        # the reference's own channel parameters (channel_18/specs/pcg_solver: 5 levels, 2/4 sweeps,
+        # tolMG 0.4 with maxIterMG 100, tolPCG 1e-10, 100 iterations) with its Fletcher-Reeves PCG
+        try:
+            chan = T.PCG(g, nLevels=5, nPreSweep=2, nPostSweep=4, tolPCG=1e-10, maxIterPCG=100, precision="fp64",
+                         flexible=0, tolMG=0.4, maxIterMG=100)
+            totc, msc, itc, _ = timed_solves(torch, chan, torch.zeros_like(drho), beta, q, 2, 1)
+            cyc = chan.mg_stats()
+            out["fp64_reference_parameters"] = {
+                "solves_per_s": round(2 / totc, 3), "iterations": int(np.median(itc)), "mg_cycles": cyc[0],
+                "mg_cycles_max": cyc[1],
+                "params": "nLevels 5, nPreSweep 2, nPostSweep 4, tolMG 0.4, maxIterMG 100, tolPCG 1e-10, FR"}
+            log("reference parameters: %.2f solves/s, %d iterations, %d V-cycles" % (2 / totc, int(np.median(itc)),
+                                                                                    cyc[0]))
+        except Exception as e:  # reported, not fatal: the headline does not depend on it
+            out["fp64_reference_parameters"] = {"error": str(e)[:200]}
         out["speedup_mixed_vs_fp64"] = round(value / v64, 3)
         out["error_vs_fp64_1e-12"] = {"mixed_rel_l2": err_l2, "mixed_rel_max": err_max, "fp64_rel_l2": err64}
         log("fp64: %.2f solves/s, iterations %s; mixed err %.2e (L2) %.2e (max); reference algorithm %.2f/s"
