@@ -179,6 +179,7 @@ def main():
     ap.add_argument("--nref", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-array (PCIe-inclusive) timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the pressure solve of one channel instance runs on a single "
@@ -235,6 +236,30 @@ def main():
                    "grid": list(dims), "levels": 4, "sweeps": [2, 2], "tol": 1e-8, "precision": "mixed"},
         "iterations": int(np.median(iters)),
     }
+
+    # the same solve through the host-array entry point, as the Fortran caller
+    # makes it: psi, beta, q in host memory with a one-cell halo (Fortran
+    # order), copied over PCIe and back in each call -- reported beside the
+    # device-resident rate, never as it
+    try:
+        if args.no_host:
+            raise RuntimeError("skipped (--no-host)")
+        pad = lambda t: np.asfortranarray(np.pad(t.detach().cpu().numpy().T, 1))
+        beta_h, q_h = pad(beta), pad(q)
+        psi_h = np.zeros_like(beta_h, order="F")
+        mixed.solvePCG(psi_h, beta_h, q_h)
+        th = []
+        for _ in range(5):
+            psi_h[...] = 0.0
+            t0 = time.perf_counter()
+            mixed.solvePCG(psi_h, beta_h, q_h)
+            th.append(time.perf_counter() - t0)
+        out["host_arrays"] = {"p50_ms": round(1e3 * float(np.median(th)), 3),
+                              "solves_per_s": round(1.0 / float(np.median(th)), 3),
+                              "what": "tbfg_pcg_solve: psi, beta, q host arrays with halo 1 copied in, psi out"}
+        log("host-array entry point: %.3f ms per solve (PCIe included)" % (1e3 * float(np.median(th))))
+    except Exception as e:  # reported, not fatal
+        out["host_arrays"] = {"error": str(e)[:200]}
 
     # FP64 GPU path: the same algorithm with FP64 multigrid
     if not args.no_fp64:
