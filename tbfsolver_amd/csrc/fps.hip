@@ -15,7 +15,9 @@
 //
 // Mixed precision: K1-K5 in FP32, the (0,0) column in FP64, followed by FP64
 // iterative-refinement sweeps p += S32(s - L p) with the exact FFT operator L.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
 #include <vector>
 
@@ -39,14 +41,28 @@ template <class T>
 __device__ __forceinline__ C2<T> cscale(C2<T> a, T s) { return {a.x * s, a.y * s}; }
 
 constexpr int kFftThreads = 256;
-constexpr int kMaxBfly = 8;        // butterflies per thread per pass
+// butterflies per thread per pass: every butterfly's inputs are held in
+// registers across the pass's barrier, so FP64 holds fewer (4 x 4 complex
+// values, 64 registers) to keep the kernels clear of spills
+template <class T>
+constexpr int max_bfly() { return sizeof(T) == 8 ? 4 : 8; }
 constexpr int kLdsElems = 4096;    // complex elements (lines + twiddles) per block: 64 KB in FP64
 
+// Radices of each Stockham pass, and per pass the multipliers that divide by
+// the pass's butterfly count N/R and its sub-transform length Ns with one
+// multiply-high: floor(x/d) = mulhi(x, ceil(2^32/d)), exact for every divisor
+// of these lengths (<= 1024) and x < 2^15 (checked exhaustively); d = 1 is the
+// identity.  A runtime integer division costs ~20 instructions, and the index
+// arithmetic of three of them per butterfly dominated the passes.
 struct FftPlan {
     int n;
     int nrad;
     int rad[16];
+    uint32_t mnb[16];  // ceil(2^32 / (n / rad[s]))
+    uint32_t mns[16];  // ceil(2^32 / Ns), Ns = rad[0] * ... * rad[s-1]
 };
+
+static uint32_t div_magic(int d) { return d <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d); }
 
 static int make_plan(int n, FftPlan& p)
 {
@@ -65,7 +81,18 @@ static int make_plan(int n, FftPlan& p)
         p.rad[p.nrad++] = 3;
         m /= 3;
     }
+    int Ns = 1;
+    for (int st = 0; st < p.nrad; ++st) {
+        p.mnb[st] = div_magic(n / p.rad[st]);
+        p.mns[st] = div_magic(Ns);
+        Ns *= p.rad[st];
+    }
     return m == 1 ? TBFG_OK : TBFG_EINVAL;
+}
+
+__device__ __forceinline__ int fast_div(int x, int d, uint32_t magic)
+{
+    return d == 1 ? x : (int)__umulhi((uint32_t)x, magic);
 }
 
 // R-point DFT in registers, SIGN = -1 forward, +1 backward
@@ -116,13 +143,15 @@ __device__ void fft_lines(C2<T>* buf, int N, int nlines, const C2<T>* tw, const 
         const int nb = N / R;
         const int total = nlines * nb;
         const int stride = N / (Ns * R);
+        const uint32_t mnb = plan.mnb[s], mns = plan.mns[s];
+        constexpr int kMaxBfly = max_bfly<T>();
         C2<T> v[kMaxBfly][4];
 #pragma unroll
         for (int u = 0; u < kMaxBfly; ++u) {
             const int b = threadIdx.x + u * kFftThreads;
             if (b < total) {
-                const int line = b / nb, j = b - line * nb;
-                const int k = j % Ns;
+                const int line = fast_div(b, nb, mnb), j = b - line * nb;
+                const int k = j - fast_div(j, Ns, mns) * Ns;
                 const C2<T>* src = buf + line * N + j;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -143,15 +172,16 @@ __device__ void fft_lines(C2<T>* buf, int N, int nlines, const C2<T>* tw, const 
         for (int u = 0; u < kMaxBfly; ++u) {
             const int b = threadIdx.x + u * kFftThreads;
             if (b < total) {
-                const int line = b / nb, j = b - line * nb;
-                const int k = j % Ns;
+                const int line = fast_div(b, nb, mnb), j = b - line * nb;
+                const int jq = fast_div(j, Ns, mns);
+                const int k = j - jq * Ns;
                 if (R == 4)
                     dft4<T, SIGN>(v[u]);
                 else if (R == 2)
                     dft2<T, SIGN>(v[u]);
                 else
                     dft3<T, SIGN>(v[u]);
-                C2<T>* dst = buf + line * N + (j / Ns) * Ns * R + k;
+                C2<T>* dst = buf + line * N + jq * Ns * R + k;
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
                     if (r < R) dst[r * Ns] = v[u][r];
@@ -168,12 +198,19 @@ __device__ __forceinline__ void load_twiddles(C2<T>* dst, const C2<T>* __restric
     for (int t = threadIdx.x; t < n; t += kFftThreads) dst[t] = src[t];
 }
 
+// Staging loops issue the loads of kStageBatch elements per thread before the
+// first LDS store, so the HBM latency is paid once per batch rather than once
+// per element (a rolled loop of load-then-store waits on every load).
+template <class T>
+constexpr int stage_batch() { return sizeof(T) == 8 ? 4 : 8; }
+
 // K1: s(:, j, k0..k0+KT-1) -> A[kx][j][k], kx = 0..nx/2.
 template <class T, class TIN>
 __global__ void __launch_bounds__(kFftThreads)
     k_xfwd(int nx, int ny, int nz, int KT, FftPlan plan, const C2<T>* __restrict__ tw, const TIN* __restrict__ s,
            C2<T>* __restrict__ A)
 {
+    constexpr int kStageBatch = stage_batch<T>();
     extern __shared__ unsigned char smem[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
     const int L = KT >> 1;
@@ -181,11 +218,23 @@ __global__ void __launch_bounds__(kFftThreads)
     const int j = blockIdx.y;
     const int k0 = blockIdx.x * KT;
     load_twiddles(twl, tw, nx);
-    for (int t = threadIdx.x; t < L * nx; t += kFftThreads) {
-        const int l = t / nx, i = t - l * nx;
-        const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
-        const size_t o1 = o0 + (size_t)nx * ny;
-        buf[t] = C2<T>{(T)s[o0], (T)s[o1]};
+    for (int t0 = threadIdx.x; t0 < L * nx; t0 += kStageBatch * kFftThreads) {
+        TIN v0[kStageBatch], v1[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < L * nx) {
+                const int l = t / nx, i = t - l * nx;
+                const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
+                v0[u] = s[o0];
+                v1[u] = s[o0 + (size_t)nx * ny];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < L * nx) buf[t] = C2<T>{(T)v0[u], (T)v1[u]};
+        }
     }
     __syncthreads();
     fft_lines<T, -1>(buf, nx, L, twl, plan);
@@ -210,6 +259,7 @@ __global__ void __launch_bounds__(kFftThreads)
     k_zfwd(int ny, int nz, int JT, FftPlan plan, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ A,
            C2<T>* __restrict__ B)
 {
+    constexpr int kStageBatch = stage_batch<T>();
     extern __shared__ unsigned char smem[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
     C2<T>* twl = buf + JT * nz;
@@ -217,7 +267,19 @@ __global__ void __launch_bounds__(kFftThreads)
     const int j0 = blockIdx.x * JT;
     load_twiddles(twl, tw, nz);
     const C2<T>* src = A + ((size_t)m * ny + j0) * nz;
-    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) buf[t] = src[t];
+    for (int t0 = threadIdx.x; t0 < JT * nz; t0 += kStageBatch * kFftThreads) {
+        C2<T> v[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < JT * nz) v[u] = src[t];
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < JT * nz) buf[t] = v[u];
+        }
+    }
     __syncthreads();
     fft_lines<T, -1>(buf, nz, JT, twl, plan);
     for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) {
@@ -232,15 +294,31 @@ __global__ void __launch_bounds__(kFftThreads)
     k_zinv(int ny, int nz, int JT, FftPlan plan, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ B,
            C2<T>* __restrict__ A)
 {
+    constexpr int kStageBatch = stage_batch<T>();
     extern __shared__ unsigned char smem[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
     C2<T>* twl = buf + JT * nz;
     const int m = blockIdx.y;
     const int j0 = blockIdx.x * JT;
     load_twiddles(twl, tw, nz);
-    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) {
-        const int kz = t / JT, l = t - kz * JT;
-        buf[l * nz + kz] = B[((size_t)m * nz + kz) * ny + j0 + l];
+    for (int t0 = threadIdx.x; t0 < JT * nz; t0 += kStageBatch * kFftThreads) {
+        C2<T> v[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < JT * nz) {
+                const int kz = t / JT, l = t - kz * JT;
+                v[u] = B[((size_t)m * nz + kz) * ny + j0 + l];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < JT * nz) {
+                const int kz = t / JT, l = t - kz * JT;
+                buf[l * nz + kz] = v[u];
+            }
+        }
     }
     __syncthreads();
     fft_lines<T, +1>(buf, nz, JT, twl, plan);
@@ -255,6 +333,7 @@ __global__ void __launch_bounds__(kFftThreads)
     k_xinv(int nx, int ny, int nz, int KT, FftPlan plan, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ A,
            double* __restrict__ p, int mode, const double* __restrict__ shift)
 {
+    constexpr int kStageBatch = stage_batch<T>();
     extern __shared__ unsigned char smem[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
     const int L = KT >> 1;
@@ -265,32 +344,69 @@ __global__ void __launch_bounds__(kFftThreads)
     load_twiddles(twl, tw, nx);
     // Z_l[m] = X1[m] + i X2[m] with Hermitian extension X[nx-m] = conj X[m];
     // DC and Nyquist keep their real parts only, as a c2r transform does.
-    for (int t = threadIdx.x; t < nh * L; t += kFftThreads) {
-        const int m = t / L, l = t - m * L;
-        const size_t o = ((size_t)m * ny + j) * nz + k0 + 2 * l;
-        C2<T> x1 = A[o], x2 = A[o + 1];
-        if (m == 0 || 2 * m == nx) {
-            buf[l * nx + m] = C2<T>{x1.x, x2.x};
-        } else {
-            buf[l * nx + m] = C2<T>{x1.x - x2.y, x1.y + x2.x};
-            buf[l * nx + nx - m] = C2<T>{x1.x + x2.y, x2.x - x1.y};
+    constexpr int kB = kStageBatch / 2;  // two complex loads per element
+    for (int t0 = threadIdx.x; t0 < nh * L; t0 += kB * kFftThreads) {
+        C2<T> v1[kB], v2[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < nh * L) {
+                const int m = t / L, l = t - m * L;
+                const size_t o = ((size_t)m * ny + j) * nz + k0 + 2 * l;
+                v1[u] = A[o];
+                v2[u] = A[o + 1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < nh * L) {
+                const int m = t / L, l = t - m * L;
+                const C2<T> x1 = v1[u], x2 = v2[u];
+                if (m == 0 || 2 * m == nx) {
+                    buf[l * nx + m] = C2<T>{x1.x, x2.x};
+                } else {
+                    buf[l * nx + m] = C2<T>{x1.x - x2.y, x1.y + x2.x};
+                    buf[l * nx + nx - m] = C2<T>{x1.x + x2.y, x2.x - x1.y};
+                }
+            }
         }
     }
     __syncthreads();
     fft_lines<T, +1>(buf, nx, L, twl, plan);
     const double scale = 1.0 / ((double)nx * (double)nz);
     const double sh = (mode == 1) ? shift[0] : 0.0;
-    for (int t = threadIdx.x; t < L * nx; t += kFftThreads) {
-        const int l = t / nx, i = t - l * nx;
-        const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
-        const size_t o1 = o0 + (size_t)nx * ny;
-        const double v0 = (double)buf[t].x * scale, v1 = (double)buf[t].y * scale;
-        if (mode == 0) {
-            p[o0] = v0;
-            p[o1] = v1;
-        } else {
-            p[o0] = p[o0] + v0 - sh;
-            p[o1] = p[o1] + v1 - sh;
+    if (mode == 0) {
+        for (int t = threadIdx.x; t < L * nx; t += kFftThreads) {
+            const int l = t / nx, i = t - l * nx;
+            const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
+            p[o0] = (double)buf[t].x * scale;
+            p[o0 + (size_t)nx * ny] = (double)buf[t].y * scale;
+        }
+        return;
+    }
+    // the correction: p's old values loaded a batch at a time
+    for (int t0 = threadIdx.x; t0 < L * nx; t0 += kStageBatch * kFftThreads) {
+        double a0[kStageBatch], a1[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < L * nx) {
+                const int l = t / nx, i = t - l * nx;
+                const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
+                a0[u] = p[o0];
+                a1[u] = p[o0 + (size_t)nx * ny];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; ++u) {
+            const int t = t0 + u * kFftThreads;
+            if (t < L * nx) {
+                const int l = t / nx, i = t - l * nx;
+                const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
+                p[o0] = a0[u] + (double)buf[t].x * scale - sh;
+                p[o0 + (size_t)nx * ny] = a1[u] + (double)buf[t].y * scale - sh;
+            }
         }
     }
 }
@@ -591,13 +707,17 @@ namespace tbfg {
 static unsigned row_threads(int nx) { return nx >= 256 ? 256u : 64u; }
 static unsigned row_blocks(int nx) { return ceil_div(nx, row_threads(nx)); }
 
-static int choose_lines(int n, int count, int pairs)
+static int choose_lines(int n, int count, int pairs, int bfly)
 {
-    // largest L with (L+1)*n <= kLdsElems (lines plus twiddles), (pairs ? 2L : L)
-    // dividing count, L <= 64
+    // largest L with (L+1)*n <= kLdsElems (lines plus twiddles) and L*n/2 <=
+    // bfly * threads (the butterflies of a radix-2 pass), (pairs ? 2L : L)
+    // dividing count, L <= 64 (preferring whole 128-byte lines in the z passes'
+    // transposed runs, L = 8 instead of 12 at ny = 192, measured slower)
     int best = 0;
-    for (int L = 1; L <= 64; ++L) {
-        if ((L + 1) * n > kLdsElems) break;
+    int lmax = 64;
+    if (const char* e = std::getenv("TBFG_FFT_LINES")) lmax = std::max(1, std::atoi(e));  // A/B switch
+    for (int L = 1; L <= lmax; ++L) {
+        if ((L + 1) * n > kLdsElems || L * n / 2 > bfly * kFftThreads) break;
         const int span = pairs ? 2 * L : L;
         if (count % span == 0) best = L;
     }
@@ -637,14 +757,15 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
     h->ny = ny;
     h->nz = nz;
     h->nh = nx / 2 + 1;
-    h->KT = 2 * choose_lines(nx, nz, 1);
-    h->JT = choose_lines(nz, ny, 0);
+    const int bfly = h->precision == TBFG_PREC_MIXED ? max_bfly<float>() : max_bfly<double>();
+    h->KT = 2 * choose_lines(nx, nz, 1, bfly);
+    h->JT = choose_lines(nz, ny, 0, bfly);
     if (h->KT < 2 || h->JT < 1) {
         set_error("no line tiling for %d x %d x %d", nx, ny, nz);
         return TBFG_EINVAL;
     }
-    // butterflies per thread: L*N/2 <= kMaxBfly * threads
-    if ((h->KT / 2) * nx / 2 > kMaxBfly * kFftThreads || h->JT * nz / 2 > kMaxBfly * kFftThreads) {
+    // butterflies per thread: L*N/2 <= max_bfly * threads
+    if ((h->KT / 2) * nx / 2 > bfly * kFftThreads || h->JT * nz / 2 > bfly * kFftThreads) {
         set_error("line tiling exceeds the per-thread butterfly budget");
         return TBFG_EINVAL;
     }
