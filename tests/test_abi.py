@@ -105,3 +105,33 @@ def test_parameter_file_reader(tmp_path):
     prm = T.read_parameter_file(str(f))
     assert prm == {"nLevels": 5, "nPreSweep": 2, "nPostSweep": 4, "tolMG": 0.4, "maxIterMG": 100,
                    "fullInfoMG": False, "tolPCG": 1e-10, "maxIterPCG": 100}
+
+
+def test_fft_pass_division_by_multiply_high():
+    """The FFT passes divide butterfly indices (< 2^15) by the pass's
+    sub-lengths with mulhi(x, ceil(2^32 / d)) instead of an integer division:
+    exact for every divisor that a length the solver accepts (2^a 3^b, even,
+    <= 1024) produces -- n / radix and the partial radix products."""
+    def radices(n):
+        out, m = [], n
+        for r in (4, 2, 3):
+            while m % r == 0:
+                out.append(r)
+                m //= r
+        return out if m == 1 else None
+
+    divisors = set()
+    for n in range(2, 1025, 2):
+        rad = radices(n)
+        if rad is None:
+            continue
+        ns = 1
+        for r in rad:
+            divisors.update((n // r, ns))
+            ns *= r
+    x = np.arange(1 << 15, dtype=np.uint64)
+    for d in sorted(divisors):
+        if d == 1:
+            continue  # the passes use the identity for d = 1
+        magic = np.uint64(((1 << 32) + d - 1) // d)
+        assert np.array_equal((x * magic) >> np.uint64(32), x // np.uint64(d)), d
