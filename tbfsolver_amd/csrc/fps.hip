@@ -132,11 +132,15 @@ __device__ __forceinline__ void dft3(C2<T>* v)
 }
 
 // Stockham autosort FFT of `nlines` lines of length N held in LDS (line l at
-// buf + l*N); tw[t] = exp(-2 pi i t / N).  Each pass reads all its butterfly
-// inputs into registers, synchronises, then writes.
+// buf + l*(N+1): one element of padding per line, so that reading or writing
+// the same index of consecutive lines -- the transposed side of the passes'
+// loads and stores -- falls in different LDS banks); tw[t] = exp(-2 pi i t / N).
+// Each pass reads all its butterfly inputs into registers, synchronises, then
+// writes.
 template <class T, int SIGN>
 __device__ void fft_lines(C2<T>* buf, int N, int nlines, const C2<T>* tw, const FftPlan& plan)
 {
+    const int LS = N + 1;
     int Ns = 1;
     for (int s = 0; s < plan.nrad; ++s) {
         const int R = plan.rad[s];
@@ -152,7 +156,7 @@ __device__ void fft_lines(C2<T>* buf, int N, int nlines, const C2<T>* tw, const 
             if (b < total) {
                 const int line = fast_div(b, nb, mnb), j = b - line * nb;
                 const int k = j - fast_div(j, Ns, mns) * Ns;
-                const C2<T>* src = buf + line * N + j;
+                const C2<T>* src = buf + line * LS + j;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     if (r < R) {
@@ -181,7 +185,7 @@ __device__ void fft_lines(C2<T>* buf, int N, int nlines, const C2<T>* tw, const 
                     dft2<T, SIGN>(v[u]);
                 else
                     dft3<T, SIGN>(v[u]);
-                C2<T>* dst = buf + line * N + jq * Ns * R + k;
+                C2<T>* dst = buf + line * LS + jq * Ns * R + k;
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
                     if (r < R) dst[r * Ns] = v[u][r];
@@ -214,7 +218,8 @@ __global__ void __launch_bounds__(kFftThreads)
     extern __shared__ unsigned char smem[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
     const int L = KT >> 1;
-    C2<T>* twl = buf + L * nx;
+    const int LS = nx + 1;  // padded line stride (fft_lines)
+    C2<T>* twl = buf + L * LS;
     const int j = blockIdx.y;
     const int k0 = blockIdx.x * KT;
     load_twiddles(twl, tw, nx);
@@ -233,7 +238,7 @@ __global__ void __launch_bounds__(kFftThreads)
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
             const int t = t0 + u * kFftThreads;
-            if (t < L * nx) buf[t] = C2<T>{(T)v0[u], (T)v1[u]};
+            if (t < L * nx) buf[t + t / nx] = C2<T>{(T)v0[u], (T)v1[u]};
         }
     }
     __syncthreads();
@@ -242,8 +247,8 @@ __global__ void __launch_bounds__(kFftThreads)
     for (int t = threadIdx.x; t < nh * KT; t += kFftThreads) {
         const int m = t / KT, kk = t - m * KT;
         const int l = kk >> 1;
-        const C2<T> Z = buf[l * nx + m];
-        const C2<T> Zn = buf[l * nx + (m == 0 ? 0 : nx - m)];
+        const C2<T> Z = buf[l * LS + m];
+        const C2<T> Zn = buf[l * LS + (m == 0 ? 0 : nx - m)];
         C2<T> val;
         if ((kk & 1) == 0)
             val = C2<T>{(T)0.5 * (Z.x + Zn.x), (T)0.5 * (Z.y - Zn.y)};  // (Z + conj Zn)/2
@@ -262,7 +267,8 @@ __global__ void __launch_bounds__(kFftThreads)
     constexpr int kStageBatch = stage_batch<T>();
     extern __shared__ unsigned char smem[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
-    C2<T>* twl = buf + JT * nz;
+    const int LS = nz + 1;  // padded line stride (fft_lines)
+    C2<T>* twl = buf + JT * LS;
     const int m = blockIdx.y;
     const int j0 = blockIdx.x * JT;
     load_twiddles(twl, tw, nz);
@@ -277,14 +283,14 @@ __global__ void __launch_bounds__(kFftThreads)
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
             const int t = t0 + u * kFftThreads;
-            if (t < JT * nz) buf[t] = v[u];
+            if (t < JT * nz) buf[t + t / nz] = v[u];
         }
     }
     __syncthreads();
     fft_lines<T, -1>(buf, nz, JT, twl, plan);
     for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) {
         const int kz = t / JT, l = t - kz * JT;
-        B[((size_t)m * nz + kz) * ny + j0 + l] = buf[l * nz + kz];
+        B[((size_t)m * nz + kz) * ny + j0 + l] = buf[l * LS + kz];
     }
 }
 
@@ -297,7 +303,8 @@ __global__ void __launch_bounds__(kFftThreads)
     constexpr int kStageBatch = stage_batch<T>();
     extern __shared__ unsigned char smem[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
-    C2<T>* twl = buf + JT * nz;
+    const int LS = nz + 1;  // padded line stride (fft_lines)
+    C2<T>* twl = buf + JT * LS;
     const int m = blockIdx.y;
     const int j0 = blockIdx.x * JT;
     load_twiddles(twl, tw, nz);
@@ -316,14 +323,14 @@ __global__ void __launch_bounds__(kFftThreads)
             const int t = t0 + u * kFftThreads;
             if (t < JT * nz) {
                 const int kz = t / JT, l = t - kz * JT;
-                buf[l * nz + kz] = v[u];
+                buf[l * LS + kz] = v[u];
             }
         }
     }
     __syncthreads();
     fft_lines<T, +1>(buf, nz, JT, twl, plan);
     C2<T>* dst = A + ((size_t)m * ny + j0) * nz;
-    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) dst[t] = buf[t];
+    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) dst[t] = buf[t + t / nz];
 }
 
 // K5: A[kx][j][k0..k0+KT-1] -> p(:, j, k) scaled by 1/(nx nz).
@@ -337,7 +344,8 @@ __global__ void __launch_bounds__(kFftThreads)
     extern __shared__ unsigned char smem[];
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
     const int L = KT >> 1;
-    C2<T>* twl = buf + L * nx;
+    const int LS = nx + 1;  // padded line stride (fft_lines)
+    C2<T>* twl = buf + L * LS;
     const int j = blockIdx.y;
     const int k0 = blockIdx.x * KT;
     const int nh = nx / 2 + 1;
@@ -364,10 +372,10 @@ __global__ void __launch_bounds__(kFftThreads)
                 const int m = t / L, l = t - m * L;
                 const C2<T> x1 = v1[u], x2 = v2[u];
                 if (m == 0 || 2 * m == nx) {
-                    buf[l * nx + m] = C2<T>{x1.x, x2.x};
+                    buf[l * LS + m] = C2<T>{x1.x, x2.x};
                 } else {
-                    buf[l * nx + m] = C2<T>{x1.x - x2.y, x1.y + x2.x};
-                    buf[l * nx + nx - m] = C2<T>{x1.x + x2.y, x2.x - x1.y};
+                    buf[l * LS + m] = C2<T>{x1.x - x2.y, x1.y + x2.x};
+                    buf[l * LS + nx - m] = C2<T>{x1.x + x2.y, x2.x - x1.y};
                 }
             }
         }
@@ -380,8 +388,8 @@ __global__ void __launch_bounds__(kFftThreads)
         for (int t = threadIdx.x; t < L * nx; t += kFftThreads) {
             const int l = t / nx, i = t - l * nx;
             const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
-            p[o0] = (double)buf[t].x * scale;
-            p[o0 + (size_t)nx * ny] = (double)buf[t].y * scale;
+            p[o0] = (double)buf[t + l].x * scale;
+            p[o0 + (size_t)nx * ny] = (double)buf[t + l].y * scale;
         }
         return;
     }
@@ -404,8 +412,8 @@ __global__ void __launch_bounds__(kFftThreads)
             if (t < L * nx) {
                 const int l = t / nx, i = t - l * nx;
                 const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
-                p[o0] = a0[u] + (double)buf[t].x * scale - sh;
-                p[o0 + (size_t)nx * ny] = a1[u] + (double)buf[t].y * scale - sh;
+                p[o0] = a0[u] + (double)buf[t + l].x * scale - sh;
+                p[o0 + (size_t)nx * ny] = a1[u] + (double)buf[t + l].y * scale - sh;
             }
         }
     }
@@ -709,7 +717,7 @@ static unsigned row_blocks(int nx) { return ceil_div(nx, row_threads(nx)); }
 
 static int choose_lines(int n, int count, int pairs, int bfly)
 {
-    // largest L with (L+1)*n <= kLdsElems (lines plus twiddles) and L*n/2 <=
+    // largest L with L*(n+1)+n <= kLdsElems (padded lines plus twiddles) and L*n/2 <=
     // bfly * threads (the butterflies of a radix-2 pass), (pairs ? 2L : L)
     // dividing count, L <= 64 (preferring whole 128-byte lines in the z passes'
     // transposed runs, L = 8 instead of 12 at ny = 192, measured slower)
@@ -717,7 +725,7 @@ static int choose_lines(int n, int count, int pairs, int bfly)
     int lmax = 64;
     if (const char* e = std::getenv("TBFG_FFT_LINES")) lmax = std::max(1, std::atoi(e));  // A/B switch
     for (int L = 1; L <= lmax; ++L) {
-        if ((L + 1) * n > kLdsElems || L * n / 2 > bfly * kFftThreads) break;
+        if (L * (n + 1) + n > kLdsElems || L * n / 2 > bfly * kFftThreads) break;
         const int span = pairs ? 2 * L : L;
         if (count % span == 0) best = L;
     }
@@ -867,8 +875,8 @@ static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipSt
     const bool f32 = sizeof(T) == 4;
     const C2<T>* twx = f32 ? h->twxf.as<C2<T>>() : h->twx.as<C2<T>>();
     const C2<T>* twz = f32 ? h->twzf.as<C2<T>>() : h->twz.as<C2<T>>();
-    const size_t ldx = ((size_t)(h->KT / 2) * nx + nx) * sizeof(C2<T>);
-    const size_t ldz = ((size_t)h->JT * nz + nz) * sizeof(C2<T>);
+    const size_t ldx = ((size_t)(h->KT / 2) * (nx + 1) + nx) * sizeof(C2<T>);  // padded lines + twiddles
+    const size_t ldz = ((size_t)h->JT * (nz + 1) + nz) * sizeof(C2<T>);
     hipLaunchKernelGGL((k_xfwd<T, TIN>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px,
                        twx, in, A);
     hipLaunchKernelGGL((k_zfwd<T>), dim3(ny / h->JT, nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz, A, B);
