@@ -63,8 +63,11 @@ int make_farr(const int64_t* lb, const int64_t* ub, const int32_t* n, const char
     return TBFG_OK;
 }
 
+// (host copies of whole fields: one plane per OpenMP iteration, so the
+// host-array entry points are not bound by one core's copy rate)
 void gather(const double* src, const FArr& f, const int32_t* n, double* dst)
 {
+#pragma omp parallel for schedule(static)
     for (int64_t k = 1; k <= n[2]; ++k)
         for (int64_t j = 1; j <= n[1]; ++j) {
             const double* row = src + f.at(1, j, k);
@@ -78,15 +81,18 @@ void gather(const double* src, const FArr& f, const int32_t* n, double* dst)
 // extent of the other axes (field.f90:950-974, boundaryField_S.f90:713-725).
 void scatter_fill(const double* src, const FArr& f, const int32_t* n, double* dst)
 {
+#pragma omp parallel for schedule(static)
     for (int64_t k = 1; k <= n[2]; ++k)
         for (int64_t j = 1; j <= n[1]; ++j)
             std::memcpy(dst + f.at(1, j, k), src + (size_t)n[0] * ((size_t)(j - 1) + (size_t)n[1] * (size_t)(k - 1)),
                         sizeof(double) * (size_t)n[0]);
+#pragma omp parallel for schedule(static)
     for (int64_t k = 1; k <= n[2]; ++k)
         for (int64_t i = 1; i <= n[0]; ++i) {
             if (f.lb[1] <= 0) dst[f.at(i, 0, k)] = dst[f.at(i, 1, k)];
             if (f.ub[1] >= n[1] + 1) dst[f.at(i, n[1] + 1, k)] = dst[f.at(i, n[1], k)];
         }
+#pragma omp parallel for schedule(static)
     for (int64_t k = 1; k <= n[2]; ++k)
         for (int64_t j = f.lb[1]; j <= f.ub[1]; ++j) {
             for (int64_t i = f.lb[0]; i <= 0; ++i) dst[f.at(i, j, k)] = dst[f.at(i + n[0], j, k)];
