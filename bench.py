@@ -76,6 +76,56 @@ def copy_peak(torch, nbytes=1 << 30, reps=5):
     return 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
 
 
+def small_configs(torch):
+    """The two smaller workloads of the benchmark family, timed on the device:
+    a projection step of the 64^3 constant-density channel with the FFT solver
+    (source, FP64 solve, projection; max |div u| after it) and the FP64
+    PCG+MG solve of the 128^3 channel with 18 bubbles to 1e-8."""
+    import tbfsolver_amd as T
+    from tbfsolver_amd import fields as F
+
+    out = {}
+    # 64^3 uniform channel, seeded N(0,1) velocity, dt = 1e-3, FAST_MODE step
+    g0 = T.channel_grid((64, 64, 64))
+    u = tuple(to_dev(a, torch) for a in F.make_velocity(g0, 1000))
+    fs = T.FastPoissonSolver(g0, precision="fp64")
+    s0 = torch.zeros_like(u[0])
+    p0 = torch.zeros_like(u[0])
+
+    def step():
+        F.computeSource(g0, *u, s0)
+        s0.mul_(1e3)  # s = div u / dt
+        fs.solveFPS(p0, s0)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(50):
+        step()
+    torch.cuda.synchronize()
+    ms0 = (time.perf_counter() - t0) / 50 * 1e3
+    fs.close()
+    out["config0_fast_64"] = {"ms_per_step": round(ms0, 4), "what": "source + FP64 FFT solve, 64^3"}
+    # 128^3, 18 bubbles, FP64 PCG + 4-level MG to 1e-8
+    g1 = T.channel_grid((128, 128, 128))
+    rho, _ = F.make_bubble_density(g1, 18, 0.3, 1001, nref=10)
+    ux, uy, uz = (to_dev(a, torch) for a in F.make_velocity(g1, 1001))
+    drho = to_dev(rho, torch)
+    q = torch.zeros_like(drho)
+    F.computeSource(g1, ux, uy, uz, q)
+    beta = torch.zeros_like(drho)
+    F.computeBeta(g1, 1e-3, drho, beta)
+    for prec in ("fp64", "mixed"):
+        s = T.PCG(g1, nLevels=4, tolPCG=1e-8, maxIterPCG=200, precision=prec, flexible=1, coarse_max_iter=0)
+        tot, ms, its, _ = timed_solves(torch, s, torch.zeros_like(drho), beta, q, 10, 2)
+        out["config1_pcg_128_" + prec] = {"solves_per_s": round(10 / tot, 2), "p50_ms": round(float(np.median(ms)), 4),
+                                          "iterations": int(np.median(its))}
+        s.close()
+    log("small configs: 64^3 FAST step %.3f ms; 128^3 PCG+MG fp64 %.2f ms, mixed %.2f ms"
+        % (ms0, out["config1_pcg_128_fp64"]["p50_ms"], out["config1_pcg_128_mixed"]["p50_ms"]))
+    return out
+
+
 def batch_leg(torch, reps=3, workers=8):
     """64 uneven instances (seed 1003; FFT with constant density on even indices,
     PCG + multigrid with 4..24 bubbles on odd ones), mixed precision, each solved
@@ -350,6 +400,10 @@ def main():
         out["fast_solver"] = fast
         log("FFT solve (FAST_MODE) on the same grid: fp64 %.3f ms, mixed %.3f ms (rel diff %.1e)"
             % (fast["fp64_ms"], fast["mixed_ms"], fast["mixed_rel_l2_vs_fp64"]))
+        try:
+            out["small_configs"] = small_configs(torch)
+        except Exception as e:  # reported, not fatal
+            out["small_configs"] = {"error": str(e)[:200]}
 
     # roofline of the dominant kernel (finest-level red/black sweep)
     launches, kms, bytes_per = prof
