@@ -65,6 +65,19 @@ program tbfgpu_test
     print '(a, i4, a, es10.3)', 'solvePCG (Fletcher-Reeves) not converged: iter_ ', iter_, ' res_ ', res_
     ierr = tbfg_pcg_destroy(pcg)
 
+    ! solveMG's loop with the reference channel case's tolMG and maxIterMG
+    psi = 0.d0
+    call tbfg_pcgCTOR(pcg, g, 4, 2, 2, 1.d-12, 100, .false., ierr, flexible = .true., tolMG = 0.4d0, &
+                      maxIterMG = 100)
+    if (ierr /= TBFG_OK) call fail('pcgCTOR (tolMG): ' // tbfg_last_error())
+    call tbfg_solvePCG(pcg, psi, beta, s, iter_, res_, ierr)
+    if (ierr /= TBFG_OK) call fail('solvePCG (tolMG): ' // tbfg_last_error())
+    psi = psi - sum(psi(1:n, 1:n, 1:n)) / real(n**3, c_double)
+    err = maxval(abs(psi(1:n, 1:n, 1:n) - s(1:n, 1:n, 1:n) / lam))
+    if (err > 1d-9) call fail('solvePCG (tolMG) error too large')
+    print '(a, i4, a, es10.3)', 'solvePCG (tolMG 0.4) iter_ ', iter_, ' max error ', err
+    ierr = tbfg_pcg_destroy(pcg)
+
     ! error path: non-divisible hierarchy is rejected with a status code
     call tbfg_pcgCTOR(pcg, tbfg_channel_grid(36, 32, 32, Lx, 2.d0, pi, .false., 1.4d0), 4, 2, 2, &
                       1.d-8, 100, .false., ierr)
