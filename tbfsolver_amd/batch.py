@@ -10,7 +10,7 @@ interpreter lock during every call, and distinct handles are independent.
 
   config3_instances(count, seed)  the instance list (sizes, kind, bubbles)
   Instance                        one instance: grid, inputs, handle, solve()
-  run_batch(instances, workers)   all solves, wall time
+  run_batch(instances, workers)   all solves from a shared queue, wall time
 """
 import concurrent.futures as cf
 import time
@@ -92,19 +92,38 @@ class Instance:
         self.solver.close()
 
 
+def _cost(inst):
+    """Relative work of a solve: PCG+MG iterations stream the grid several times
+    more than the five passes of an FFT solve."""
+    cells = inst.n[0] * inst.n[1] * inst.n[2]
+    return cells * (1 if inst.kind == "FAST" else 8)
+
+
 def run_batch(instances, workers=8):
-    """Solves every instance once, instance i on worker i % workers, each worker
-    on its own stream; returns the wall time in seconds."""
+    """Solves every instance once on `workers` host threads, each with its own
+    stream; returns the wall time in seconds.  The threads take instances from
+    one shared queue, largest expected work first, so no stream idles while
+    another still holds several solves."""
+    import threading
+
     import torch
 
     streams = [torch.cuda.Stream() for _ in range(max(1, workers))]
+    order = sorted(range(len(instances)), key=lambda i: -_cost(instances[i]))
+    lock = threading.Lock()
+    nxt = [0]
     torch.cuda.synchronize()
 
     def work(w):
         s = streams[w]
         with torch.cuda.stream(s):
-            for inst in instances[w::len(streams)]:
-                inst.solve(stream=s)
+            while True:
+                with lock:
+                    if nxt[0] >= len(order):
+                        break
+                    i = order[nxt[0]]
+                    nxt[0] += 1
+                instances[i].solve(stream=s)
         s.synchronize()
 
     t0 = time.perf_counter()
