@@ -843,6 +843,11 @@ __global__ void k_convert(size_t n, const TS* __restrict__ src, TD* __restrict__
 // them back to back with workgroup barriers; their arrays stay in L2, or in
 // LDS (LDS = true: the level's pointers point into the workgroup's LDS and are
 // used as LDS addresses).
+// floor(x / d) by one multiply-high with m = ceil(2^32 / d): exact when x * d
+// < 2^32 (the error term x (m d - 2^32) / (d 2^32) stays below 1/d); d = 1 is
+// the identity
+__device__ __forceinline__ int mdiv(int x, int d, uint32_t m) { return d == 1 ? x : (int)__umulhi((uint32_t)x, m); }
+
 template <class T, bool LDS>
 struct TailView;
 template <class T>
@@ -851,7 +856,12 @@ struct TailView<T, false> {
     T* p;
     T* q;
     const T* beta;
-    __device__ explicit TailView(const TailLevel<T>& lv) : geo(lv.geo), p(lv.p), q(lv.q), beta(lv.beta) {}
+    uint32_t dm_nx, dm_ny, dm_hx, dm_hy;
+    __device__ explicit TailView(const TailLevel<T>& lv)
+        : geo(lv.geo), p(lv.p), q(lv.q), beta(lv.beta), dm_nx(lv.dm_nx), dm_ny(lv.dm_ny), dm_hx(lv.dm_hx),
+          dm_hy(lv.dm_hy)
+    {
+    }
 };
 template <class T>
 struct TailView<T, true> {
@@ -859,8 +869,13 @@ struct TailView<T, true> {
     lds_ptr<T> p;
     lds_ptr<T> q;
     lds_ptr<const T> beta;
+    uint32_t dm_nx, dm_ny, dm_hx, dm_hy;
     __device__ explicit TailView(const TailLevel<T>& lv)
     {
+        dm_nx = lv.dm_nx;
+        dm_ny = lv.dm_ny;
+        dm_hx = lv.dm_hx;
+        dm_hy = lv.dm_hy;
         geo.nx = lv.geo.nx;
         geo.ny = lv.geo.ny;
         geo.nz = lv.geo.nz;
@@ -897,8 +912,8 @@ __device__ void tail_smooth(const TailLevel<T>& lvl, int sweeps, bool first, boo
         for (int half_sweep = 0; half_sweep < 2; ++half_sweep) {
             const bool red = (half_sweep == 0) != reverse;
             for (int t = threadIdx.x; t < work; t += blockDim.x) {
-                const int h = t % half, row = t / half;
-                const int j = row % L.ny, k = row / L.ny;
+                const int row = mdiv(t, half, lv.dm_hx), h = t - row * half;
+                const int k = mdiv(row, L.ny, lv.dm_ny), j = row - k * L.ny;
                 if (red) {
                     if (f0)
                         rbgs_cell<T, T, RED, true>(L, lv.beta, lv.q, lv.p, h, j, k);
@@ -928,8 +943,8 @@ __device__ void tail_resid_restrict(const TailLevel<T>& lvl, const TailLevel<T>&
     // two plane sums, the same on both lanes
     for (int t2 = threadIdx.x; t2 < 2 * cells; t2 += blockDim.x) {
         const int t = t2 >> 1, dk = t2 & 1;
-        const int I = t % ncx, r = t / ncx;
-        const int J = r % ncy, K = r / ncy;
+        const int r = mdiv(t, ncx, lv.dm_hx), I = t - r * ncx;
+        const int K = mdiv(r, ncy, lv.dm_hy), J = r - K * ncy;
         const T part = resid_restrict_plane<T, T>(L, lv.beta, lv.q, lv.p, I, J, K, dk, (T)0);
         const T other = __shfl_xor(part, 1, 64);
         if (dk == 0) lc.q[t] = part + other;
@@ -944,8 +959,8 @@ __device__ void tail_prolong(const TailLevel<T>& lvl, const TailLevel<T>& lcl)
     const auto& L = lv.geo;
     const int cells = L.nx * L.ny * L.nz;
     for (int t = threadIdx.x; t < cells; t += blockDim.x) {
-        const int i = t % L.nx, r = t / L.nx;
-        const int j = r % L.ny, k = r / L.ny;
+        const int r = mdiv(t, L.nx, lv.dm_nx), i = t - r * L.nx;
+        const int k = mdiv(r, L.ny, lv.dm_ny), j = r - k * L.ny;
         prolong_cell<T>(L, lc.geo.nx, lc.geo.ny, lc.geo.nz, lc.p, lv.p, i, j, k);
     }
     __syncthreads();

@@ -73,6 +73,10 @@ struct TailLevel {
     // the level's place in the LDS-resident tail (elements from the dynamic
     // LDS base): p, q, beta, the table block
     int lds_p, lds_q, lds_beta, lds_tab;
+    // ceil(2^32 / d) for d = nx, ny, nx/2, ny/2: the tail's index arithmetic
+    // divides cell indices by these with one multiply-high, exact while
+    // cells * max(nx, ny) < 2^32 (checked when the plan is built)
+    uint32_t dm_nx, dm_ny, dm_hx, dm_hy;
 };
 
 constexpr int kTailSegs = 2 * kMaxTail + 1;

@@ -195,6 +195,18 @@ static int build_plan(tbfg_pcg* h)
         P.lv[m].lds_q = (int)(e0 + n);
         P.lv[m].lds_beta = (int)(e0 + 2 * n);
         P.lv[m].lds_tab = (int)(e0 + 3 * n);
+        auto magic = [](int d) {
+            return d <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d);
+        };
+        if ((uint64_t)n * (uint64_t)std::max(h->hl[l].n[0], h->hl[l].n[1]) >= ((uint64_t)1 << 32)) {
+            set_error("tail level %d x %d x %d too large for its index arithmetic", h->hl[l].n[0], h->hl[l].n[1],
+                      h->hl[l].n[2]);
+            return TBFG_EINVAL;
+        }
+        P.lv[m].dm_nx = magic(h->hl[l].n[0]);
+        P.lv[m].dm_ny = magic(h->hl[l].n[1]);
+        P.lv[m].dm_hx = magic(h->hl[l].n[0] / 2);
+        P.lv[m].dm_hy = magic(h->hl[l].n[1] / 2);
         lds += 3 * n * sizeof(T);
         size_t nt = 0;
         for (int a = 0; a < 3; ++a)
