@@ -32,8 +32,10 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
+# (the last two: the longest transform the solver takes, one FFT line pair per
+# workgroup, and the tallest y column, 16 rows per lane of the tridiagonal solve)
 CASES = [((64, 64, 64), 0), ((32, 24, 16), 1), ((96, 32, 48), 1), ((256, 192, 256), 1), ((192, 128, 96), 0),
-         ((8, 6, 4), 1)]
+         ((8, 6, 4), 1), ((1024, 8, 768), 0), ((16, 1024, 8), 1)]
 
 
 @pytest.mark.parametrize("n,stretch", CASES)
@@ -48,14 +50,17 @@ def test_fft_solve_fp64(torch_gpu, n, stretch):
     fs.solveFPS(p, to_dev(s, torch_gpu))
     torch_gpu.cuda.synchronize()
     out = to_host(p)
+    # round-off grows with the conditioning of the y solve, ~ ny^2 (1024 stretched
+    # rows: 1.7e-12 forward, 3.5e-12 backward against 1e-12 up to 256 rows)
+    scale = max(1.0, (n[1] / 256) ** 2)
     if np.prod(n) <= 300000:
         ref = O.fft_solve(go, s)
-        assert rel(out, ref) < 1e-12
+        assert rel(out, ref) < 1e-12 * scale
     # residual of the operator the solver inverts, and the volume-weighted mean
     q = torch_gpu.zeros_like(p)
     fs.apply_operator(p, q)
     torch_gpu.cuda.synchronize()
-    assert np.linalg.norm(to_host(q) - s) < 1e-12 * np.linalg.norm(s)
+    assert np.linalg.norm(to_host(q) - s) < 1e-12 * scale * np.linalg.norm(s)
     assert abs(np.sum(V * out)) / V.sum() < 1e-13 * np.abs(out).max()
 
 
