@@ -7,10 +7,13 @@ Workload (one "step" = one pressure solve from psi = 0 on device-resident data):
   two smoothing passes), rho_g/rho_l = 1/20, beta = alpha dt / rho with
   alpha dt = 1e-3; q = div(u*) of a seeded N(0,1) staggered velocity.
   Variable-density PCG + 4-level multigrid (2+2 red/black sweeps) to
-  ||r||/||q|| <= 1e-8.  The timed path is the mixed-precision form (FP32
-  multigrid inside an FP64 flexible PCG); the same algorithm in FP64 is the
-  "GPU FP64 path" timed beside it, and the reference's own algorithm
-  (Fletcher-Reeves PCG, coarsest level solved to 1e-10) is timed as well.
+  ||r||/||q|| <= 1e-8.  The timed path is the mixed-precision form: an FP32
+  multigrid (the finest level's right-hand side and coefficients stored in
+  bf16, its iterate and every coarser level in FP32) inside an FP64 flexible
+  PCG whose FP64 stop test decides the result.  The same solve with FP32
+  storage, the same algorithm in FP64 (the "GPU FP64 path") and the
+  reference's own algorithm (Fletcher-Reeves PCG, coarsest level solved to
+  1e-10) are timed beside it.
   Below the reference's coarsest level (32x24x32) the V-cycle continues on
   16x12x16, 8x6x8 and a dense 4x3x4 solve.
 
@@ -80,7 +83,8 @@ def small_configs(torch):
     """The two smaller workloads of the benchmark family, timed on the device:
     a projection step of the 64^3 constant-density channel with the FFT solver
     (source, FP64 solve, projection; max |div u| after it) and the FP64
-    PCG+MG solve of the 128^3 channel with 18 bubbles to 1e-8."""
+    PCG+MG solve of the 128^3 channel with 18 bubbles to 1e-8 (FP64 and the
+    two mixed forms)."""
     import tbfsolver_amd as T
     from tbfsolver_amd import fields as F
 
@@ -115,14 +119,15 @@ def small_configs(torch):
     F.computeSource(g1, ux, uy, uz, q)
     beta = torch.zeros_like(drho)
     F.computeBeta(g1, 1e-3, drho, beta)
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "mixed-bf16"):
         s = T.PCG(g1, nLevels=4, tolPCG=1e-8, maxIterPCG=200, precision=prec, flexible=1, coarse_max_iter=0)
         tot, ms, its, _ = timed_solves(torch, s, torch.zeros_like(drho), beta, q, 10, 2)
         out["config1_pcg_128_" + prec] = {"solves_per_s": round(10 / tot, 2), "p50_ms": round(float(np.median(ms)), 4),
                                           "iterations": int(np.median(its))}
         s.close()
-    log("small configs: 64^3 FAST step %.3f ms; 128^3 PCG+MG fp64 %.2f ms, mixed %.2f ms"
-        % (ms0, out["config1_pcg_128_fp64"]["p50_ms"], out["config1_pcg_128_mixed"]["p50_ms"]))
+    log("small configs: 64^3 FAST step %.3f ms; 128^3 PCG+MG fp64 %.2f ms, mixed %.2f ms, mixed-bf16 %.2f ms"
+        % (ms0, out["config1_pcg_128_fp64"]["p50_ms"], out["config1_pcg_128_mixed"]["p50_ms"],
+           out["config1_pcg_128_mixed-bf16"]["p50_ms"]))
     return out
 
 
@@ -258,7 +263,7 @@ def main():
     F.computeBeta(g, adt, drho, beta)
     torch.cuda.synchronize()
 
-    mixed = T.PCG(g, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=200, precision="mixed",
+    mixed = T.PCG(g, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=200, precision="mixed-bf16",
                   flexible=1, coarse_max_iter=0)
     psi = torch.zeros_like(drho)
     total, ms, iters, prof = timed_solves(torch, mixed, psi, beta, q, args.steps, args.warmup, profile="sweep_fine")
@@ -279,13 +284,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 multigrid in fp64 flexible PCG",
+        "dtype": "fp32 multigrid (finest-level q and beta stored as bf16) in fp64 flexible PCG",
         "data": "synthetic (seeded bubbles and velocity, generated on the host)",
         "config": {"workload": "256x192x256 tanh-y (gamma 1.5) channel, 72 bubbles rho_g/rho_l=1/20, "
                                "PCG + 4-level RBGS multigrid to 1e-8, solve from psi=0",
-                   "grid": list(dims), "levels": 4, "sweeps": [2, 2], "tol": 1e-8, "precision": "mixed"},
+                   "grid": list(dims), "levels": 4, "sweeps": [2, 2], "tol": 1e-8, "precision": "mixed-bf16"},
         "iterations": int(np.median(iters)),
     }
+
+    # the same solve with the finest level stored in FP32
+    m32 = T.PCG(g, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=200, precision="mixed",
+                flexible=1, coarse_max_iter=0)
+    tot32, ms32, it32, prof32 = timed_solves(torch, m32, torch.zeros_like(drho), beta, q, args.steps, args.warmup,
+                                             profile="sweep_fine")
+    out["mixed_fp32_storage"] = {"solves_per_s": round(args.steps / tot32, 3),
+                                 "p50_ms": round(float(np.percentile(ms32, 50)), 4),
+                                 "iterations": int(np.median(it32))}
+    if prof32[0] > 0:
+        us32 = prof32[1] / prof32[0] * 1e3
+        out["mixed_fp32_storage"]["sweep_fine"] = {
+            "kernel": "k_sweep<float, float, float, 32, 16, 256, false, false, 4> (16 B per cell)",
+            "avg_us": round(us32, 2), "achieved_gbs": round(prof32[2] / us32 / 1e3, 1),
+            "frac": round(prof32[2] / us32 / 1e3 / HBM_PEAK_GBS, 4)}
+    log("mixed, FP32 storage: %.2f solves/s, iterations %s" % (args.steps / tot32, sorted(set(it32))))
+    m32.close()
 
     # the same solve through the host-array entry point, as the Fortran caller
     # makes it: psi, beta, q in host memory with a one-cell halo (Fortran
@@ -420,17 +442,21 @@ def main():
         copy_gbs = copy_peak(torch)
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                           "kernel": "k_sweep<float, float, 32, 16, 256, false, false, 4>: finest-level red/black "
-                                     "sweep from a non-zero start, out of place (16 B per cell)",
+                           "kernel": "k_sweep<float, bf16s, bf16s, 32, 16, 256, false, false, 4>: finest-level "
+                                     "red/black sweep from a non-zero start, out of place (p FP32 in and out, q "
+                                     "and beta bf16 in: 12 B per cell)",
                            "avg_us": round(avg_s * 1e6, 2), "launches": launches,
                            "algorithmic_bytes_per_launch": bytes_per,
                            "copy_kernel_gbs": round(copy_gbs, 1)}
         log("sweep fine: %d launches, avg %.1f us, %.0f GB/s algorithmic (device copy: %.0f GB/s)"
             % (launches, avg_s * 1e6, achieved, copy_gbs))
         # the whole solve against the bytes a PCG iteration must stream: CG vectors
-        # 128 N plus the multigrid, b N_l (4 (pre + post) + 7.25) summed over levels
+        # 128 N plus the multigrid, per level N_l (16 B per sweep, 4 sweeps; 12.5
+        # B residual-restriction, 8.5 B prolongation, 4 + 4 B for r's copy and z)
+        # and on the bf16-stored finest level 12 B per sweep (8 from zero),
+        # 8.5 + 8.5 + 2 + 4 B
         n0 = float(np.prod(dims))
-        mg = 4.0 * sum(n0 / 8 ** l for l in range(12)) * (4 * (2 + 2) + 7.25)
+        mg = sum(n0 / 8 ** l for l in range(1, 12)) * (4 * 16 + 29.0) + n0 * (8 + 3 * 12 + 23.0)
         per_it = 128.0 * n0 + mg
         solve_gbs = out["iterations"] * per_it / (out["p50_ms"] * 1e-3) / 1e9
         out["solve_roofline"] = {"bytes_per_iteration": per_it, "achieved": round(solve_gbs, 1), "unit": "GB/s",

@@ -41,6 +41,12 @@ extern "C" {
 
 #define TBFG_PREC_FP64 0  /* all arithmetic and storage in FP64                  */
 #define TBFG_PREC_MIXED 1 /* FP32 inner solver inside an FP64 outer iteration    */
+/* PCG only: TBFG_PREC_MIXED with the finest level's multigrid right-hand side
+ * and coefficients stored in bf16 (the smoother's iterate, the arithmetic and
+ * every coarser level stay FP32; the outer iteration and its stop test stay
+ * FP64).  Where the finest level is not smoothed by the plane-march kernel
+ * (too small, or a shape it does not tile) it acts as TBFG_PREC_MIXED. */
+#define TBFG_PREC_MIXED_BF16 2
 
 /* Grid: x and z periodic, y walls.  stretch[a] = 1 selects the hyperbolic
  * tangent distribution of grid.f90:293-330 with factor gamma (the reference
@@ -106,7 +112,8 @@ typedef struct tbfg_pcg_params {
     int32_t max_iter;      /* maxIterPCG                                             */
     double tol;            /* tolPCG on ||r||_2 / ||q||_2                             */
     int32_t mg_cycles;     /* V-cycles per preconditioner application (default 1)    */
-    int32_t precision;     /* TBFG_PREC_FP64 (reference algorithm) or TBFG_PREC_MIXED */
+    int32_t precision;     /* TBFG_PREC_FP64 (reference algorithm), TBFG_PREC_MIXED or
+                              TBFG_PREC_MIXED_BF16                                   */
     int32_t coarse_max_iter; /* coarsest reference level: > 0 inner CG iterations to
                                 coarse_tol, 0 the V-cycle continues below it (one
                                 cycle), -1 by precision (FP64 40, mixed 0)           */

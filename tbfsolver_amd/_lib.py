@@ -18,6 +18,7 @@ TBFG_EDEVICE = 5
 TBFG_EIO = 6
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_MIXED_BF16 = 2
 
 STATUS_NAMES = {0: "TBFG_OK", 1: "TBFG_EINVAL", 2: "TBFG_ENOMEM", 3: "TBFG_ENOTCONVERGED",
                 4: "TBFG_ENUMERIC", 5: "TBFG_EDEVICE", 6: "TBFG_EIO"}

@@ -393,8 +393,9 @@ __global__ void k_fin_alpha(unsigned nblocks, PcgState* st, Reducer red)
 
 // r = q - A psi with ||r||^2, ||q||^2 (computeResiduals, rbgs.f90:276-333, used at pcg.f90:116).
 // r32 (optional): an FP32 copy of r, the right-hand side of the FP32 multigrid.
+template <class TR>
 __global__ void k_residual_init(LevelDev<double> L, const double* __restrict__ beta, const double* __restrict__ q,
-                                const double* __restrict__ psi, double* __restrict__ r, float* __restrict__ r32,
+                                const double* __restrict__ psi, double* __restrict__ r, TR* __restrict__ r32,
                                 PcgState* st, Reducer red)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -410,7 +411,7 @@ __global__ void k_residual_init(LevelDev<double> L, const double* __restrict__ b
             const double qc = q[c];
             const double rc = qc - apply_cell<double>(a, psi, c, e, w, n, s, f, b);
             r[c] = rc;
-            if (r32) r32[c] = (float)rc;
+            if (r32) r32[c] = store_as<TR>((float)rc);
             v[0] += rc * rc;
             v[1] += qc * qc;
         }
@@ -443,7 +444,8 @@ __global__ void k_fin_init(unsigned nblocks, PcgState* st, Reducer red)
 }
 
 // psi += alpha d; r -= alpha qv; ||r|| and the stop test (pcg.f90:228-262,331-355,359-429).
-__global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __restrict__ r, float* __restrict__ r32,
+template <class TR>
+__global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __restrict__ r, TR* __restrict__ r32,
                              const double* __restrict__ d, const double* __restrict__ qv, PcgState* st, Reducer red)
 {
     if (st->done) return;
@@ -454,7 +456,7 @@ __global__ void k_pcg_update(size_t ncell, double* __restrict__ psi, double* __r
         if (psi) psi[t] = psi[t] + alpha * d[t];  // (NULL: deferred to the next operator pass)
         const double rn = r[t] - alpha * qv[t];
         r[t] = rn;
-        if (r32) r32[t] = (float)rn;
+        if (r32) r32[t] = store_as<TR>((float)rn);
         v[0] += rn * rn;
     }
     block_partial<1>(v, red);
@@ -644,8 +646,8 @@ __device__ __forceinline__ T cell_residual(T qc, T bc, T be, T bw, T bn, T bs, T
     return qc - (aR * (pe - pc) - aL * (pc - pw) + aT * (pn - pc) - aB * (pc - ps) + aF * (pf - pc) - aK * (pc - pb));
 }
 
-template <class T, class TQ>
-__global__ void k_resid_restrict(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q,
+template <class T, class TQ, class TB>
+__global__ void k_resid_restrict(LevelDev<T> L, const TB* __restrict__ beta, const TQ* __restrict__ q,
                                  const T* __restrict__ p, T* __restrict__ qc, const int* done)
 {
     if (done && *done) return;
@@ -833,7 +835,7 @@ template <class TS, class TD>
 __global__ void k_convert(size_t n, const TS* __restrict__ src, TD* __restrict__ dst)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) dst[t] = (TD)src[t];
+    if (t < n) dst[t] = store_as<TD>(src[t]);
 }
 
 // ------------------------------------------------------------ tail kernel
@@ -1135,8 +1137,8 @@ __device__ void tail_coarse_pcg(const TailPlan<T>& P)
 // preconditioner application, ||q - A p|| / (||q|| + tiny) on the finest level
 // (computeResiduals, rbgs.f90:276-333, in the multigrid's precision, summed in
 // FP64); the application stops at tolMG or after max_cycles cycles.
-template <class T, class TQ>
-__global__ void k_mg_res(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q,
+template <class T, class TQ, class TB>
+__global__ void k_mg_res(LevelDev<T> L, const TB* __restrict__ beta, const TQ* __restrict__ q,
                          const T* __restrict__ p, const PcgState* st, Reducer red)
 {
     double v[2] = {0.0, 0.0};
@@ -1172,21 +1174,23 @@ __global__ void k_fin_mgres(unsigned nb, double tol, int cyc, int max_cycles, Pc
     }
 }
 
-template <class T>
-int launch_mg_res(const LevelDev<T>& L, const T* beta, const T* q, const T* p, double tol, int cyc, int max_cycles,
+template <class T, class T0>
+int launch_mg_res(const LevelDev<T>& L, const T0* beta, const T0* q, const T* p, double tol, int cyc, int max_cycles,
                   PcgState* st, const Reducer& red, void* stream)
 {
     const size_t n = (size_t)L.nx * L.ny * L.nz;
     // grid-stride: at most 2048 blocks (the reduction scratch holds 5 sums for that many)
     const unsigned nb = std::min<unsigned>(ceil_div(n, 256), 2048u);
-    hipLaunchKernelGGL((k_mg_res<T, T>), dim3(nb), dim3(256), 0, (hipStream_t)stream, L, beta, q, p, st, red);
+    hipLaunchKernelGGL((k_mg_res<T, T0, T0>), dim3(nb), dim3(256), 0, (hipStream_t)stream, L, beta, q, p, st, red);
     hipLaunchKernelGGL(k_fin_mgres, dim3(1), dim3(256), 0, (hipStream_t)stream, nb, tol, cyc, max_cycles, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
-template int launch_mg_res<float>(const LevelDev<float>&, const float*, const float*, const float*, double, int, int,
+template int launch_mg_res<float, float>(const LevelDev<float>&, const float*, const float*, const float*, double, int, int,
                                   PcgState*, const Reducer&, void*);
-template int launch_mg_res<double>(const LevelDev<double>&, const double*, const double*, const double*, double, int,
+template int launch_mg_res<float, bf16s>(const LevelDev<float>&, const bf16s*, const bf16s*, const float*, double, int,
+                                         int, PcgState*, const Reducer&, void*);
+template int launch_mg_res<double, double>(const LevelDev<double>&, const double*, const double*, const double*, double, int,
                                    int, PcgState*, const Reducer&, void*);
 
 // ---- the coarsest-level solve by multi-workgroup kernels (tail_coarse_pcg's
@@ -1782,22 +1786,24 @@ int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, c
     return TBFG_OK;
 }
 
+template <class TR>
 int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
-                         double* r, float* r32, PcgState* st, const Reducer& red, void* stream)
+                         double* r, TR* r32, PcgState* st, const Reducer& red, void* stream)
 {
     dim3 blk = block2d(L.nx);
     dim3 grd = red_grid3(L, blk);
-    hipLaunchKernelGGL(k_residual_init, grd, blk, 0, S(stream), L, beta, q, psi, r, r32, st, red);
+    hipLaunchKernelGGL(k_residual_init<TR>, grd, blk, 0, S(stream), L, beta, q, psi, r, r32, st, red);
     hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, S(stream), grd.x * grd.y * grd.z, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
 
-int launch_pcg_update(size_t n, double* psi, double* r, float* r32, const double* d, const double* qv,
+template <class TR>
+int launch_pcg_update(size_t n, double* psi, double* r, TR* r32, const double* d, const double* qv,
                       PcgState* st, const Reducer& red, void* stream)
 {
     const unsigned nb = red_blocks1(n);
-    hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, S(stream), n, psi, r, r32, d, qv, st, red);
+    hipLaunchKernelGGL(k_pcg_update<TR>, dim3(nb), dim3(256), 0, S(stream), n, psi, r, r32, d, qv, st, red);
     hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(256), 0, S(stream), nb, st, red);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
@@ -1850,13 +1856,13 @@ int launch_rbgs(const LevelDev<T>& L, const T* beta, const TQ* q, T* p, int colo
     return TBFG_OK;
 }
 
-template <class T, class TQ>
-int launch_resid_restrict(const LevelDev<T>& L, const T* beta, const TQ* q, const T* p, T* qc, const int* done,
+template <class T, class TQ, class TB>
+int launch_resid_restrict(const LevelDev<T>& L, const TB* beta, const TQ* q, const T* p, T* qc, const int* done,
                           void* stream)
 {
     dim3 blk = block2d(L.nx >> 1);
     const dim3 grd(ceil_div(L.nx >> 1, blk.x) * ceil_div(L.ny >> 1, blk.y) * ceil_div(L.nz >> 1, kRrPlanes));
-    hipLaunchKernelGGL((k_resid_restrict<T, TQ>), grd, blk, 0, S(stream), L, beta, q, p, qc, done);
+    hipLaunchKernelGGL((k_resid_restrict<T, TQ, TB>), grd, blk, 0, S(stream), L, beta, q, p, qc, done);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
@@ -1929,12 +1935,14 @@ template int launch_rbgs<float, double>(const LevelDev<float>&, const float*, co
                                         const int*, void*);
 template int launch_rbgs<float, float>(const LevelDev<float>&, const float*, const float*, float*, int, bool,
                                        const int*, void*);
-template int launch_resid_restrict<double, double>(const LevelDev<double>&, const double*, const double*,
-                                                   const double*, double*, const int*, void*);
-template int launch_resid_restrict<float, double>(const LevelDev<float>&, const float*, const double*, const float*,
-                                                  float*, const int*, void*);
-template int launch_resid_restrict<float, float>(const LevelDev<float>&, const float*, const float*, const float*,
-                                                 float*, const int*, void*);
+template int launch_resid_restrict<double, double, double>(const LevelDev<double>&, const double*, const double*,
+                                                           const double*, double*, const int*, void*);
+template int launch_resid_restrict<float, double, float>(const LevelDev<float>&, const float*, const double*,
+                                                         const float*, float*, const int*, void*);
+template int launch_resid_restrict<float, float, float>(const LevelDev<float>&, const float*, const float*,
+                                                        const float*, float*, const int*, void*);
+template int launch_resid_restrict<float, bf16s, bf16s>(const LevelDev<float>&, const bf16s*, const bf16s*,
+                                                        const float*, float*, const int*, void*);
 template int launch_restrict<double, double>(const LevelDev<double>&, const double*, double*, void*);
 template int launch_restrict<float, float>(const LevelDev<float>&, const float*, float*, void*);
 template int launch_restrict<double, float>(const LevelDev<float>&, const double*, float*, void*);
@@ -1942,6 +1950,15 @@ template int launch_prolong_add<double>(const LevelDev<double>&, const double*, 
 template int launch_prolong_add<float>(const LevelDev<float>&, const float*, float*, const int*, void*);
 template int launch_convert<double, float>(size_t, const double*, float*, void*);
 template int launch_convert<float, double>(size_t, const float*, double*, void*);
+template int launch_convert<double, bf16s>(size_t, const double*, bf16s*, void*);
+template int launch_residual_init<float>(const LevelDev<double>&, const double*, const double*, const double*,
+                                         double*, float*, PcgState*, const Reducer&, void*);
+template int launch_residual_init<bf16s>(const LevelDev<double>&, const double*, const double*, const double*,
+                                         double*, bf16s*, PcgState*, const Reducer&, void*);
+template int launch_pcg_update<float>(size_t, double*, double*, float*, const double*, const double*, PcgState*,
+                                      const Reducer&, void*);
+template int launch_pcg_update<bf16s>(size_t, double*, double*, bf16s*, const double*, const double*, PcgState*,
+                                      const Reducer&, void*);
 template int launch_tail<double>(const TailPlan<double>*, const int*, size_t, void*);
 template int launch_tail<float>(const TailPlan<float>*, const int*, size_t, void*);
 

@@ -14,6 +14,39 @@ constexpr int RED = 0;
 constexpr int BLACK = 1;
 constexpr int kMaxTail = 16;  // levels handled by the single-workgroup tail kernel
 
+// A bf16 storage value: the upper half of an FP32 word.  Only the finest
+// level's multigrid right-hand side and coefficients are stored in it (the
+// arithmetic stays FP32, the smoother's iterate stays FP32); it converts to
+// float exactly.
+struct bf16s {
+    uint16_t bits;
+    __host__ __device__ operator float() const { return __builtin_bit_cast(float, (uint32_t)bits << 16); }
+};
+// float -> bf16, rounded to nearest even (NaN stays a quiet NaN)
+__host__ __device__ inline bf16s to_bf16(float f)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return bf16s{(uint16_t)((u >> 16) | 0x40u)};
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return bf16s{(uint16_t)(u >> 16)};
+}
+// a value converted to a storage type (bf16 through FP32)
+template <class TD, class TS>
+__host__ __device__ inline TD store_as(TS v)
+{
+    return (TD)v;
+}
+template <>
+__host__ __device__ inline bf16s store_as<bf16s, float>(float v)
+{
+    return to_bf16(v);
+}
+template <>
+__host__ __device__ inline bf16s store_as<bf16s, double>(double v)
+{
+    return to_bf16((float)v);
+}
+
 // Scalars of one PCG solve, resident on the device so that no host round trip
 // sits between the kernels of an iteration.
 struct PcgState {

@@ -26,9 +26,12 @@ int launch_op_dir(const LevelDev<double>& L, const OpPlan& plan, const double* b
                   double* dnew, double* psi, int init, double* qv, PcgState* st, const Reducer& red, void* stream);
 // psi += alpha d of the last iteration that ran (d in d0 or d1 by its parity)
 int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, const PcgState* st, void* stream);
+// r32 (optional): the multigrid's copy of r, TR = float or bf16s
+template <class TR>
 int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
-                         double* r, float* r32, PcgState* st, const Reducer& red, void* stream);
-int launch_pcg_update(size_t n, double* psi, double* r, float* r32, const double* d, const double* qv,
+                         double* r, TR* r32, PcgState* st, const Reducer& red, void* stream);
+template <class TR>
+int launch_pcg_update(size_t n, double* psi, double* r, TR* r32, const double* d, const double* qv,
                       PcgState* st, const Reducer& red, void* stream);
 template <class TZ>
 int launch_pcg_dots(size_t n, const double* r, const TZ* z, const double* qv, int init, PcgState* st,
@@ -63,13 +66,15 @@ bool brick_supported(const LevelDev<T>& L);
 template <class T, class TQ>
 int launch_brick_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
                        const int* done, void* stream);
-template <class T, class TQ>
-int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
+// (TB: the storage type of beta, T or, on the finest level, bf16s)
+template <class T, class TQ, class TB = T>
+int launch_sweep(const LevelDev<T>& L, const TB* beta, const TQ* q, const T* pin, T* pout, bool first,
                  const int* done, void* stream, const SweepDots* dots = nullptr, unsigned* nblocks_out = nullptr,
                  const KernelEvents* kev = nullptr);
 // solveMG's stop test after V-cycle cyc: sets PcgState::mg_stop
-template <class T>
-int launch_mg_res(const LevelDev<T>& L, const T* beta, const T* q, const T* p, double tol, int cyc, int max_cycles,
+// (T0: the storage type of the finest level's beta and q)
+template <class T, class T0>
+int launch_mg_res(const LevelDev<T>& L, const T0* beta, const T0* q, const T* p, double tol, int cyc, int max_cycles,
                   PcgState* st, const Reducer& red, void* stream);
 
 // the coarsest-level solve by multi-workgroup kernels, in phases driven by the
@@ -90,8 +95,8 @@ template <class T>
 int launch_inner(InnerOp op, const LevelDev<T>& L, const InnerArgs<T>& a, InnerState* is, PcgState* st,
                  const Reducer& red, void* stream);
 int launch_fin_dots(unsigned nblocks, size_t ncell, int init, PcgState* st, const Reducer& red, void* stream);
-template <class T, class TQ>
-int launch_resid_restrict(const LevelDev<T>& L, const T* beta, const TQ* q, const T* p, T* qc, const int* done,
+template <class T, class TQ, class TB = T>
+int launch_resid_restrict(const LevelDev<T>& L, const TB* beta, const TQ* q, const T* p, T* qc, const int* done,
                           void* stream);
 template <class TF, class T>
 int launch_restrict(const LevelDev<T>& L, const TF* fine, T* coarse, void* stream);

@@ -123,16 +123,18 @@ int pcg_profile_end(tbfg_pcg* h, int64_t* launches, double* total_ms, double* by
     TBFG_HIP(hipDeviceSynchronize());
     prof_harvest(h);
     const double n = (double)h->n0;
-    const double bp = h->mixed ? 4.0 : 8.0;  // MG storage (p, beta, q of the finest level)
+    const double bp = h->mixed ? 4.0 : 8.0;  // MG storage of p (and beta, q) on the finest level
+    const double bq = h->b16 ? 2.0 : bp;      // ... of beta and q
+    const double rq = h->b16 ? 2.0 : 4.0;     // the multigrid's copy of r (mixed)
     double bytes = 0.0;
     switch (h->prof_kernel) {
-        case 0: bytes = 4.0 * bp * n; break;                               // full sweep: p, beta, q in; p out
-        case 1: bytes = 3.0 * bp * n + bp * n / 8; break;                  // p, beta, q in; coarse q out
+        case 0: bytes = (2.0 * bp + 2.0 * bq) * n; break;                 // full sweep: p, beta, q in; p out
+        case 1: bytes = (bp + 2.0 * bq) * n + bp * n / 8; break;          // p, beta, q in; coarse q out
         case 2:  // d, beta in; qv out (fused: z, d_old, beta, psi in; d, qv, psi out)
             bytes = h->fuse_op ? (48.0 + (h->mixed && !h->fps ? 4.0 : 8.0)) * n : 24.0 * n;
             break;
         case 3:  // psi, r, d, qv in; psi, r (+ FP32 r) out (fused: r, qv in; r (+ FP32 r) out)
-            bytes = (h->fuse_op ? 24.0 * n : 48.0 * n) + (h->mixed && !h->fps ? 4.0 * n : 0.0);
+            bytes = (h->fuse_op ? 24.0 * n : 48.0 * n) + (h->mixed && !h->fps ? rq * n : 0.0);
             break;
     }
     if (launches) *launches = h->prof_launches;
@@ -245,7 +247,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
                   prm.n_post, prm.mg_cycles, prm.max_iter, prm.tol, prm.coarse_max_iter, prm.coarse_tol);
         return TBFG_EINVAL;
     }
-    if (prm.precision != TBFG_PREC_FP64 && prm.precision != TBFG_PREC_MIXED) {
+    if (prm.precision != TBFG_PREC_FP64 && prm.precision != TBFG_PREC_MIXED && prm.precision != TBFG_PREC_MIXED_BF16) {
         set_error("precision = %d", prm.precision);
         return TBFG_EINVAL;
     }
@@ -264,7 +266,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     std::unique_ptr<tbfg_pcg> h(new tbfg_pcg());
     h->g = *g;
     h->prm = prm;
-    h->mixed = prm.precision == TBFG_PREC_MIXED;
+    h->mixed = prm.precision == TBFG_PREC_MIXED || prm.precision == TBFG_PREC_MIXED_BF16;
     if (h->prm.flexible < 0) h->prm.flexible = h->mixed ? 1 : 0;
     if (!(h->prm.coarse_tol > 0.0)) h->prm.coarse_tol = h->mixed ? 1e-5 : 1e-10;
     if (h->prm.coarse_max_iter < 0) h->prm.coarse_max_iter = h->mixed ? 0 : 40;
@@ -357,7 +359,6 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
         const size_t n = h->hl[l].cells();
         TBFG_TRY(h->p[l].alloc(n * es));
         if (l > 0) TBFG_TRY(h->q[l].alloc(n * es));
-        if (l > 0 || h->mixed) TBFG_TRY(h->beta[l].alloc(n * es));
         if (l < (h->kin >= 0 ? h->kin : t0)) {
             h->fused[l] = h->mixed ? sweep_supported<float>(h->tabf[l]->view) : sweep_supported<double>(h->tabd[l]->view);
             // below ~1M cells a plane march of a few steps per workgroup is
@@ -371,8 +372,13 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
                 h->fused[l] = 2;
             if (h->fused[l]) TBFG_TRY(h->palt[l].alloc(n * es));
         }
+        // bf16 storage of the finest level's q and beta: where the plane march smooths it
+        if (l == 0)
+            h->b16 = prm.precision == TBFG_PREC_MIXED_BF16 && h->fused[0] == 1 &&
+                     prm.preconditioner != TBFG_PRECOND_FFT;
+        if (l > 0 || h->mixed) TBFG_TRY(h->beta[l].alloc(n * (l == 0 && h->b16 ? sizeof(bf16s) : es)));
     }
-    if (h->mixed) TBFG_TRY(h->r32.alloc(h->n0 * 4));
+    if (h->mixed) TBFG_TRY(h->r32.alloc(h->n0 * (h->b16 ? sizeof(bf16s) : sizeof(float))));
     h->fuse_dots = !std::getenv("TBFG_UNFUSED_DOTS");
     if (prm.preconditioner == TBFG_PRECOND_FFT) {
         // FP32 transforms without refinement in mixed precision (a preconditioner)
@@ -458,17 +464,24 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
 
 // beta on every level (restricted once per solve; multiGrid.f90:210 restricts it
 // every V-cycle, but it does not change within a solve) and the dense bottom.
-template <class T>
+template <class T, class T0 = T>
 static int setup_levels(tbfg_pcg* h, const double* beta0, void* stream)
 {
     const T* prev;
-    if constexpr (sizeof(T) == 4) {
+    int l = 1;
+    if constexpr (std::is_same<T0, bf16s>::value) {
+        // bf16 finest level: level 1 restricted from the FP64 beta
+        TBFG_TRY(launch_convert<double, bf16s>(h->n0, beta0, h->beta[0].as<bf16s>(), stream));
+        TBFG_TRY(launch_restrict<double, float>(geo<float>(h, 0), beta0, h->beta[1].as<float>(), stream));
+        prev = h->beta[1].as<T>();
+        l = 2;
+    } else if constexpr (sizeof(T) == 4) {
         TBFG_TRY(launch_convert<double, float>(h->n0, beta0, h->beta[0].as<float>(), stream));
         prev = h->beta[0].as<T>();
     } else {
         prev = beta0;
     }
-    for (int l = 1; l < h->ntot; ++l) {
+    for (; l < h->ntot; ++l) {
         TBFG_TRY(launch_restrict<T, T>(geo<T>(h, l - 1), prev, h->beta[l].as<T>(), stream));
         prev = h->beta[l].as<T>();
     }
@@ -482,14 +495,28 @@ static int setup_levels(tbfg_pcg* h, const double* beta0, void* stream)
 // their two buffers, so p (and alt) may be swapped on return.
 // last_dots: the dot products of the PCG are formed by the last sweep (see
 // launch_sweep); *nb_out receives its partial count.
-template <class T>
-static int smooth(tbfg_pcg* h, int l, const T* b, const T* q, T*& p, T*& alt, int nsweeps, bool first,
+// (T0: the storage type of the level's q and beta, T or bf16s; bf16s only
+// where the plane march smooths the level)
+template <class T, class T0 = T>
+static int smooth(tbfg_pcg* h, int l, const T0* b, const T0* q, T*& p, T*& alt, int nsweeps, bool first,
                   const int* done, void* stream, const SweepDots* last_dots = nullptr, unsigned* nb_out = nullptr)
 {
     const LevelDev<T>& L = geo<T>(h, l);
+    constexpr bool narrow = !std::is_same<T0, T>::value;
+    if (narrow && h->fused[l] != 1) {
+        set_error("smooth: bf16 storage on a level without the plane-march sweep");
+        return TBFG_EINVAL;
+    }
     for (int s = 0; s < nsweeps; ++s) {
         const bool f0 = first && s == 0;
-        if (h->fused[l] == 2) {
+        if constexpr (narrow) {
+            const bool last = s == nsweeps - 1;
+            const bool plain = l == 0 && !f0 && !(last && last_dots);
+            ProfScope ps(h, plain ? 0 : -1, stream, true);
+            TBFG_TRY(launch_sweep<T, T0, T0>(L, b, q, p, alt, f0, done, stream, last ? last_dots : nullptr,
+                                             last ? nb_out : nullptr, ps.events()));
+            std::swap(p, alt);
+        } else if (h->fused[l] == 2) {
             ProfScope ps(h, l == 0 ? 0 : -1, stream);
             TBFG_TRY(launch_brick_sweep<T, T>(L, b, q, p, alt, f0, done, stream));
             std::swap(p, alt);
@@ -604,8 +631,8 @@ static int inner_solve(tbfg_pcg* h, void* stream)
 // points at the level-0 buffer that holds the result.  dots_init >= 0: the
 // finest level's last sweep also forms the PCG dot products of z (k_pcg_dots
 // with that init flag, finalized here); *dots_done tells whether it could.
-template <class T>
-static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* done, void* stream,
+template <class T, class T0 = T>
+static int vcycle(tbfg_pcg* h, const T0* r0, T** z, const T0* beta0, const int* done, void* stream,
                   int dots_init = -1, bool* dots_done = nullptr)
 {
     if (dots_done) *dots_done = false;
@@ -635,13 +662,17 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
     for (int cyc = 0; cyc < ncycles; ++cyc) {
         for (int l = 0; l < t0; ++l) {
             const bool first = (l > 0) || (cyc == 0);
-            const T* b = (l == 0) ? beta0 : h->beta[l].as<T>();
-            const T* q = (l == 0) ? r0 : h->q[l].as<T>();
             if (h->prm.n_pre == 0 && first)
                 TBFG_HIP(hipMemsetAsync(cur[l], 0, h->hl[l].cells() * sizeof(T), (hipStream_t)stream));
-            TBFG_TRY(smooth<T>(h, l, b, q, cur[l], alt[l], h->prm.n_pre, first, done, stream));
-            {
-                ProfScope ps(h, l == 0 ? 1 : -1, stream);
+            if (l == 0) {
+                TBFG_TRY(smooth<T, T0>(h, 0, beta0, r0, cur[0], alt[0], h->prm.n_pre, first, done, stream));
+                ProfScope ps(h, 1, stream);
+                TBFG_TRY(launch_resid_restrict<T, T0, T0>(geo<T>(h, 0), beta0, r0, cur[0], h->q[1].as<T>(), done,
+                                                          stream));
+            } else {
+                const T* b = h->beta[l].as<T>();
+                const T* q = h->q[l].as<T>();
+                TBFG_TRY(smooth<T>(h, l, b, q, cur[l], alt[l], h->prm.n_pre, first, done, stream));
                 TBFG_TRY(launch_resid_restrict<T, T>(geo<T>(h, l), b, q, cur[l], h->q[l + 1].as<T>(), done, stream));
             }
         }
@@ -650,14 +681,16 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
         else
             TBFG_TRY(launch_tail<T>(h->plan.as<TailPlan<T>>(), done, h->tail_lds, stream));
         for (int l = t0 - 1; l >= 0; --l) {
-            const T* b = (l == 0) ? beta0 : h->beta[l].as<T>();
-            const T* q = (l == 0) ? r0 : h->q[l].as<T>();
             const T* pc = (l + 1 < t0) ? cur[l + 1] : h->p[l + 1].as<T>();
             TBFG_TRY(launch_prolong_add<T>(geo<T>(h, l), pc, cur[l], done, stream));
             const bool fuse = want_dots && l == 0 && cyc == ncycles - 1;
             unsigned nb = 0;
-            TBFG_TRY(smooth<T>(h, l, b, q, cur[l], alt[l], h->prm.n_post, false, done, stream, fuse ? &sd : nullptr,
-                               fuse ? &nb : nullptr));
+            if (l == 0)
+                TBFG_TRY(smooth<T, T0>(h, 0, beta0, r0, cur[0], alt[0], h->prm.n_post, false, done, stream,
+                                       fuse ? &sd : nullptr, fuse ? &nb : nullptr));
+            else
+                TBFG_TRY(smooth<T>(h, l, h->beta[l].as<T>(), h->q[l].as<T>(), cur[l], alt[l], h->prm.n_post, false,
+                                   done, stream));
             if (fuse) {
                 TBFG_TRY(launch_fin_dots(nb, h->n0, dots_init, h->st.as<PcgState>(), sd.red, stream));
                 if (dots_done) *dots_done = true;
@@ -665,7 +698,7 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
         }
         if (tol_mode) {
             PcgState* st = h->st.as<PcgState>();
-            TBFG_TRY(launch_mg_res<T>(geo<T>(h, 0), beta0, r0, cur[0], h->prm.mg_tol, cyc, ncycles, st,
+            TBFG_TRY(launch_mg_res<T, T0>(geo<T>(h, 0), beta0, r0, cur[0], h->prm.mg_tol, cyc, ncycles, st,
                                       Reducer{h->partials.as<double>()}, stream));
             if (cyc + 1 < ncycles) {
                 TBFG_HIP(hipMemcpyAsync(h->hmg, &st->mg_stop, sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -680,8 +713,9 @@ static int vcycle(tbfg_pcg* h, const T* r0, T** z, const T* beta0, const int* do
 }
 
 // T: the multigrid's precision; FFT: the preconditioner is the FFT solve of
-// the constant-coefficient operator (z in FP64) instead of the V-cycle
-template <class T, bool FFT = false>
+// the constant-coefficient operator (z in FP64) instead of the V-cycle; T0:
+// the storage type of the finest level's multigrid q and beta (T or bf16s)
+template <class T, bool FFT = false, class T0 = T>
 static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t* iters, double* res,
                    void* stream)
 {
@@ -695,10 +729,10 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     h->hst->max_iter = h->prm.max_iter;
     h->hst->flexible = h->prm.flexible;
     TBFG_HIP(hipMemcpyAsync(st, h->hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
-    if (!FFT) TBFG_TRY(setup_levels<T>(h, beta, stream));
-    const T* beta0;
+    if (!FFT) TBFG_TRY(setup_levels<T, T0>(h, beta, stream));
+    const T0* beta0;
     if constexpr (sizeof(T) == 4)
-        beta0 = h->beta[0].as<T>();
+        beta0 = h->beta[0].as<T0>();
     else
         beta0 = beta;
     TZ* z = nullptr;
@@ -707,8 +741,9 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     double* r = h->r.as<double>();
     double* d = h->d.as<double>();
     double* qv = h->qv.as<double>();
-    float* r32 = (h->mixed && !FFT) ? h->r32.as<float>() : nullptr;
-    const T* r0;
+    using TR = typename std::conditional<std::is_same<T0, bf16s>::value, bf16s, float>::type;
+    TR* r32 = (h->mixed && !FFT) ? h->r32.as<TR>() : nullptr;
+    const T0* r0;
     if constexpr (sizeof(T) == 4)
         r0 = r32;
     else
@@ -721,7 +756,7 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
             TBFG_TRY(fps_solve_dev(h->fps, r, z, stream));
             TBFG_TRY(launch_pcg_dots<double>(n, r, z, qv, init, st, red, stream));
         } else {
-            TBFG_TRY(vcycle<T>(h, r0, &z, beta0, done, stream, h->fuse_dots ? init : -1, &dots_done));
+            TBFG_TRY(vcycle<T, T0>(h, r0, &z, beta0, done, stream, h->fuse_dots ? init : -1, &dots_done));
             if (!dots_done) TBFG_TRY(launch_pcg_dots<T>(n, r, z, qv, init, st, red, stream));
         }
         return TBFG_OK;
@@ -811,11 +846,12 @@ int pcg_solve_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q,
     }
     if (h->fps)
         return solve_t<double, true>(h, psi, beta, q, iters, res, stream);
+    if (h->b16) return solve_t<float, false, bf16s>(h, psi, beta, q, iters, res, stream);
     return h->mixed ? solve_t<float>(h, psi, beta, q, iters, res, stream)
                     : solve_t<double>(h, psi, beta, q, iters, res, stream);
 }
 
-template <class T>
+template <class T, class T0 = T>
 static int precond_t(tbfg_pcg* h, const double* beta, const double* r, double* z, void* stream)
 {
     hipStream_t s = (hipStream_t)stream;
@@ -823,21 +859,21 @@ static int precond_t(tbfg_pcg* h, const double* beta, const double* r, double* z
     Reducer red{h->partials.as<double>()};
     std::memset(h->hst, 0, sizeof(PcgState));
     TBFG_HIP(hipMemcpyAsync(st, h->hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
-    TBFG_TRY(setup_levels<T>(h, beta, stream));
-    const T* beta0;
+    TBFG_TRY(setup_levels<T, T0>(h, beta, stream));
+    const T0* beta0;
     if constexpr (sizeof(T) == 4)
-        beta0 = h->beta[0].as<T>();
+        beta0 = h->beta[0].as<T0>();
     else
         beta0 = beta;
-    const T* r0;
+    const T0* r0;
     if constexpr (sizeof(T) == 4) {
-        TBFG_TRY(launch_convert<double, float>(h->n0, r, h->r32.as<float>(), stream));
-        r0 = h->r32.as<float>();
+        TBFG_TRY(launch_convert<double, T0>(h->n0, r, h->r32.as<T0>(), stream));
+        r0 = h->r32.as<T0>();
     } else {
         r0 = r;
     }
     T* zt = nullptr;
-    TBFG_TRY(vcycle<T>(h, r0, &zt, beta0, nullptr, stream));
+    TBFG_TRY(vcycle<T, T0>(h, r0, &zt, beta0, nullptr, stream));
     // z with its arithmetic mean removed, as the reference's sweeps leave it
     TBFG_TRY(launch_pcg_dots<T>(h->n0, r, zt, nullptr, 1, st, red, stream));
     TBFG_TRY(launch_pcg_dir<T>(h->n0, zt, z, 1, st, stream));
@@ -855,6 +891,7 @@ int pcg_precondition_dev(tbfg_pcg* h, const double* beta, const double* r, doubl
         return TBFG_EINVAL;
     }
     if (h->fps) return fps_solve_dev(h->fps, r, z, stream);
+    if (h->b16) return precond_t<float, bf16s>(h, beta, r, z, stream);
     return h->mixed ? precond_t<float>(h, beta, r, z, stream) : precond_t<double>(h, beta, r, z, stream);
 }
 

@@ -10,6 +10,7 @@ struct tbfg_pcg {
     tbfg_grid_desc g{};
     tbfg_pcg_params prm{};
     bool mixed = false;
+    bool b16 = false;  // mixed, with the finest level's multigrid q and beta stored in bf16
     int nuser = 0;   // reference levels (nLevels)
     int ntot = 0;    // reference levels + sub-levels of the coarsest-solve hierarchy
     int t0 = 0;      // first level run by the tail kernel
@@ -21,7 +22,7 @@ struct tbfg_pcg {
     std::vector<std::unique_ptr<tbfg::LevelTables<float>>> tabf;   // FP32 tables (mixed)
 
     // per-level MG arrays (element type float in mixed mode, double otherwise);
-    // level 0: p = z, q = r (double), beta = caller's (FP64) or a float copy (mixed)
+    // level 0: p = z, q = r (double), beta = caller's (FP64) or a float (b16: bf16) copy (mixed)
     std::vector<tbfg::DevBuf> p, q, beta;
     std::vector<tbfg::DevBuf> palt;     // second p buffer of the levels smoothed out of place
     std::vector<char> fused;            // level smoothed out of place, one launch per sweep: 1 plane march, 2 bricks
@@ -30,7 +31,7 @@ struct tbfg_pcg {
     tbfg::OpPlan opp;                   // launch of the finest level's operator product
     bool fuse_op = false;               // direction + operator + deferred psi update in one pass (d2: the pair)
     tbfg::DevBuf r, d, d2, qv;          // CG vectors (FP64)
-    tbfg::DevBuf r32;                   // FP32 copy of r (mixed precision: the multigrid's right-hand side)
+    tbfg::DevBuf r32;                   // FP32 (b16: bf16) copy of r, the multigrid's right-hand side (mixed)
     tbfg::DevBuf st, partials, counter; // PcgState + reduction scratch
     tbfg::DevBuf plan;                  // TailPlan<T>
     tbfg::DevBuf G, dwork, vol;         // dense bottom operator and its set-up scratch

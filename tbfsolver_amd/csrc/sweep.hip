@@ -79,6 +79,11 @@ __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uns
 {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
+template <>
+__device__ __forceinline__ bf16s buf_load<bf16s>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return bf16s{__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0)};
+}
 
 template <class T>
 struct Vec2;
@@ -117,9 +122,11 @@ struct ColHist {
 // accumulate while the plane is written, one partial per workgroup, instead of
 // a separate pass over r, z and qv.  r and qv of a plane are loaded with the
 // plane registers, two steps ahead.
-template <class T, class TQ, int TX, int TY, int NT, bool FIRST, bool DOTS, int PD>
+// TQ, TB: the storage types of q and beta (bf16s on a bf16-stored finest
+// level: the loads convert to T, LDS and arithmetic stay in T).
+template <class T, class TQ, class TB, int TX, int TY, int NT, bool FIRST, bool DOTS, int PD>
 __global__ void __launch_bounds__(NT)
-    k_sweep(LevelDev<T> L, const T* __restrict__ beta, const TQ* __restrict__ q, const T* __restrict__ pin,
+    k_sweep(LevelDev<T> L, const TB* __restrict__ beta, const TQ* __restrict__ q, const T* __restrict__ pin,
             T* __restrict__ pout, int kc, int tiles_x, int ntiles, const int* done, SweepDots dots)
 {
     using Sh = SweepShape<T, TX, TY, NT>;
@@ -199,7 +206,7 @@ __global__ void __launch_bounds__(NT)
     const size_t plane_sz = (size_t)nx * ny;
     const size_t level_sz = plane_sz * nz;
     const __amdgpu_buffer_rsrc_t rs_p = plane_rsrc(pin, level_sz * sizeof(T));
-    const __amdgpu_buffer_rsrc_t rs_b = plane_rsrc(beta, level_sz * sizeof(T));
+    const __amdgpu_buffer_rsrc_t rs_b = plane_rsrc(beta, level_sz * sizeof(TB));
     const __amdgpu_buffer_rsrc_t rs_q = plane_rsrc(q, level_sz * sizeof(TQ));
     // this thread's tile cell pair and the column used at steps of each parity
     const int row = threadIdx.x / (TX / 2), h = threadIdx.x - row * (TX / 2);
@@ -217,10 +224,11 @@ __global__ void __launch_bounds__(NT)
     auto fetch_pb = [&](T(&rp)[Sh::NP], T(&rb)[Sh::NP], int kp) {
         kp = min(kp, klast);
         const unsigned so = (unsigned)(plane_sz * modz(kp) * sizeof(T));
+        const unsigned sob = (unsigned)(plane_sz * modz(kp) * sizeof(TB));
 #pragma unroll
         for (int m = 0; m < Sh::NP; ++m) {
             if (!FIRST) rp[m] = buf_load<T>(rs_p, offp[m], so);
-            rb[m] = buf_load<T>(rs_b, offp[m], so);
+            rb[m] = (T)buf_load<TB>(rs_b, offp[m] / (unsigned)sizeof(T) * (unsigned)sizeof(TB), sob);
         }
     };
     // ... and, for DOTS, r and qv of this thread's pair on plane kp - 2 (the
@@ -678,8 +686,8 @@ constexpr int sweep_pd_max()
     return sizeof(T) == 4 && !DOTS ? 4 : 2;
 }
 
-template <class T, class TQ, int TX, int TY, int NT, bool FIRST, bool DOTS>
-static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout,
+template <class T, class TQ, class TB, int TX, int TY, int NT, bool FIRST, bool DOTS>
+static int launch_sweep_shape(const LevelDev<T>& L, const TB* beta, const TQ* q, const T* pin, T* pout,
                               const int* done, const SweepDots& dots, unsigned* nblocks_out, void* stream,
                               const KernelEvents* kev)
 {
@@ -692,7 +700,7 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
         int dev = 0, ncu_q = 0, nb = 0;
         TBFG_HIP(hipGetDevice(&dev));
         TBFG_HIP(hipDeviceGetAttribute(&ncu_q, hipDeviceAttributeMultiprocessorCount, dev));
-        TBFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, pdm>, NT, 0));
+        TBFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<T, TQ, TB, TX, TY, NT, FIRST, DOTS, pdm>, NT, 0));
         o = ((int64_t)(nb < 1 ? 1 : nb) << 32) | (uint32_t)ncu_q;
         occ.store(o, std::memory_order_release);
     }
@@ -716,7 +724,7 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
         set_error("launch_sweep: %u workgroups exceed the reduction scratch", nblocks);
         return TBFG_EINVAL;
     }
-    auto k = pd == 4 ? k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, pdm> : k_sweep<T, TQ, TX, TY, NT, FIRST, DOTS, 2>;
+    auto k = pd == 4 ? k_sweep<T, TQ, TB, TX, TY, NT, FIRST, DOTS, pdm> : k_sweep<T, TQ, TB, TX, TY, NT, FIRST, DOTS, 2>;
     if (kev)
         hipExtLaunchKernelGGL(k, dim3(nblocks), dim3(NT), 0, (hipStream_t)stream, kev->start, kev->stop, 0, L, beta,
                               q, pin, pout, kc, tiles_x, ntiles, done, dots);
@@ -728,23 +736,23 @@ static int launch_sweep_shape(const LevelDev<T>& L, const T* beta, const TQ* q, 
     return TBFG_OK;
 }
 
-template <class T, class TQ, int TX, int TY, int NT>
-static int launch_sweep_first(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
+template <class T, class TQ, class TB, int TX, int TY, int NT>
+static int launch_sweep_first(const LevelDev<T>& L, const TB* beta, const TQ* q, const T* pin, T* pout, bool first,
                               const int* done, const SweepDots* dots, unsigned* nblocks_out, void* stream,
                               const KernelEvents* kev)
 {
     const SweepDots none{};
     if (dots)  // the dot products follow a post-smoothing sweep, never the first
-        return launch_sweep_shape<T, TQ, TX, TY, NT, false, true>(L, beta, q, pin, pout, done, *dots, nblocks_out,
+        return launch_sweep_shape<T, TQ, TB, TX, TY, NT, false, true>(L, beta, q, pin, pout, done, *dots, nblocks_out,
                                                                   stream, kev);
-    return first ? launch_sweep_shape<T, TQ, TX, TY, NT, true, false>(L, beta, q, pin, pout, done, none,
+    return first ? launch_sweep_shape<T, TQ, TB, TX, TY, NT, true, false>(L, beta, q, pin, pout, done, none,
                                                                       nblocks_out, stream, kev)
-                 : launch_sweep_shape<T, TQ, TX, TY, NT, false, false>(L, beta, q, pin, pout, done, none,
+                 : launch_sweep_shape<T, TQ, TB, TX, TY, NT, false, false>(L, beta, q, pin, pout, done, none,
                                                                        nblocks_out, stream, kev);
 }
 
-template <class T, class TQ>
-int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin, T* pout, bool first,
+template <class T, class TQ, class TB>
+int launch_sweep(const LevelDev<T>& L, const TB* beta, const TQ* q, const T* pin, T* pout, bool first,
                  const int* done, void* stream, const SweepDots* dots, unsigned* nblocks_out,
                  const KernelEvents* kev)
 {
@@ -759,7 +767,7 @@ int launch_sweep(const LevelDev<T>& L, const T* beta, const TQ* q, const T* pin,
     }
 #define TBFG_SWEEP_CASE(X, Y, N)                                                             \
     if (c.tx == X && c.ty == Y && c.nt == N)                                                \
-        return launch_sweep_first<T, TQ, X, Y, N>(L, beta, q, pin, pout, first, done, dots, nblocks_out, stream, kev);
+        return launch_sweep_first<T, TQ, TB, X, Y, N>(L, beta, q, pin, pout, first, done, dots, nblocks_out, stream, kev);
     if constexpr (sizeof(T) == 4) {
         TBFG_SWEEP_CASE(64, 8, 256)
         TBFG_SWEEP_CASE(64, 16, 512)
@@ -783,9 +791,13 @@ template int launch_brick_sweep<double, double>(const LevelDev<double>&, const d
                                                 const double*, double*, bool, const int*, void*);
 template bool sweep_supported<float>(const LevelDev<float>&);
 template bool sweep_supported<double>(const LevelDev<double>&);
-template int launch_sweep<float, float>(const LevelDev<float>&, const float*, const float*, const float*, float*,
-                                        bool, const int*, void*, const SweepDots*, unsigned*, const KernelEvents*);
-template int launch_sweep<double, double>(const LevelDev<double>&, const double*, const double*, const double*,
+template int launch_sweep<float, float, float>(const LevelDev<float>&, const float*, const float*, const float*,
+                                               float*, bool, const int*, void*, const SweepDots*, unsigned*,
+                                               const KernelEvents*);
+template int launch_sweep<float, bf16s, bf16s>(const LevelDev<float>&, const bf16s*, const bf16s*, const float*,
+                                               float*, bool, const int*, void*, const SweepDots*, unsigned*,
+                                               const KernelEvents*);
+template int launch_sweep<double, double, double>(const LevelDev<double>&, const double*, const double*, const double*,
                                           double*, bool, const int*, void*, const SweepDots*, unsigned*,
                                           const KernelEvents*);
 

@@ -13,7 +13,7 @@ module tbfgpu_mod
 
     integer(c_int), parameter :: TBFG_OK = 0, TBFG_EINVAL = 1, TBFG_ENOMEM = 2, &
                                  TBFG_ENOTCONVERGED = 3, TBFG_ENUMERIC = 4, TBFG_EDEVICE = 5
-    integer(c_int32_t), parameter :: TBFG_PREC_FP64 = 0, TBFG_PREC_MIXED = 1
+    integer(c_int32_t), parameter :: TBFG_PREC_FP64 = 0, TBFG_PREC_MIXED = 1, TBFG_PREC_MIXED_BF16 = 2
 
     type, bind(C) :: tbfg_grid_desc
         integer(c_int32_t) :: n(3)

@@ -66,6 +66,8 @@ class FastPoissonSolver:
         # fastPoissonSolverCTOR
         self.grid = grid
         o = FpsOpts()
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         o.precision = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
         o.refine_sweeps = int(refine_sweeps)
         h = C.c_void_p()
@@ -121,7 +123,11 @@ class PCG:
     Parameters carry the reference's names (channel_18/specs/pcg_solver).  After
     solvePCG, `iter_` and `res_` hold what the reference's fields hold
     (pcg.f90:36-37: iter_ counts stop tests, i.e. updates + 1 on convergence) and
-    `iterations` the number of direction updates."""
+    `iterations` the number of direction updates.
+
+    precision: "fp64" (the reference's arithmetic), "mixed" (FP32 multigrid
+    inside the FP64 iteration) or "mixed-bf16" (as "mixed", with the finest
+    level's multigrid right-hand side and coefficients stored in bf16)."""
 
     def __init__(self, grid, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=100,
                  precision="fp64", mg_cycles=1, coarse_max_iter=-1, coarse_tol=0.0, flexible=None,
@@ -131,7 +137,10 @@ class PCG:
         p = default_params()
         p.n_levels, p.n_pre, p.n_post = int(nLevels), int(nPreSweep), int(nPostSweep)
         p.tol, p.max_iter, p.mg_cycles = float(tolPCG), int(maxIterPCG), int(mg_cycles)
-        p.precision = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
+        precs = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED, "mixed-bf16": _lib.PREC_MIXED_BF16}
+        if precision not in precs:
+            raise ValueError("precision must be one of %s" % ", ".join(sorted(precs)))
+        p.precision = precs[precision]
         p.coarse_max_iter = int(coarse_max_iter)
         p.coarse_tol = float(coarse_tol)
         p.flexible = -1 if flexible is None else int(flexible)

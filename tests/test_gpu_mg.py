@@ -303,6 +303,69 @@ def test_pcg_mixed_vs_fp64_oracle(torch_gpu, n, stretch, nb):
         assert rel(mean_free(out), mean_free(ref)) < 1e3 * tol
 
 
+def bf16_engaged(g, **kw):
+    """The finest level's q and beta are stored in bf16 exactly when the handle
+    is 4 bytes per finest cell smaller than its FP32-storage twin."""
+    a = T.PCG(g, precision="mixed", **kw)
+    b = T.PCG(g, precision="mixed-bf16", **kw)
+    d = a.footprint - b.footprint
+    a.close()
+    b.close()
+    return d == 4 * g.n[0] * g.n[1] * g.n[2]
+
+
+@pytest.mark.parametrize("n,stretch,nb", [((64, 64, 64), 0, 18), ((64, 48, 64), 1, 20)])
+def test_pcg_mixed_bf16_vs_fp64_oracle(torch_gpu, monkeypatch, n, stretch, nb):
+    """bf16 storage of the finest level's multigrid right-hand side and
+    coefficients (plane-march sweeps forced at this size): the same bounds as
+    the FP32-storage solve -- the FP64 iteration's stop test decides the
+    result, the preconditioner's storage only the iteration count."""
+    print("case pcg mixed-bf16", n, stretch, flush=True)
+    monkeypatch.setenv("TBFG_SWEEP_MIN_CELLS", "0")
+    g, go, rho, _, q, beta = bubble_case(n, nb, stretch, 31)
+    assert bf16_engaged(g, nLevels=4)
+    ref, _, _, _ = O.pcg_solve(go, O.params(n_levels=4, tol=1e-13, max_iter=300, flexible=1), beta, q)
+    its = {}
+    for tol in (1e-8, 1e-11):
+        for precision in ("mixed", "mixed-bf16"):
+            s = T.PCG(g, nLevels=4, tolPCG=tol, maxIterPCG=300, precision=precision)
+            psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+            s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+            assert s.converged and s.res_ <= tol
+            its[precision] = s.iterations
+            s.close()
+        out = to_host(psi)
+        true_res = np.linalg.norm(q - O.apply_op(go, beta, out)) / np.linalg.norm(q)
+        print("  tol", tol, "iterations", its, "true residual", true_res, flush=True)
+        assert true_res < 2 * tol
+        assert rel(mean_free(out), mean_free(ref)) < 1e3 * tol
+        assert its["mixed-bf16"] <= its["mixed"] + 2
+
+
+def test_preconditioner_bf16_storage_close_to_fp32(torch_gpu, monkeypatch):
+    """One preconditioner application with the finest level's q and beta in
+    bf16 (unit roundoff u = 2^-9) reduces the FP64 residual ||q - A z|| / ||q||
+    as well as the FP32-storage one, up to what the rounding of q and beta
+    itself contributes (a few u), and differs from it (the storage is in use)."""
+    monkeypatch.setenv("TBFG_SWEEP_MIN_CELLS", "0")
+    n = (64, 48, 64)
+    g, go, rho, _, q, beta = bubble_case(n, 12, 1, 13)
+    assert bf16_engaged(g, nLevels=4)
+    zs, res = {}, {}
+    for precision in ("mixed", "mixed-bf16"):
+        s = T.PCG(g, nLevels=4, precision=precision)
+        z = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.precondition(to_dev(beta, torch_gpu), to_dev(q, torch_gpu), z)
+        zs[precision] = to_host(z)
+        res[precision] = np.linalg.norm(q - O.apply_op(go, beta, zs[precision])) / np.linalg.norm(q)
+        s.close()
+    d = rel(zs["mixed-bf16"], zs["mixed"])
+    print("  bf16 vs fp32 storage: z differs by", d, "residuals", res, flush=True)
+    assert d > 0
+    assert res["mixed"] < 0.5
+    assert abs(res["mixed-bf16"] - res["mixed"]) <= 0.05 * res["mixed"] + 8 * 2.0**-9
+
+
 def test_host_array_entry_point_with_halo(torch_gpu):
     print("case pcg host arrays", flush=True)
     n = (32, 32, 32)
@@ -378,7 +441,8 @@ def test_config1_bubbles_128(torch_gpu):
     ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=4, tol=1e-12, max_iter=200, coarse_mode=0, flexible=1),
                                    beta, q)
     assert rc == 0
-    for precision, tol, bound in (("fp64", 1e-12, 1e-9), ("mixed", 1e-8, 1e-6)):
+    assert bf16_engaged(g, nLevels=4)
+    for precision, tol, bound in (("fp64", 1e-12, 1e-9), ("mixed", 1e-8, 1e-6), ("mixed-bf16", 1e-8, 1e-6)):
         s = T.PCG(g, nLevels=4, tolPCG=tol, maxIterPCG=200, precision=precision, flexible=1)
         psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
         s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
@@ -480,7 +544,7 @@ def test_config2_full_size_properties(torch_gpu):
     beta = torch_gpu.zeros_like(drho)
     F.computeBeta(g, 1e-3, drho, beta)
     psis = {}
-    for precision in ("mixed", "fp64"):
+    for precision in ("mixed", "mixed-bf16", "fp64"):
         s = T.PCG(g, nLevels=4, tolPCG=1e-8, maxIterPCG=200, precision=precision, flexible=1, coarse_max_iter=0)
         psi = torch_gpu.zeros_like(q)
         s.solvePCG(psi, beta, q)
@@ -491,13 +555,14 @@ def test_config2_full_size_properties(torch_gpu):
         true_res = float(torch_gpu.linalg.norm(q - aq) / torch_gpu.linalg.norm(q))
         print("  ", precision, "iterations", s.iterations, "recursive", s.res_, "true", true_res, flush=True)
         assert true_res <= 1.05e-8
-        if precision == "mixed":
+        if precision != "fp64":
             again = torch_gpu.zeros_like(q)
             s.solvePCG(again, beta, q)
             assert torch_gpu.equal(again, psi)
         psis[precision] = psi - psi.mean()
         s.close()
-    assert float(torch_gpu.linalg.norm(psis["mixed"] - psis["fp64"]) / torch_gpu.linalg.norm(psis["fp64"])) < 1e-6
+    for precision in ("mixed", "mixed-bf16"):
+        assert float(torch_gpu.linalg.norm(psis[precision] - psis["fp64"]) / torch_gpu.linalg.norm(psis["fp64"])) < 1e-6
     F.makeVelocityDivFree(g, 1e-3, drho, psis["mixed"], ux, uy, uz)
     div0 = float(torch_gpu.linalg.norm(q))
     d = torch_gpu.zeros_like(q)
@@ -644,10 +709,11 @@ def test_mg_tolerance_loop_matches_oracle(torch_gpu, mg_tol):
         s.close()
 
 
-def test_mg_tolerance_loop_cap_and_fixed_cycles(torch_gpu):
+def test_mg_tolerance_loop_cap_and_fixed_cycles(torch_gpu, monkeypatch):
     """maxIterMG caps the loop: a target no residual reaches with a cap of two
     equals two fixed cycles per application; the fixed form reports its cycle
-    count; mixed precision converges under the reference's tolMG 0.4."""
+    count; mixed precision (FP32 and bf16 finest-level storage) converges under
+    the reference's tolMG 0.4."""
     n = (64, 64, 64)
     print("case mg tolerance cap", n, flush=True)
     g, go, rho, _, q, beta = bubble_case(n, 18, 0, 23)
@@ -663,12 +729,15 @@ def test_mg_tolerance_loop_cap_and_fixed_cycles(torch_gpu):
     assert i1 == i2 and st1 == (i1 * 2, 2) and st2 == st1
     # the same cycles; the dot products summed in another order (separate pass)
     assert rel(mean_free(p2), mean_free(p1)) < 1e-9
-    s = T.PCG(g, nLevels=4, tolPCG=1e-8, maxIterPCG=200, precision="mixed", tolMG=0.4, maxIterMG=100)
-    psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
-    s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
-    total, most = s.mg_stats()
-    print("  mixed tolMG 0.4:", s.iterations, (total, most), flush=True)
-    assert s.converged and s.iterations <= total <= s.iterations * most
-    true_res = np.linalg.norm(q - O.apply_op(go, beta, to_host(psi))) / np.linalg.norm(q)
-    assert true_res < 2e-8
-    s.close()
+    monkeypatch.setenv("TBFG_SWEEP_MIN_CELLS", "0")  # the plane march, so that bf16 storage is used
+    assert bf16_engaged(g, nLevels=4, tolMG=0.4)
+    for precision in ("mixed", "mixed-bf16"):
+        s = T.PCG(g, nLevels=4, tolPCG=1e-8, maxIterPCG=200, precision=precision, tolMG=0.4, maxIterMG=100)
+        psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+        s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+        total, most = s.mg_stats()
+        print("  %s tolMG 0.4:" % precision, s.iterations, (total, most), flush=True)
+        assert s.converged and s.iterations <= total <= s.iterations * most
+        true_res = np.linalg.norm(q - O.apply_op(go, beta, to_host(psi))) / np.linalg.norm(q)
+        assert true_res < 2e-8
+        s.close()
