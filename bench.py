@@ -267,6 +267,8 @@ def main():
                   flexible=1, coarse_max_iter=0)
     psi = torch.zeros_like(drho)
     total, ms, iters, prof = timed_solves(torch, mixed, psi, beta, q, args.steps, args.warmup, profile="sweep_fine")
+    # the operator pass (the solve's largest kernel), timed alone in a few more solves
+    prof_op = timed_solves(torch, mixed, torch.zeros_like(drho), beta, q, 5, 0, profile="spmv")[3]
     psi_mixed = psi.clone()
     value = args.steps / total
     log("mixed: %.2f solves/s, median %.3f ms, iterations %s" % (value, np.median(ms), sorted(set(iters))))
@@ -427,29 +429,40 @@ def main():
         except Exception as e:  # reported, not fatal
             out["small_configs"] = {"error": str(e)[:200]}
 
-    # roofline of the dominant kernel (finest-level red/black sweep)
+    # roofline of the dominant kernel: the operator pass k_op_march (direction,
+    # A d and the deferred pressure update, ~21 % of the solve); the finest
+    # level's plain red/black sweep beside it
+    def traffic_of(name):
+        try:
+            return json.load(open(os.path.join(ROOT, "profiles", name))).get("hbm_bytes_per_launch")
+        except Exception:
+            return None
+
     launches, kms, bytes_per = prof
-    if launches > 0:
+    launches_op, kms_op, bytes_op = prof_op
+    if launches > 0 and launches_op > 0:
+        copy_gbs = copy_peak(torch)
+        avg_op = kms_op / launches_op * 1e-3
+        ach_op = bytes_op / avg_op / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": round(ach_op, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(ach_op / HBM_PEAK_GBS, 4), "traffic": traffic_of("op_march_traffic.json"),
+                           "kernel": "k_op_march<128, 512, 4, float, true>: d = (z - m) + gamma d_old, qv = A d, "
+                                     "d.qv and the deferred psi += alpha d_old on the finest level (z FP32 in; "
+                                     "d_old, beta, psi FP64 in; d, qv, psi FP64 out: 52 B per cell)",
+                           "avg_us": round(avg_op * 1e6, 2), "launches": launches_op,
+                           "algorithmic_bytes_per_launch": bytes_op,
+                           "copy_kernel_gbs": round(copy_gbs, 1)}
         avg_s = kms / launches * 1e-3
         achieved = bytes_per / avg_s / 1e9
-        traffic = None
-        tf = os.path.join(ROOT, "profiles", "sweep_fine_traffic.json")
-        if os.path.exists(tf):
-            try:
-                traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
-            except Exception:
-                traffic = None
-        copy_gbs = copy_peak(torch)
-        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                           "kernel": "k_sweep<float, bf16s, bf16s, 32, 16, 256, false, false, 4>: finest-level "
-                                     "red/black sweep from a non-zero start, out of place (p FP32 in and out, q "
-                                     "and beta bf16 in: 12 B per cell)",
-                           "avg_us": round(avg_s * 1e6, 2), "launches": launches,
-                           "algorithmic_bytes_per_launch": bytes_per,
-                           "copy_kernel_gbs": round(copy_gbs, 1)}
-        log("sweep fine: %d launches, avg %.1f us, %.0f GB/s algorithmic (device copy: %.0f GB/s)"
-            % (launches, avg_s * 1e6, achieved, copy_gbs))
+        out["roofline_sweep_fine"] = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of("sweep_fine_traffic.json"),
+            "kernel": "k_sweep<float, bf16s, bf16s, 32, 16, 256, false, false, 4>: finest-level red/black sweep "
+                      "from a non-zero start, out of place (p FP32 in and out, q and beta bf16 in: 12 B per cell)",
+            "avg_us": round(avg_s * 1e6, 2), "launches": launches, "algorithmic_bytes_per_launch": bytes_per}
+        log("operator pass: %d launches, avg %.1f us, %.0f GB/s algorithmic; sweep fine: %d launches, avg %.1f us, "
+            "%.0f GB/s (device copy: %.0f GB/s)"
+            % (launches_op, avg_op * 1e6, ach_op, launches, avg_s * 1e6, achieved, copy_gbs))
         # the whole solve against the bytes a PCG iteration must stream: CG vectors
         # 128 N plus the multigrid, per level N_l (16 B per sweep, 4 sweeps; 12.5
         # B residual-restriction, 8.5 B prolongation, 4 + 4 B for r's copy and z)

@@ -174,7 +174,11 @@ int64_t tbfg_pcg_footprint(const tbfg_pcg* h);
  *   2 operator apply + d.Ad (PCG)                 3 psi/r update + ||r|| (PCG)
  * tbfg_pcg_profile_end returns the launches seen since _begin, their summed
  * duration and the algorithmic bytes one launch must move (each input value
- * read once, each output written once). */
+ * read once, each output written once).  Launches queued ahead of a solve's
+ * stop test that found it already stopped (shorter than a quarter of that
+ * solve's median launch) are not counted.  Families 0 and 2 (plane-marching
+ * forms) are timed by events the dispatch itself records, so the interval
+ * holds that kernel alone. */
 #define TBFG_KERNEL_SWEEP_FINE 0
 #define TBFG_KERNEL_RESTRICT_FINE 1
 #define TBFG_KERNEL_SPMV 2

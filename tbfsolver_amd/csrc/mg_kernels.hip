@@ -9,6 +9,8 @@
 #include "smooth.hpp"
 #include "tune.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -1758,13 +1760,21 @@ int launch_op_dot(const LevelDev<double>& L, const OpPlan& m, const double* beta
 
 template <class TZ>
 int launch_op_dir(const LevelDev<double>& L, const OpPlan& m, const double* beta, const TZ* z, const double* dold,
-                  double* dnew, double* psi, int init, double* qv, PcgState* st, const Reducer& red, void* stream)
+                  double* dnew, double* psi, int init, double* qv, PcgState* st, const Reducer& red, void* stream,
+                  const KernelEvents* kev)
 {
     const OpDir<TZ> od{z, dold, dnew, psi, init};
+    // (kev: events recorded by the dispatch of the pass itself, for timing it alone)
 #define TBFG_OPD(TX_, NT_)                                                                                         \
-    if (m.tx == TX_ && m.nt == NT_)                                                                                \
-        hipLaunchKernelGGL((k_op_march<TX_, NT_, 4, TZ, true>), dim3(m.nblocks), dim3(NT_), 0, S(stream), L, beta, \
-                           nullptr, qv, st, red, m.kc, m.tiles_x, m.ntiles, od);
+    if (m.tx == TX_ && m.nt == NT_) {                                                                              \
+        if (kev)                                                                                                   \
+            hipExtLaunchKernelGGL((k_op_march<TX_, NT_, 4, TZ, true>), dim3(m.nblocks), dim3(NT_), 0, S(stream),  \
+                                  kev->start, kev->stop, 0, L, beta, nullptr, qv, st, red, m.kc, m.tiles_x,        \
+                                  m.ntiles, od);                                                                   \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_op_march<TX_, NT_, 4, TZ, true>), dim3(m.nblocks), dim3(NT_), 0, S(stream), L,  \
+                               beta, nullptr, qv, st, red, m.kc, m.tiles_x, m.ntiles, od);                        \
+    }
     TBFG_OPD(128, 512) else TBFG_OPD(64, 512) else TBFG_OPD(32, 256) else TBFG_OPD(64, 256) else {
         set_error("launch_op_dir: no operator tile %dx%d", m.tx, m.nt);
         return TBFG_EINVAL;
@@ -1775,9 +1785,11 @@ int launch_op_dir(const LevelDev<double>& L, const OpPlan& m, const double* beta
     return TBFG_OK;
 }
 template int launch_op_dir<float>(const LevelDev<double>&, const OpPlan&, const double*, const float*, const double*,
-                                  double*, double*, int, double*, PcgState*, const Reducer&, void*);
+                                  double*, double*, int, double*, PcgState*, const Reducer&, void*,
+                                  const KernelEvents*);
 template int launch_op_dir<double>(const LevelDev<double>&, const OpPlan&, const double*, const double*,
-                                   const double*, double*, double*, int, double*, PcgState*, const Reducer&, void*);
+                                   const double*, double*, double*, int, double*, PcgState*, const Reducer&, void*,
+                                   const KernelEvents*);
 
 int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, const PcgState* st, void* stream)
 {

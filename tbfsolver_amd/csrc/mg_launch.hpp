@@ -21,9 +21,11 @@ int launch_op_dot(const LevelDev<double>& L, const OpPlan& plan, const double* b
                   PcgState* st, const Reducer& red, void* stream);
 // the direction d = (z - m) + gamma d_old into dnew, qv = A d, d.qv and psi +=
 // alpha_prev d_old in one pass (plane-marching tiles: plan.tx != 0)
+struct KernelEvents;
 template <class TZ>
 int launch_op_dir(const LevelDev<double>& L, const OpPlan& plan, const double* beta, const TZ* z, const double* dold,
-                  double* dnew, double* psi, int init, double* qv, PcgState* st, const Reducer& red, void* stream);
+                  double* dnew, double* psi, int init, double* qv, PcgState* st, const Reducer& red, void* stream,
+                  const KernelEvents* kev = nullptr);
 // psi += alpha d of the last iteration that ran (d in d0 or d1 by its parity)
 int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, const PcgState* st, void* stream);
 // r32 (optional): the multigrid's copy of r, TR = float or bf16s

@@ -84,17 +84,28 @@ struct ProfScope {
     }
 };
 
-// adds up the recorded pairs; the stream must have been synchronised
+// adds up the recorded pairs (the launches of one solve); the stream must have
+// been synchronised.  Launches queued after the solve's stop test return at
+// once (their kernels read the stop flag first): a launch shorter than a
+// quarter of the solve's median is one of them and is not counted.
 static void prof_harvest(tbfg_pcg* h)
 {
+    std::vector<float> ms;
+    ms.reserve(h->prof_used);
     for (int i = 0; i < h->prof_used; ++i) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, h->prof_ev[2 * i], h->prof_ev[2 * i + 1]) == hipSuccess) {
-            h->prof_ms += ms;
-            h->prof_launches++;
-        }
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, h->prof_ev[2 * i], h->prof_ev[2 * i + 1]) == hipSuccess) ms.push_back(t);
     }
     h->prof_used = 0;
+    if (ms.empty()) return;
+    std::vector<float> sorted(ms);
+    std::nth_element(sorted.begin(), sorted.begin() + sorted.size() / 2, sorted.end());
+    const float cut = 0.25f * sorted[sorted.size() / 2];
+    for (float t : ms)
+        if (t >= cut) {
+            h->prof_ms += t;
+            h->prof_launches++;
+        }
 }
 
 int pcg_profile_begin(tbfg_pcg* h, int kernel)
@@ -775,9 +786,9 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
         if (h->fuse_op) {
             double* dj = dbuf[j & 1];
             {
-                ProfScope ps(h, 2, stream);
+                ProfScope ps(h, 2, stream, true);  // the operator pass alone (not its finalize)
                 TBFG_TRY(launch_op_dir<TZ>(L0, h->opp, beta, z, dbuf[(j - 1) & 1], dj, psi, init, qv, st, red,
-                                           stream));
+                                           stream, ps.events()));
             }
             ProfScope ps(h, 3, stream);
             TBFG_TRY(launch_pcg_update(n, nullptr, r, r32, dj, qv, st, red, stream));
