@@ -235,6 +235,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
     ap.add_argument("--no-host", action="store_true", help="skip the host-array (PCIe-inclusive) timing")
+    ap.add_argument("--no-fp32-storage", action="store_true", help="skip the solve with FP32 finest-level storage")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the pressure solve of one channel instance runs on a single "
@@ -295,21 +296,22 @@ def main():
     }
 
     # the same solve with the finest level stored in FP32
-    m32 = T.PCG(g, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=200, precision="mixed",
-                flexible=1, coarse_max_iter=0)
-    tot32, ms32, it32, prof32 = timed_solves(torch, m32, torch.zeros_like(drho), beta, q, args.steps, args.warmup,
-                                             profile="sweep_fine")
-    out["mixed_fp32_storage"] = {"solves_per_s": round(args.steps / tot32, 3),
-                                 "p50_ms": round(float(np.percentile(ms32, 50)), 4),
-                                 "iterations": int(np.median(it32))}
-    if prof32[0] > 0:
-        us32 = prof32[1] / prof32[0] * 1e3
-        out["mixed_fp32_storage"]["sweep_fine"] = {
-            "kernel": "k_sweep<float, float, float, 32, 16, 256, false, false, 4> (16 B per cell)",
-            "avg_us": round(us32, 2), "achieved_gbs": round(prof32[2] / us32 / 1e3, 1),
-            "frac": round(prof32[2] / us32 / 1e3 / HBM_PEAK_GBS, 4)}
-    log("mixed, FP32 storage: %.2f solves/s, iterations %s" % (args.steps / tot32, sorted(set(it32))))
-    m32.close()
+    if not args.no_fp32_storage:
+        m32 = T.PCG(g, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=200, precision="mixed",
+                    flexible=1, coarse_max_iter=0)
+        tot32, ms32, it32, prof32 = timed_solves(torch, m32, torch.zeros_like(drho), beta, q, args.steps, args.warmup,
+                                                 profile="sweep_fine")
+        out["mixed_fp32_storage"] = {"solves_per_s": round(args.steps / tot32, 3),
+                                     "p50_ms": round(float(np.percentile(ms32, 50)), 4),
+                                     "iterations": int(np.median(it32))}
+        if prof32[0] > 0:
+            us32 = prof32[1] / prof32[0] * 1e3
+            out["mixed_fp32_storage"]["sweep_fine"] = {
+                "kernel": "k_sweep<float, float, float, 32, 16, 256, false, false, 4> (16 B per cell)",
+                "avg_us": round(us32, 2), "achieved_gbs": round(prof32[2] / us32 / 1e3, 1),
+                "frac": round(prof32[2] / us32 / 1e3 / HBM_PEAK_GBS, 4)}
+        log("mixed, FP32 storage: %.2f solves/s, iterations %s" % (args.steps / tot32, sorted(set(it32))))
+        m32.close()
 
     # the same solve through the host-array entry point, as the Fortran caller
     # makes it: psi, beta, q in host memory with a one-cell halo (Fortran
