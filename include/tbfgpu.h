@@ -73,7 +73,7 @@ int tbfg_device_count(void);
 typedef struct tbfg_fps tbfg_fps;
 
 typedef struct tbfg_fps_opts {
-    int32_t precision;     /* TBFG_PREC_FP64 or TBFG_PREC_MIXED                       */
+    int32_t precision;     /* TBFG_PREC_FP64 or TBFG_PREC_MIXED (FFT solver)          */
     int32_t refine_sweeps; /* MIXED only: FP64 iterative-refinement sweeps (default 2) */
 } tbfg_fps_opts;
 

@@ -135,3 +135,24 @@ def test_fft_pass_division_by_multiply_high():
             continue  # the passes use the identity for d = 1
         magic = np.uint64(((1 << 32) + d - 1) // d)
         assert np.array_equal((x * magic) >> np.uint64(32), x // np.uint64(d)), d
+
+
+def test_precision_codes_match_header_and_are_validated():
+    """The precision codes of the header and of the ctypes binding agree; the
+    Python mirror rejects names it does not know, the C ABI codes it does not
+    know (before any device call)."""
+    txt = open(os.path.join(ROOT, "include", "tbfgpu.h")).read()
+    defs = {k: int(v) for k, v in re.findall(r"#define (TBFG_PREC_[A-Z0-9_]+) (\d+)", txt)}
+    assert defs == {"TBFG_PREC_FP64": _lib.PREC_FP64, "TBFG_PREC_MIXED": _lib.PREC_MIXED,
+                    "TBFG_PREC_MIXED_BF16": _lib.PREC_MIXED_BF16}
+    g = T.channel_grid((64, 64, 64))
+    for bad in ("mixed16", "bf16", "FP64"):
+        with pytest.raises(ValueError):
+            T.PCG(g, precision=bad)
+    with pytest.raises(ValueError):
+        T.FastPoissonSolver(g, precision="mixed-bf16")   # the FFT solver has no bf16 form
+    p = T.default_params()
+    p.precision = 7
+    h = C.c_void_p()
+    assert T.lib().tbfg_pcg_create(C.byref(g), C.byref(p), C.byref(h)) == _lib.TBFG_EINVAL
+    assert b"precision" in T.lib().tbfg_last_error()
