@@ -133,7 +133,8 @@ def small_configs(torch):
 
 def batch_leg(torch, reps=3, workers=8):
     """64 uneven instances (seed 1003; FFT with constant density on even indices,
-    PCG + multigrid with 4..24 bubbles on odd ones), mixed precision, each solved
+    PCG + multigrid with 4..24 bubbles on odd ones; the multigrid in mixed
+    precision, the FFT in FP64, its faster form on MI355X), each solved
     once per pass; passes timed with 8 host threads on 8 streams and with one."""
     from tbfsolver_amd import batch as Bt
     from tbfsolver_amd import tune as Tu
@@ -168,7 +169,8 @@ def batch_leg(torch, reps=3, workers=8):
             "error_sample": "every third instance", "setup_s": round(setup, 1),
             "tile_shapes": "tuned per size (%d sizes, tbfsolver_amd/tuning/mi355x_config3_mixed.json)" % tuned
             if tuned else "built-in order",
-            "precision": "mixed (FP32 FFT + FP64 refinement; FP32 multigrid in FP64 flexible PCG to 1e-8)"}
+            "precision": "FAST instances FP64 FFT (faster than its FP32 form here); MG instances FP32 multigrid "
+                         "in FP64 flexible PCG to 1e-8"}
 
 
 def serving_leg(torch, rates=(50, 200, 500, 1000, 2000), workers=8, fp64_rates=(50, 200, 500, 1000)):

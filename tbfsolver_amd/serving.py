@@ -73,8 +73,8 @@ def estimate_footprint(n, kind, precision):
     the accounting then uses the measured value)."""
     cells = n[0] * n[1] * n[2]
     fields = 8 * (5 + 9 + 3) * cells  # psi, p, rho, s, beta; u*, u base, u perturbation; old levels
-    per_cell = {("MG", "mixed"): 56, ("MG", "fp64"): 80, ("FAST", "mixed"): 40, ("FAST", "fp64"): 56}
-    return fields + per_cell[(kind, precision)] * cells + (8 << 20)
+    per_cell = {("MG", "mixed"): 56, ("MG", "fp64"): 80, ("FAST", "fp64"): 56}
+    return fields + per_cell[(kind, "fp64" if kind == "FAST" else precision)] * cells + (8 << 20)
 
 
 class HbmAdmission:
@@ -155,7 +155,9 @@ class Resident:
                                   maxIterPCG=200, flexible=1, coarse_max_iter=0)
         else:
             self.rho = torch.ones(self.n[2], self.n[1], self.n[0], dtype=torch.float64, device="cuda")
-            self.eqn = PoissonEqn(self.grid, "FAST", 1.0, 1.0, precision=precision)
+            # the FFT solve in FP64 in both modes: faster on MI355X than its FP32 form
+            # with FP64 refinement, and exact
+            self.eqn = PoissonEqn(self.grid, "FAST", 1.0, 1.0, precision="fp64")
         self.psi = torch.zeros_like(self.rho)
         self.p = torch.zeros_like(self.rho)
         self.steps = 0
