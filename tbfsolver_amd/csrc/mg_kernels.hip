@@ -1686,7 +1686,9 @@ OpPlan op_plan(const LevelDev<double>& L)
 {
     OpPlan m;
     const char* env = std::getenv("TBFG_OP_TILE");
-    std::vector<std::pair<int, int>> cand = {{128, 512}, {64, 512}, {32, 256}};
+    // 256-thread tiles first: with long marches they measured fastest (256x192x256:
+    // 64x256 tiles 136 us, 32x256 138 us, 128x512 152 us, all with 64-plane chunks)
+    std::vector<std::pair<int, int>> cand = {{64, 256}, {32, 256}, {128, 512}, {64, 512}};
     int tuned[3];
     if (env) {
         int a = 0, b = 0;
@@ -1709,8 +1711,10 @@ OpPlan op_plan(const LevelDev<double>& L)
     if (!m.tx) return m;
     m.tiles_x = L.nx / m.tx;
     m.ntiles = m.tiles_x * (L.ny / (m.nt / m.tx));
-    // about 6 x 256 threads resident per CU, chunks a multiple of 4 planes
-    const int chunks = std::max(1, (1536 * 256 / m.nt + m.ntiles - 1) / m.ntiles);
+    // about 3 x 256 threads per CU, chunks a multiple of 4 planes: fewer, longer
+    // marches re-read fewer halo planes (256x192x256, 64x256 tiles: 64-plane
+    // chunks 136 us, 32-plane chunks 149 us)
+    const int chunks = std::max(1, (768 * 256 / m.nt + m.ntiles - 1) / m.ntiles);
     int kc = (L.nz + chunks - 1) / chunks;
     kc = std::min(kOpMaxChunk, std::max(4, (kc + 3) / 4 * 4));
     if (const char* e = std::getenv("TBFG_OP_KC")) kc = std::min(kOpMaxChunk, std::max(4, std::atoi(e) / 4 * 4));
