@@ -450,7 +450,7 @@ def main():
         ach_op = bytes_op / avg_op / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": round(ach_op, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(ach_op / HBM_PEAK_GBS, 4), "traffic": traffic_of("op_march_traffic.json"),
-                           "kernel": "k_op_march<128, 512, 4, float, true>: d = (z - m) + gamma d_old, qv = A d, "
+                           "kernel": "k_op_march<64, 256, 4, float, true> (the plan's tile at this size): d = (z - m) + gamma d_old, qv = A d, "
                                      "d.qv and the deferred psi += alpha d_old on the finest level (z FP32 in; "
                                      "d_old, beta, psi FP64 in; d, qv, psi FP64 out: 52 B per cell)",
                            "avg_us": round(avg_op * 1e6, 2), "launches": launches_op,
