@@ -220,20 +220,37 @@ int tbfg_beta_dev(const tbfg_grid_desc* grid, double alpha_dt, const double* rho
 int tbfg_project_mg_dev(const tbfg_grid_desc* grid, double alpha_dt, const double* rho, const double* psi,
                         double* ux, double* uy, double* uz, void* stream);
 
-/* FAST_MODE computeSource + computeOldPressDiv (poissonEqn.f90:193-305), nl = 2,
- * no surface tension: s = rho0/(alpha dt) div u* + div[(1 - rho0 (1/rho)_f) G_f]
- * with G_f the face gradient of 2 psi1 - psi2 (psi1 = psi^n, psi2 = psi^(n-1);
- * either may be NULL on the first steps, meaning zero) and (1/rho)_f the face
- * mean of 1/rho.  rho0 = min(rho_l, rho_g) (poissonEqn.f90:127).  x, z uniform. */
+/* FAST_MODE computeSource + computeOldPressDiv (poissonEqn.f90:193-305), nl = 2:
+ * s = rho0/(alpha dt) div u* + div[(1 - rho0 (1/rho)_f) G_f] with the face value
+ * G_f = d_f(2 psi1 - psi2) - 2 st1_f + st2_f (poissonEqn.f90:255-301), psi1 =
+ * psi^n, psi2 = psi^(n-1) (psi%ptrf_, psi%ptrf_%ptrf_), and st1 = st^n, st2 =
+ * st^(n-1) the old surface-tension levels (st%ptrf_, st%ptrf_%ptrf_), and (1/rho)_f
+ * the face mean of 1/rho.  st1 and st2 point to three device arrays each (x, y, z
+ * components, laid out like ux, uy, uz; the y wall faces are read as the reference
+ * reads them, and updatePressure keeps them zero).  Any level, or component, may
+ * be NULL, meaning zero (the first steps; no surface tension).
+ * rho0 = min(rho_l, rho_g) (poissonEqn.f90:127).  x, z uniform. */
 int tbfg_source_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rho0, const double* rho,
                          const double* ux, const double* uy, const double* uz, const double* psi1,
-                         const double* psi2, double* s, void* stream);
+                         const double* psi2, const double* const st1[3], const double* const st2[3], double* s,
+                         void* stream);
 
 /* FAST_MODE makeVelocityDivFree + computeOldPressGrad (momentumEqn.f90:999-1127):
- * u_f -= alpha dt [(1/rho0) d psi + ((1/rho)_f - 1/rho0) G_f], G_f as above. */
+ * u_f -= alpha dt [(1/rho0) d psi + ((1/rho)_f - 1/rho0) G_f], G_f as above
+ * (momentumEqn.f90:1104-1121), on the periodic x/z faces and interior y faces. */
 int tbfg_project_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rho0, const double* rho,
-                          const double* psi, const double* psi1, const double* psi2, double* ux, double* uy,
-                          double* uz, void* stream);
+                          const double* psi, const double* psi1, const double* psi2, const double* const st1[3],
+                          const double* const st2[3], double* ux, double* uy, double* uz, void* stream);
+
+/* updatePressure (poissonEqn.f90:374-415) with clean_st_prev_bc (:419-459), one
+ * kernel: p = psi (p may be NULL); storeOldField: psi2 <- psi1 <- psi;
+ * storeOldFieldV: st2 <- st1 <- st (st NULL = zero surface tension); then the old
+ * st levels are zeroed on the Neumann walls, i.e. the y faces j = 0 and j = ny
+ * (x and z are periodic).  st1 and st2 both NULL skip the surface-tension levels
+ * (st must then be NULL too).  The reference's restart copies p0 and st0 are
+ * psi2 and st2 after the call. */
+int tbfg_update_pressure_dev(const tbfg_grid_desc* grid, const double* psi, double* p, double* psi1, double* psi2,
+                             const double* const st[3], double* const st1[3], double* const st2[3], void* stream);
 
 /* computeContinuityError (auxiliaryRoutines.f90:42-91): returns max|div u| (not
  * multiplied by dt) in *out.  Blocking. */

@@ -202,20 +202,49 @@ def project_mg(g, alpha_dt, rho, psi, ux, uy, uz):
     return ux, uy, uz
 
 
-def project_fast(g, alpha_dt, rho0, rho, psi, psi1, psi2, ux, uy, uz):
+def _levels(st):
+    """A surface-tension level (ux-, uy-, uz-shaped arrays) as a C array of three
+    pointers and the arrays it points into; None is a NULL level (zero)."""
+    if st is None:
+        return None, ()
+    arrs = tuple(_f(a) for a in st)
+    return (C.POINTER(C.c_double) * 3)(*(_p(a) for a in arrs)), arrs
+
+
+def project_fast(g, alpha_dt, rho0, rho, psi, psi1, psi2, ux, uy, uz, st1=None, st2=None):
     rho, psi, psi1, psi2 = _f(rho), _f(psi), _f(psi1), _f(psi2)
     ux, uy, uz = (_f(a).copy(order="F") for a in (ux, uy, uz))
+    l1, keep1 = _levels(st1)
+    l2, keep2 = _levels(st2)
     lib().or_project_fast(C.byref(g), C.c_double(alpha_dt), C.c_double(rho0), _p(rho), _p(psi),
-                          _p(psi1), _p(psi2), _p(ux), _p(uy), _p(uz))
+                          _p(psi1), _p(psi2), l1, l2, _p(ux), _p(uy), _p(uz))
     return ux, uy, uz
 
 
-def source_fast(g, alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2):
+def source_fast(g, alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2, st1=None, st2=None):
     rho, ux, uy, uz, psi1, psi2 = (_f(a) for a in (rho, ux, uy, uz, psi1, psi2))
     s = np.zeros(shape(g), order="F")
+    l1, keep1 = _levels(st1)
+    l2, keep2 = _levels(st2)
     lib().or_source_fast(C.byref(g), C.c_double(alpha_dt), C.c_double(rho0), _p(rho), _p(ux),
-                         _p(uy), _p(uz), _p(psi1), _p(psi2), _p(s))
+                         _p(uy), _p(uz), _p(psi1), _p(psi2), l1, l2, _p(s))
     return s
+
+
+def update_pressure(g, psi, psi1, psi2, st=None, st1=None, st2=None):
+    """updatePressure with clean_st_prev_bc: returns (p, psi1, psi2, st1, st2),
+    new arrays (st1, st2 None when they were)."""
+    psi = _f(psi)
+    p = np.zeros_like(psi, order="F")
+    psi1, psi2 = _f(psi1).copy(order="F"), _f(psi2).copy(order="F")
+    if st1 is not None:
+        st1 = tuple(_f(a).copy(order="F") for a in st1)
+        st2 = tuple(_f(a).copy(order="F") for a in st2)
+    lc, keepc = _levels(st)
+    l1, _ = _levels(st1)
+    l2, _ = _levels(st2)
+    lib().or_update_pressure(C.byref(g), _p(psi), _p(p), _p(psi1), _p(psi2), lc, l1, l2)
+    return p, psi1, psi2, st1, st2
 
 
 def material_props(g, c, rho_l=1.0, rho_g=1.0 / 20.0, mu_l=1.0, mu_g=1.0 / 50.0):

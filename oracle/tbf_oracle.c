@@ -938,45 +938,64 @@ void or_project_mg(const or_grid_desc* g, double alpha_dt, const double* rho, co
     lev_free(&L);
 }
 
-/* momentumEqn.f90:999-1127 (FAST makeVelocityDivFree + computeOldPressGrad) without
- * surface-tension terms: u -= a*dt*[(1/rho0) d(psi) + ((1/rho)_f - 1/rho0) d(2 psi1 - psi2)]. */
+/* Old surface-tension levels of the FAST splitting: st1 = st^n (st%ptrf_), st2 =
+ * st^(n-1) (st%ptrf_%ptrf_), each three staggered components laid out like the
+ * velocity (x, z: face i / k after cell i / k; y: faces 0..ny).  A NULL level is
+ * zero. */
+#define STC(st, c, idx) ((st)[c] ? (st)[c][idx] : 0.0)
+
+/* momentumEqn.f90:999-1127 (FAST makeVelocityDivFree + computeOldPressGrad, nl = 2):
+ * u_f -= a dt [(1/rho0) d_f psi + ((1/rho)_f - 1/rho0) G_f] with
+ * G_f = d_f(2 psi1 - psi2) - 2 st1_f + st2_f (momentumEqn.f90:1104-1121), on all
+ * periodic x/z faces and the interior y faces. */
 void or_project_fast(const or_grid_desc* g, double alpha_dt, double rho0, const double* rho, const double* psi,
-                     const double* psi1, const double* psi2, double* ux, double* uy, double* uz)
+                     const double* psi1, const double* psi2, const double* const st1[3],
+                     const double* const st2[3], double* ux, double* uy, double* uz)
 {
     lev_t L;
     lev_build(&L, g, 0);
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     const double dxc = L.ax[0].dc[1], dzc = L.ax[2].dc[1];
     const double inv0 = 1.0 / rho0;
+    const double* s1[3] = {st1 ? st1[0] : 0, st1 ? st1[1] : 0, st1 ? st1[2] : 0};
+    const double* s2[3] = {st2 ? st2[0] : 0, st2 ? st2[1] : 0, st2 ? st2[2] : 0};
 #define PH(a, i, j, k) ((a) ? (a)[IDX(i, j, k)] : 0.0)
 #pragma omp parallel for schedule(static)
     for (int k = 1; k <= nz; ++k)
         for (int j = 1; j <= ny; ++j)
             for (int i = 1; i <= nx; ++i) {
                 const int E = wrapi(i + 1, nx), F = wrapi(k + 1, nz);
-                double inv = 0.5 * (1.0 / rho[IDX(E, j, k)] + 1.0 / rho[IDX(i, j, k)]);
-                double dpr = (psi[IDX(E, j, k)] - psi[IDX(i, j, k)]) / dxc;
+                const size_t c = IDX(i, j, k);
+                double inv = 0.5 * (1.0 / rho[IDX(E, j, k)] + 1.0 / rho[c]);
+                double dpr = (psi[IDX(E, j, k)] - psi[c]) / dxc;
                 double dpr0 = (2.0 * (PH(psi1, E, j, k) - PH(psi1, i, j, k)) - (PH(psi2, E, j, k) - PH(psi2, i, j, k))) / dxc;
-                ux[IDX(i, j, k)] = ux[IDX(i, j, k)] - alpha_dt * (inv0 * dpr + (inv - inv0) * dpr0);
-                inv = 0.5 * (1.0 / rho[IDX(i, j, F)] + 1.0 / rho[IDX(i, j, k)]);
-                dpr = (psi[IDX(i, j, F)] - psi[IDX(i, j, k)]) / dzc;
+                dpr0 = dpr0 - 2.0 * STC(s1, 0, c) + STC(s2, 0, c);
+                ux[c] = ux[c] - alpha_dt * (inv0 * dpr + (inv - inv0) * dpr0);
+                inv = 0.5 * (1.0 / rho[IDX(i, j, F)] + 1.0 / rho[c]);
+                dpr = (psi[IDX(i, j, F)] - psi[c]) / dzc;
                 dpr0 = (2.0 * (PH(psi1, i, j, F) - PH(psi1, i, j, k)) - (PH(psi2, i, j, F) - PH(psi2, i, j, k))) / dzc;
-                uz[IDX(i, j, k)] = uz[IDX(i, j, k)] - alpha_dt * (inv0 * dpr + (inv - inv0) * dpr0);
+                dpr0 = dpr0 - 2.0 * STC(s1, 2, c) + STC(s2, 2, c);
+                uz[c] = uz[c] - alpha_dt * (inv0 * dpr + (inv - inv0) * dpr0);
                 if (j < ny) {
+                    const size_t yf = UYI(i, j, k);
                     const double dyc = L.ax[1].dc[j + 1];
-                    inv = 0.5 * (1.0 / rho[IDX(i, j + 1, k)] + 1.0 / rho[IDX(i, j, k)]);
-                    dpr = (psi[IDX(i, j + 1, k)] - psi[IDX(i, j, k)]) / dyc;
+                    inv = 0.5 * (1.0 / rho[IDX(i, j + 1, k)] + 1.0 / rho[c]);
+                    dpr = (psi[IDX(i, j + 1, k)] - psi[c]) / dyc;
                     dpr0 = (2.0 * (PH(psi1, i, j + 1, k) - PH(psi1, i, j, k)) - (PH(psi2, i, j + 1, k) - PH(psi2, i, j, k))) / dyc;
-                    uy[UYI(i, j, k)] = uy[UYI(i, j, k)] - alpha_dt * (inv0 * dpr + (inv - inv0) * dpr0);
+                    dpr0 = dpr0 - 2.0 * STC(s1, 1, yf) + STC(s2, 1, yf);
+                    uy[yf] = uy[yf] - alpha_dt * (inv0 * dpr + (inv - inv0) * dpr0);
                 }
             }
     lev_free(&L);
 }
 
-/* poissonEqn.f90:193-305 (FAST computeSource + computeOldPressDiv, nl=2) without
- * surface-tension terms. */
+/* poissonEqn.f90:193-305 (FAST computeSource + computeOldPressDiv, nl = 2):
+ * s = rho0/(a dt) div u* + div[(1 - rho0 (1/rho)_f) G_f], G_f as in or_project_fast
+ * (computeOldPressDiv, poissonEqn.f90:255-301: the face after the cell reads the
+ * st values of its own face index, the face before it those of index - 1). */
 void or_source_fast(const or_grid_desc* g, double alpha_dt, double rho0, const double* rho, const double* ux,
-                    const double* uy, const double* uz, const double* psi1, const double* psi2, double* s)
+                    const double* uy, const double* uz, const double* psi1, const double* psi2,
+                    const double* const st1[3], const double* const st2[3], double* s)
 {
     lev_t L;
     lev_build(&L, g, 0);
@@ -984,6 +1003,8 @@ void or_source_fast(const or_grid_desc* g, double alpha_dt, double rho0, const d
     const double dxf = L.ax[0].df[1], dzf = L.ax[2].df[1];
     const double dxc = L.ax[0].dc[1], dzc = L.ax[2].dc[1];
     const double r = 1.0 / alpha_dt;
+    const double* s1[3] = {st1 ? st1[0] : 0, st1 ? st1[1] : 0, st1 ? st1[2] : 0};
+    const double* s2[3] = {st2 ? st2[0] : 0, st2 ? st2[1] : 0, st2 ? st2[2] : 0};
 #pragma omp parallel for schedule(static)
     for (int k = 1; k <= nz; ++k)
         for (int j = 1; j <= ny; ++j)
@@ -991,31 +1012,73 @@ void or_source_fast(const or_grid_desc* g, double alpha_dt, double rho0, const d
                 const int E = wrapi(i + 1, nx), W = wrapi(i - 1, nx);
                 const int N = clampi(j + 1, ny), S = clampi(j - 1, ny);
                 const int F = wrapi(k + 1, nz), B = wrapi(k - 1, nz);
-                double dx = (ux[IDX(i, j, k)] - ux[IDX(W, j, k)]) / dxf;
-                double dy = (uy[UYI(i, j, k)] - uy[UYI(i, j - 1, k)]) / L.ax[1].df[j];
-                double dz = (uz[IDX(i, j, k)] - uz[IDX(i, j, B)]) / dzf;
+                const size_t c = IDX(i, j, k), cw = IDX(W, j, k), cb = IDX(i, j, B);
+                const size_t yn = UYI(i, j, k), ys = UYI(i, j - 1, k);
+                double dx = (ux[c] - ux[cw]) / dxf;
+                double dy = (uy[yn] - uy[ys]) / L.ax[1].df[j];
+                double dz = (uz[c] - uz[cb]) / dzf;
                 double sv = (dx + dy + dz) * r * rho0;
-                const double ic = 1.0 / rho[IDX(i, j, k)];
+                const double ic = 1.0 / rho[c];
                 double apx = 1.0 - rho0 * 0.5 * (1.0 / rho[IDX(E, j, k)] + ic);
-                double amx = 1.0 - rho0 * 0.5 * (1.0 / rho[IDX(W, j, k)] + ic);
+                double amx = 1.0 - rho0 * 0.5 * (1.0 / rho[cw] + ic);
                 double apy = 1.0 - rho0 * 0.5 * (1.0 / rho[IDX(i, N, k)] + ic);
                 double amy = 1.0 - rho0 * 0.5 * (1.0 / rho[IDX(i, S, k)] + ic);
                 double apz = 1.0 - rho0 * 0.5 * (1.0 / rho[IDX(i, j, F)] + ic);
-                double amz = 1.0 - rho0 * 0.5 * (1.0 / rho[IDX(i, j, B)] + ic);
+                double amz = 1.0 - rho0 * 0.5 * (1.0 / rho[cb] + ic);
                 double dpxp = (2.0 * (PH(psi1, E, j, k) - PH(psi1, i, j, k)) - (PH(psi2, E, j, k) - PH(psi2, i, j, k))) / dxc;
+                dpxp = dpxp - 2.0 * STC(s1, 0, c) + STC(s2, 0, c);
                 double dpxm = (2.0 * (PH(psi1, i, j, k) - PH(psi1, W, j, k)) - (PH(psi2, i, j, k) - PH(psi2, W, j, k))) / dxc;
+                dpxm = dpxm - 2.0 * STC(s1, 0, cw) + STC(s2, 0, cw);
                 double dpyp = (2.0 * (PH(psi1, i, N, k) - PH(psi1, i, j, k)) - (PH(psi2, i, N, k) - PH(psi2, i, j, k))) / L.ax[1].dc[j + 1];
+                dpyp = dpyp - 2.0 * STC(s1, 1, yn) + STC(s2, 1, yn);
                 double dpym = (2.0 * (PH(psi1, i, j, k) - PH(psi1, i, S, k)) - (PH(psi2, i, j, k) - PH(psi2, i, S, k))) / L.ax[1].dc[j];
+                dpym = dpym - 2.0 * STC(s1, 1, ys) + STC(s2, 1, ys);
                 double dpzp = (2.0 * (PH(psi1, i, j, F) - PH(psi1, i, j, k)) - (PH(psi2, i, j, F) - PH(psi2, i, j, k))) / dzc;
+                dpzp = dpzp - 2.0 * STC(s1, 2, c) + STC(s2, 2, c);
                 double dpzm = (2.0 * (PH(psi1, i, j, k) - PH(psi1, i, j, B)) - (PH(psi2, i, j, k) - PH(psi2, i, j, B))) / dzc;
+                dpzm = dpzm - 2.0 * STC(s1, 2, cb) + STC(s2, 2, cb);
                 dx = (apx * dpxp - amx * dpxm) / dxf;
                 dy = (apy * dpyp - amy * dpym) / L.ax[1].df[j];
                 dz = (apz * dpzp - amz * dpzm) / dzf;
-                s[IDX(i, j, k)] = sv + dx + dy + dz;
+                s[c] = sv + dx + dy + dz;
             }
 #undef PH
     lev_free(&L);
 }
+
+/* updatePressure (poissonEqn.f90:374-415) with clean_st_prev_bc (:419-459) for the
+ * channel: p = psi; storeOldField (field.f90:1585-1600): psi2 <- psi1 <- psi;
+ * storeOldFieldV: st2 <- st1 <- st; then the old st levels are zeroed on the
+ * faces at and beyond the Neumann walls, which in the stored range 0..ny of the
+ * y component are the wall faces j = 0 and j = ny (the x and z sides are
+ * periodic).  p may be NULL; st (the current level) NULL means zero; st1 and st2
+ * NULL skip the surface-tension levels. */
+void or_update_pressure(const or_grid_desc* g, const double* psi, double* p, double* psi1, double* psi2,
+                        const double* const st[3], double* const st1[3], double* const st2[3])
+{
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    const size_t nc = (size_t)nx * ny * nz, ny1 = (size_t)nx * (ny + 1) * nz;
+    for (size_t t = 0; t < nc; ++t) {
+        if (p) p[t] = psi[t];
+        psi2[t] = psi1[t];
+        psi1[t] = psi[t];
+    }
+    if (!st1 || !st2) return;
+    for (int c = 0; c < 3; ++c) {
+        const size_t n = c == 1 ? ny1 : nc;
+        const double* cur = st ? st[c] : 0;
+        for (size_t t = 0; t < n; ++t) {
+            st2[c][t] = st1[c][t];
+            st1[c][t] = cur ? cur[t] : 0.0;
+        }
+    }
+    for (int k = 1; k <= nz; ++k)
+        for (int i = 1; i <= nx; ++i) {
+            st1[1][UYI(i, 0, k)] = st2[1][UYI(i, 0, k)] = 0.0;
+            st1[1][UYI(i, ny, k)] = st2[1][UYI(i, ny, k)] = 0.0;
+        }
+}
+#undef STC
 
 double or_max_abs(int64_t n, const double* a)
 {

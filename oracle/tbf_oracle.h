@@ -96,10 +96,14 @@ void or_project_mg(const or_grid_desc* g, double alpha_dt, const double* rho, co
                    double* ux, double* uy, double* uz);
 void or_project_fast(const or_grid_desc* g, double alpha_dt, double rho0, const double* rho,
                      const double* psi, const double* psi1, const double* psi2,
+                     const double* const st1[3], const double* const st2[3],
                      double* ux, double* uy, double* uz);
 void or_source_fast(const or_grid_desc* g, double alpha_dt, double rho0, const double* rho,
                     const double* ux, const double* uy, const double* uz,
-                    const double* psi1, const double* psi2, double* s);
+                    const double* psi1, const double* psi2,
+                    const double* const st1[3], const double* const st2[3], double* s);
+void or_update_pressure(const or_grid_desc* g, const double* psi, double* p, double* psi1, double* psi2,
+                        const double* const st[3], double* const st1[3], double* const st2[3]);
 double or_max_abs(int64_t n, const double* a);
 void or_material_props(const or_grid_desc* g, double rho_l, double rho_g, double mu_l, double mu_g,
                        const double* c, double* cs, double* rho, double* mu);

@@ -62,6 +62,7 @@ class PcgParams(C.Structure):
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I64x3 = C.POINTER(C.c_int64)
+_P3 = C.POINTER(C.c_void_p)  # three device pointers (x, y, z components of a staggered field), or NULL
 
 # name -> (restype, argtypes); mirrors include/tbfgpu.h
 SIGNATURES = {
@@ -92,10 +93,11 @@ SIGNATURES = {
     "tbfg_beta_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, _P, _P, _P]),
     "tbfg_project_mg_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, _P, _P, _P, _P, _P, _P]),
     "tbfg_max_divergence_dev": (C.c_int, [C.POINTER(GridDesc), _P, _P, _P, _D, _P]),
-    "tbfg_source_fast_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, _P, _P, _P, _P, _P, _P, _P,
-                                       _P]),
-    "tbfg_project_fast_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, _P, _P, _P, _P, _P, _P, _P,
-                                        _P]),
+    "tbfg_source_fast_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, _P, _P, _P, _P, _P, _P, _P3,
+                                       _P3, _P, _P]),
+    "tbfg_project_fast_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, _P, _P, _P, _P, _P3, _P3, _P,
+                                        _P, _P, _P]),
+    "tbfg_update_pressure_dev": (C.c_int, [C.POINTER(GridDesc), _P, _P, _P, _P, _P3, _P3, _P3, _P]),
     "tbfg_material_work_doubles": (C.c_int64, [C.POINTER(GridDesc)]),
     "tbfg_material_props_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, C.c_double, C.c_double, _P,
                                           _P, _P, _P, _P, _P]),
@@ -146,6 +148,14 @@ def ptr(x):
     if isinstance(x, int):
         return C.c_void_p(x)
     return C.c_void_p(x.data_ptr())
+
+
+def ptr3(v):
+    """A staggered vector field (three tensors, or None) as a C array of three
+    device pointers; None gives NULL."""
+    if v is None:
+        return None
+    return (C.c_void_p * 3)(*(None if t is None else t.data_ptr() for t in v))
 
 
 def stream_ptr(stream):

@@ -344,16 +344,23 @@ int tbfg_project_mg_dev(const tbfg_grid_desc* grid, double alpha_dt, const doubl
 
 int tbfg_source_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rho0, const double* rho,
                          const double* ux, const double* uy, const double* uz, const double* psi1,
-                         const double* psi2, double* s, void* stream)
+                         const double* psi2, const double* const st1[3], const double* const st2[3], double* s,
+                         void* stream)
 {
-    ABI_GUARD(return source_fast_dev(grid, alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2, s, stream);)
+    ABI_GUARD(return source_fast_dev(grid, alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2, st1, st2, s, stream);)
 }
 
 int tbfg_project_fast_dev(const tbfg_grid_desc* grid, double alpha_dt, double rho0, const double* rho,
-                          const double* psi, const double* psi1, const double* psi2, double* ux, double* uy,
-                          double* uz, void* stream)
+                          const double* psi, const double* psi1, const double* psi2, const double* const st1[3],
+                          const double* const st2[3], double* ux, double* uy, double* uz, void* stream)
 {
-    ABI_GUARD(return project_fast_dev(grid, alpha_dt, rho0, rho, psi, psi1, psi2, ux, uy, uz, stream);)
+    ABI_GUARD(return project_fast_dev(grid, alpha_dt, rho0, rho, psi, psi1, psi2, st1, st2, ux, uy, uz, stream);)
+}
+
+int tbfg_update_pressure_dev(const tbfg_grid_desc* grid, const double* psi, double* p, double* psi1, double* psi2,
+                             const double* const st[3], double* const st1[3], double* const st2[3], void* stream)
+{
+    ABI_GUARD(return update_pressure_dev(grid, psi, p, psi1, psi2, st, st1, st2, stream);)
 }
 
 int tbfg_max_divergence_dev(const tbfg_grid_desc* grid, const double* ux, const double* uy, const double* uz,

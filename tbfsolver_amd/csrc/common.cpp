@@ -55,6 +55,14 @@ void Axis::build(int n_, double L_, int stretch_, double gamma_)
     for (int i = -G; i <= n + G; ++i) fbuf[i + G] = coord_face(i, n, L, stretch, gamma);
 }
 
+std::string grid_key(const tbfg_grid_desc* g, int dev)
+{
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%d,%d,%d|%a,%a,%a|%d,%d,%d|%a|dev%d", g->n[0], g->n[1], g->n[2], g->L[0], g->L[1],
+                  g->L[2], g->stretch[0], g->stretch[1], g->stretch[2], g->gamma, dev);
+    return std::string(buf);
+}
+
 int validate_grid(const tbfg_grid_desc* g)
 {
     if (!g) {

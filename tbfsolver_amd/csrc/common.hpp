@@ -63,6 +63,9 @@ struct HostLevel {
 };
 
 int validate_grid(const tbfg_grid_desc* g);
+// Cache key of a grid on a device, built from the descriptor's values (not its
+// bytes, whose padding a caller may leave indeterminate).
+std::string grid_key(const tbfg_grid_desc* g, int dev);
 HostLevel make_level(const tbfg_grid_desc* g, int lev);
 
 // ------------------------------------------------------------ device memory

@@ -35,8 +35,7 @@ static int get_geo(const tbfg_grid_desc* g, FieldGeo& out)
     TBFG_TRY(validate_grid(g));
     int dev = 0;
     TBFG_HIP(hipGetDevice(&dev));
-    std::string key(reinterpret_cast<const char*>(g), sizeof(*g));
-    key += std::to_string(dev);
+    const std::string key = grid_key(g, dev);
     std::lock_guard<std::mutex> lk(g_geo_mu);
     auto it = g_geo_cache.find(key);
     if (it == g_geo_cache.end()) {
@@ -158,13 +157,34 @@ __device__ __forceinline__ double old_grad(const double* __restrict__ psi1, cons
     return (2.0 * (old_level(psi1, b) - old_level(psi1, a)) - (old_level(psi2, b) - old_level(psi2, a))) / dc;
 }
 
-// FAST computeSource + computeOldPressDiv with two old levels (poissonEqn.f90:193-305),
-// without surface tension: s = rho0/(alpha dt) div u* + div[(1 - rho0 (1/rho)_f) G_f],
-// G_f the face gradient of 2 psi^n - psi^(n-1); (1/rho)_f the arithmetic face mean.
+// Old surface-tension levels st^n (1) and st^(n-1) (2), three staggered
+// components laid out like the velocity; a NULL component is zero.
+struct StLevels {
+    const double* x1;
+    const double* y1;
+    const double* z1;
+    const double* x2;
+    const double* y2;
+    const double* z2;
+};
+
+// the face value G_f = old_grad - 2 st^n_f + st^(n-1)_f (poissonEqn.f90:255-301,
+// momentumEqn.f90:1104-1121), in the reference's order of operations
+__device__ __forceinline__ double with_st(double grad, const double* __restrict__ s1, const double* __restrict__ s2,
+                                          size_t f)
+{
+    return grad - 2.0 * old_level(s1, f) + old_level(s2, f);
+}
+
+// FAST computeSource + computeOldPressDiv with two old levels (poissonEqn.f90:193-305):
+// s = rho0/(alpha dt) div u* + div[(1 - rho0 (1/rho)_f) G_f], G_f the face gradient of
+// 2 psi^n - psi^(n-1) minus 2 st^n_f - st^(n-1)_f; (1/rho)_f the arithmetic face mean.
+// The face after a cell reads the st values of the cell's own face index, the face
+// before it those of the previous face (in y the wall faces 0 and ny included).
 __global__ void k_source_fast(FieldGeo G, double r_adt, double rho0, const double* __restrict__ rho,
                               const double* __restrict__ ux, const double* __restrict__ uy,
                               const double* __restrict__ uz, const double* __restrict__ psi1,
-                              const double* __restrict__ psi2, double* __restrict__ s)
+                              const double* __restrict__ psi2, StLevels st, double* __restrict__ s)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
@@ -179,6 +199,8 @@ __global__ void k_source_fast(FieldGeo G, double r_adt, double rho0, const doubl
     const size_t so = i + (size_t)nx * (j > 0 ? j - 1 : j) + plane * k;
     const size_t f = i + (size_t)nx * j + plane * (k + 1 < nz ? k + 1 : 0);
     const size_t b = i + (size_t)nx * j + plane * (k > 0 ? k - 1 : nz - 1);
+    const size_t yt = i + (size_t)nx * ((j + 1) + (size_t)(ny + 1) * k);  // y face after the cell
+    const size_t yb = yt - nx;                                             // y face before it
     const double dxf = G.df[0][0], dzf = G.df[2][0], dyf = G.df[1][j];
     const double dxc = G.dcn[0][0], dzc = G.dcn[2][0];
     const double dycp = G.dcn[1][j], dycm = G.dcn[1][j > 0 ? j - 1 : 0];
@@ -190,19 +212,25 @@ __global__ void k_source_fast(FieldGeo G, double r_adt, double rho0, const doubl
     const double amy = 1.0 - rho0 * 0.5 * (1.0 / rho[so] + ic);
     const double apz = 1.0 - rho0 * 0.5 * (1.0 / rho[f] + ic);
     const double amz = 1.0 - rho0 * 0.5 * (1.0 / rho[b] + ic);
-    const double dx = (apx * old_grad(psi1, psi2, c, e, dxc) - amx * old_grad(psi1, psi2, w, c, dxc)) / dxf;
-    const double dy = (apy * old_grad(psi1, psi2, c, n, dycp) - amy * old_grad(psi1, psi2, so, c, dycm)) / dyf;
-    const double dz = (apz * old_grad(psi1, psi2, c, f, dzc) - amz * old_grad(psi1, psi2, b, c, dzc)) / dzf;
+    const double gxp = with_st(old_grad(psi1, psi2, c, e, dxc), st.x1, st.x2, c);
+    const double gxm = with_st(old_grad(psi1, psi2, w, c, dxc), st.x1, st.x2, w);
+    const double gyp = with_st(old_grad(psi1, psi2, c, n, dycp), st.y1, st.y2, yt);
+    const double gym = with_st(old_grad(psi1, psi2, so, c, dycm), st.y1, st.y2, yb);
+    const double gzp = with_st(old_grad(psi1, psi2, c, f, dzc), st.z1, st.z2, c);
+    const double gzm = with_st(old_grad(psi1, psi2, b, c, dzc), st.z1, st.z2, b);
+    const double dx = (apx * gxp - amx * gxm) / dxf;
+    const double dy = (apy * gyp - amy * gym) / dyf;
+    const double dz = (apz * gzp - amz * gzm) / dzf;
     s[c] = sv + dx + dy + dz;
 }
 
-// FAST makeVelocityDivFree + computeOldPressGrad (momentumEqn.f90:999-1127), without
-// surface tension: u_f -= alpha dt [(1/rho0) d psi + ((1/rho)_f - 1/rho0) G_f] on
-// the periodic x/z faces and the interior y faces.
+// FAST makeVelocityDivFree + computeOldPressGrad (momentumEqn.f90:999-1127):
+// u_f -= alpha dt [(1/rho0) d psi + ((1/rho)_f - 1/rho0) G_f] on the periodic x/z
+// faces and the interior y faces.
 __global__ void k_project_fast(FieldGeo G, double alpha_dt, double rho0, const double* __restrict__ rho,
                                const double* __restrict__ psi, const double* __restrict__ psi1,
-                               const double* __restrict__ psi2, double* __restrict__ ux, double* __restrict__ uy,
-                               double* __restrict__ uz)
+                               const double* __restrict__ psi2, StLevels st, double* __restrict__ ux,
+                               double* __restrict__ uy, double* __restrict__ uz)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
@@ -217,15 +245,62 @@ __global__ void k_project_fast(FieldGeo G, double alpha_dt, double rho0, const d
     const double inv0 = 1.0 / rho0;
     const double ic = 1.0 / rho[c];
     double inv = 0.5 * (1.0 / rho[e] + ic);
-    ux[c] = ux[c] - alpha_dt * (inv0 * ((psi[e] - psi[c]) / dxc) + (inv - inv0) * old_grad(psi1, psi2, c, e, dxc));
+    ux[c] = ux[c] - alpha_dt * (inv0 * ((psi[e] - psi[c]) / dxc) +
+                                (inv - inv0) * with_st(old_grad(psi1, psi2, c, e, dxc), st.x1, st.x2, c));
     inv = 0.5 * (1.0 / rho[f] + ic);
-    uz[c] = uz[c] - alpha_dt * (inv0 * ((psi[f] - psi[c]) / dzc) + (inv - inv0) * old_grad(psi1, psi2, c, f, dzc));
+    uz[c] = uz[c] - alpha_dt * (inv0 * ((psi[f] - psi[c]) / dzc) +
+                                (inv - inv0) * with_st(old_grad(psi1, psi2, c, f, dzc), st.z1, st.z2, c));
     if (j + 1 < ny) {
         const size_t n = c + nx;
         const size_t yf = i + (size_t)nx * ((j + 1) + (size_t)(ny + 1) * k);
         const double dyc = G.dcn[1][j];
         inv = 0.5 * (1.0 / rho[n] + ic);
-        uy[yf] = uy[yf] - alpha_dt * (inv0 * ((psi[n] - psi[c]) / dyc) + (inv - inv0) * old_grad(psi1, psi2, c, n, dyc));
+        uy[yf] = uy[yf] - alpha_dt * (inv0 * ((psi[n] - psi[c]) / dyc) +
+                                      (inv - inv0) * with_st(old_grad(psi1, psi2, c, n, dyc), st.y1, st.y2, yf));
+    }
+}
+
+// updatePressure (poissonEqn.f90:374-415) + clean_st_prev_bc (:419-459) for the
+// channel, one thread per face row entry: p = psi; psi2 <- psi1 <- psi
+// (storeOldField); st2 <- st1 <- st (storeOldFieldV, st NULL = zero); the old st
+// levels zeroed on the y wall faces (j = 0, ny).  Block row j = ny carries only
+// the top wall face of the y component.  Each entry is read and written by its
+// own thread, so the rotation is in place.
+struct StRotate {
+    const double* cur[3];
+    double* l1[3];
+    double* l2[3];
+};
+
+__global__ void k_update_pressure(int nx, int ny, int nz, const double* __restrict__ psi, double* __restrict__ p,
+                                  double* __restrict__ psi1, double* __restrict__ psi2, StRotate st, int with_st)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx) return;
+    if (j < ny) {
+        const size_t c = i + (size_t)nx * (j + (size_t)ny * k);
+        const double v = psi[c];
+        if (p) p[c] = v;
+        psi2[c] = psi1[c];
+        psi1[c] = v;
+        if (with_st) {
+            for (int a = 0; a < 3; a += 2) {
+                st.l2[a][c] = st.l1[a][c];
+                st.l1[a][c] = st.cur[a] ? st.cur[a][c] : 0.0;
+            }
+        }
+    }
+    if (with_st) {
+        const size_t yf = i + (size_t)nx * (j + (size_t)(ny + 1) * k);
+        if (j == 0 || j == ny) {
+            st.l2[1][yf] = 0.0;
+            st.l1[1][yf] = 0.0;
+        } else {
+            st.l2[1][yf] = st.l1[1][yf];
+            st.l1[1][yf] = st.cur[1] ? st.cur[1][yf] : 0.0;
+        }
     }
 }
 
@@ -287,9 +362,16 @@ int beta_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, double
     return TBFG_OK;
 }
 
+// a NULL level, or a NULL component of one, is zero
+static StLevels st_levels(const double* const st1[3], const double* const st2[3])
+{
+    return StLevels{st1 ? st1[0] : nullptr, st1 ? st1[1] : nullptr, st1 ? st1[2] : nullptr,
+                    st2 ? st2[0] : nullptr, st2 ? st2[1] : nullptr, st2 ? st2[2] : nullptr};
+}
+
 int source_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, const double* rho, const double* ux,
-                    const double* uy, const double* uz, const double* psi1, const double* psi2, double* s,
-                    void* stream)
+                    const double* uy, const double* uz, const double* psi1, const double* psi2,
+                    const double* const st1[3], const double* const st2[3], double* s, void* stream)
 {
     if (!rho || !ux || !uy || !uz || !s) {
         set_error("tbfg_source_fast_dev: NULL argument");
@@ -305,15 +387,17 @@ int source_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, const
         set_error("tbfg_source_fast_dev: x and z must be uniform (FAST_MODE)");
         return TBFG_EINVAL;
     }
+    const StLevels st = st_levels(st1, st2);
     dim3 blk = row_block(G.nx);
     hipLaunchKernelGGL(k_source_fast, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G,
-                       1.0 / alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2, s);
+                       1.0 / alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2, st, s);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
 
 int project_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, const double* rho, const double* psi,
-                     const double* psi1, const double* psi2, double* ux, double* uy, double* uz, void* stream)
+                     const double* psi1, const double* psi2, const double* const st1[3],
+                     const double* const st2[3], double* ux, double* uy, double* uz, void* stream)
 {
     if (!rho || !psi || !ux || !uy || !uz) {
         set_error("tbfg_project_fast_dev: NULL argument");
@@ -329,9 +413,46 @@ int project_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, cons
         set_error("tbfg_project_fast_dev: x and z must be uniform (FAST_MODE)");
         return TBFG_EINVAL;
     }
+    const StLevels st = st_levels(st1, st2);
     dim3 blk = row_block(G.nx);
     hipLaunchKernelGGL(k_project_fast, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G,
-                       alpha_dt, rho0, rho, psi, psi1, psi2, ux, uy, uz);
+                       alpha_dt, rho0, rho, psi, psi1, psi2, st, ux, uy, uz);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
+
+int update_pressure_dev(const tbfg_grid_desc* g, const double* psi, double* p, double* psi1, double* psi2,
+                        const double* const st[3], double* const st1[3], double* const st2[3], void* stream)
+{
+    TBFG_TRY(validate_grid(g));
+    if (!psi || !psi1 || !psi2) {
+        set_error("tbfg_update_pressure_dev: psi, psi1 and psi2 must not be NULL");
+        return TBFG_EINVAL;
+    }
+    if ((st1 == nullptr) != (st2 == nullptr)) {
+        set_error("tbfg_update_pressure_dev: st1 and st2 must both be given or both be NULL");
+        return TBFG_EINVAL;
+    }
+    StRotate r{};
+    const int with_st = st1 != nullptr;
+    if (with_st) {
+        for (int a = 0; a < 3; ++a) {
+            if (!st1[a] || !st2[a]) {
+                set_error("tbfg_update_pressure_dev: st1[%d] or st2[%d] is NULL", a, a);
+                return TBFG_EINVAL;
+            }
+            r.cur[a] = st ? st[a] : nullptr;
+            r.l1[a] = st1[a];
+            r.l2[a] = st2[a];
+        }
+    } else if (st) {
+        set_error("tbfg_update_pressure_dev: st given without st1, st2 to store it in");
+        return TBFG_EINVAL;
+    }
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    dim3 blk = row_block(nx);
+    hipLaunchKernelGGL(k_update_pressure, dim3(ceil_div(nx, blk.x), ny + 1, nz), blk, 0, (hipStream_t)stream, nx, ny,
+                       nz, psi, p, psi1, psi2, r, with_st);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }

@@ -36,8 +36,7 @@ static int get_vof_geo(const tbfg_grid_desc* g, VofGeo& out)
     TBFG_TRY(validate_grid(g));
     int dev = 0;
     TBFG_HIP(hipGetDevice(&dev));
-    std::string key(reinterpret_cast<const char*>(g), sizeof(*g));
-    key += std::to_string(dev);
+    const std::string key = grid_key(g, dev);
     std::lock_guard<std::mutex> lk(g_vof_mu);
     auto it = g_vof_cache.find(key);
     if (it == g_vof_cache.end()) {

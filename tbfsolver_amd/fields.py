@@ -5,6 +5,7 @@
   makeVelocityDivFree     momentumEqn.f90:795-869,980-993
   computeSourceFast       poissonEqn.f90:193-305      FAST_MODE source (rho0 splitting, old pressure)
   makeVelocityDivFreeFast momentumEqn.f90:999-1127
+  updatePressure          poissonEqn.f90:374-459      p = psi, old pressure and surface-tension levels
   computeContinuityError  auxiliaryRoutines.f90:42-91 max|div u| * dt
 
 and the host-side input mirrors used by the tests and benchmarks (seeded
@@ -18,7 +19,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import check, lib, ptr, stream_ptr
+from ._lib import check, lib, ptr, ptr3, stream_ptr
 
 
 def computeSource(grid, ux, uy, uz, s, stream=None):
@@ -36,21 +37,32 @@ def makeVelocityDivFree(grid, alpha_dt, rho, psi, ux, uy, uz, stream=None):
                                     ptr(uz), stream_ptr(stream)))
 
 
-def computeSourceFast(grid, alpha_dt, rho0, rho, ux, uy, uz, s, psi1=None, psi2=None, stream=None):
+def computeSourceFast(grid, alpha_dt, rho0, rho, ux, uy, uz, s, psi1=None, psi2=None, st1=None, st2=None,
+                      stream=None):
     """FAST_MODE source with the old-pressure splitting (poissonEqn.f90:193-305);
-    psi1 = psi^n, psi2 = psi^(n-1), None before they exist."""
+    psi1 = psi^n, psi2 = psi^(n-1); st1 = st^n, st2 = st^(n-1), each a tuple
+    (stx, sty, stz) of staggered tensors laid out like (ux, uy, uz).  None before
+    they exist (zero)."""
     check(lib().tbfg_source_fast_dev(C.byref(grid), float(alpha_dt), float(rho0), ptr(rho), ptr(ux), ptr(uy),
-                                     ptr(uz), ptr(psi1) if psi1 is not None else None,
-                                     ptr(psi2) if psi2 is not None else None, ptr(s), stream_ptr(stream)))
+                                     ptr(uz), ptr(psi1), ptr(psi2), ptr3(st1), ptr3(st2), ptr(s),
+                                     stream_ptr(stream)))
     return s
 
 
-def makeVelocityDivFreeFast(grid, alpha_dt, rho0, rho, psi, ux, uy, uz, psi1=None, psi2=None, stream=None):
+def makeVelocityDivFreeFast(grid, alpha_dt, rho0, rho, psi, ux, uy, uz, psi1=None, psi2=None, st1=None, st2=None,
+                            stream=None):
     """FAST_MODE projection (momentumEqn.f90:999-1127)."""
-    check(lib().tbfg_project_fast_dev(C.byref(grid), float(alpha_dt), float(rho0), ptr(rho), ptr(psi),
-                                      ptr(psi1) if psi1 is not None else None,
-                                      ptr(psi2) if psi2 is not None else None, ptr(ux), ptr(uy), ptr(uz),
+    check(lib().tbfg_project_fast_dev(C.byref(grid), float(alpha_dt), float(rho0), ptr(rho), ptr(psi), ptr(psi1),
+                                      ptr(psi2), ptr3(st1), ptr3(st2), ptr(ux), ptr(uy), ptr(uz),
                                       stream_ptr(stream)))
+
+
+def updatePressure(grid, psi, p, psi1, psi2, st=None, st1=None, st2=None, stream=None):
+    """updatePressure + clean_st_prev_bc (poissonEqn.f90:374-459) in one kernel:
+    p = psi, psi2 <- psi1 <- psi, st2 <- st1 <- st, old st levels zero on the
+    walls.  p may be None; st None with st1, st2 given stores zero surface tension."""
+    check(lib().tbfg_update_pressure_dev(C.byref(grid), ptr(psi), ptr(p), ptr(psi1), ptr(psi2), ptr3(st),
+                                         ptr3(st1), ptr3(st2), stream_ptr(stream)))
 
 
 def updateMaterialProps(grid, c, cs, rho, mu=None, rho_l=1.0, rho_g=1.0 / 20.0, mu_l=1.0, mu_g=1.0 / 50.0,

@@ -93,6 +93,35 @@ module tbfgpu_mod
             import
             type(c_ptr), value :: h
         end function
+
+        ! FAST_MODE step on device arrays (computeSource + computeOldPressDiv,
+        ! poissonEqn.f90:193-305; makeVelocityDivFree + computeOldPressGrad,
+        ! momentumEqn.f90:999-1127; updatePressure + clean_st_prev_bc,
+        ! poissonEqn.f90:374-459).  Every field argument is a device address
+        ! (c_ptr).  st, st1, st2 are c_loc of a type(c_ptr), target :: st(3) array
+        ! holding the x, y, z components' device addresses, or c_null_ptr.
+        integer(c_int) function tbfg_source_fast_dev(grid, alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2, &
+                                                     st1, st2, s, stream) bind(C, name='tbfg_source_fast_dev')
+            import
+            type(tbfg_grid_desc), intent(in) :: grid
+            real(c_double), value :: alpha_dt, rho0
+            type(c_ptr), value :: rho, ux, uy, uz, psi1, psi2, st1, st2, s, stream
+        end function
+
+        integer(c_int) function tbfg_project_fast_dev(grid, alpha_dt, rho0, rho, psi, psi1, psi2, st1, st2, &
+                                                      ux, uy, uz, stream) bind(C, name='tbfg_project_fast_dev')
+            import
+            type(tbfg_grid_desc), intent(in) :: grid
+            real(c_double), value :: alpha_dt, rho0
+            type(c_ptr), value :: rho, psi, psi1, psi2, st1, st2, ux, uy, uz, stream
+        end function
+
+        integer(c_int) function tbfg_update_pressure_dev(grid, psi, p, psi1, psi2, st, st1, st2, stream) &
+                bind(C, name='tbfg_update_pressure_dev')
+            import
+            type(tbfg_grid_desc), intent(in) :: grid
+            type(c_ptr), value :: psi, p, psi1, psi2, st, st1, st2, stream
+        end function
     end interface
 
 contains

@@ -169,8 +169,8 @@ class Resident:
     def _footprint(self):
         t = [self.rho, self.psi, self.p] + self.ub + self.up + self.u
         nbytes = sum(a.numel() * a.element_size() for a in t)
-        # allocated by the first steps: s and beta (MG); s and the two old levels (FAST)
-        nbytes += (2 if self.kind == "MG" else 3) * self.psi.numel() * 8
+        # allocated by the first steps: s and the two old pressure levels, and beta (MG)
+        nbytes += (4 if self.kind == "MG" else 3) * self.psi.numel() * 8
         solver = self.eqn.fftSolver_ if self.kind == "FAST" else self.eqn.pcgs_
         return nbytes + solver.footprint
 
@@ -184,7 +184,7 @@ class Resident:
             u.add_(up, alpha=c)
         self.eqn.solvePoissonEqn(self.psi, self.rho, *self.u, ALPHA_DT, stream=stream)
         self.eqn.makeVelocityDivFree(self.psi, self.rho, *self.u, ALPHA_DT, stream=stream)
-        self.eqn.updatePressure(self.psi, self.p)
+        self.eqn.updatePressure(self.psi, self.p, stream=stream)
         self.steps += 1
         return self.psi
 

@@ -126,24 +126,43 @@ def test_host_array_entry_point(torch_gpu):
     assert np.array_equal(p_h[1:-1, -1, 1:-1], p_h[1:-1, -2, 1:-1])
 
 
-@pytest.mark.parametrize("old_levels", [False, True])
-def test_fast_source_and_projection_variable_density(torch_gpu, old_levels):
+def random_st(n, walls=True):
+    """A random staggered field laid out like (ux, uy, uz); with walls, the y wall
+    faces are non-zero too (the reference reads them as stored)."""
+    nx, ny, nz = n
+    sx, sz = rng.standard_normal(n), rng.standard_normal(n)
+    sy = rng.standard_normal((nx, ny + 1, nz))
+    if not walls:
+        sy[:, 0, :] = 0
+        sy[:, -1, :] = 0
+    return sx, sy, sz
+
+
+@pytest.mark.parametrize("levels", ["none", "psi", "psi+st"])
+def test_fast_source_and_projection_variable_density(torch_gpu, levels):
     """FAST_MODE source and projection with the rho0 splitting (poissonEqn.f90:193-305,
     momentumEqn.f90:999-1127) on a stretched channel with bubbles, against the oracle;
-    then the full step s -> solveFPS -> projection, compared field by field."""
+    then the full step s -> solveFPS -> projection, compared field by field.  With
+    "psi+st" the two old surface-tension levels are non-zero everywhere, wall faces
+    included."""
     n = (64, 48, 32)
-    print("case fast source/projection", n, old_levels, flush=True)
+    print("case fast source/projection", n, levels, flush=True)
     g, go = grids(n, 1)
     rho, _ = F.make_bubble_density(g, 6, 0.3, 1003, nref=8)
     ux, uy, uz = F.make_velocity(g, 1003)
     rho0, adt = 1.0 / 20.0, 2e-3
-    psi1 = rng.standard_normal(n) if old_levels else np.zeros(n)
-    psi2 = rng.standard_normal(n) if old_levels else np.zeros(n)
+    old = levels != "none"
+    psi1 = rng.standard_normal(n) if old else np.zeros(n)
+    psi2 = rng.standard_normal(n) if old else np.zeros(n)
+    st1 = random_st(n) if levels == "psi+st" else None
+    st2 = random_st(n) if levels == "psi+st" else None
     d = {k: to_dev(v, torch_gpu) for k, v in dict(rho=rho, ux=ux, uy=uy, uz=uz, psi1=psi1, psi2=psi2).items()}
+    dst1 = None if st1 is None else tuple(to_dev(a, torch_gpu) for a in st1)
+    dst2 = None if st2 is None else tuple(to_dev(a, torch_gpu) for a in st2)
     s = torch_gpu.zeros_like(d["rho"])
     F.computeSourceFast(g, adt, rho0, d["rho"], d["ux"], d["uy"], d["uz"], s,
-                        d["psi1"] if old_levels else None, d["psi2"] if old_levels else None)
-    so = O.source_fast(go, adt, rho0, rho, ux, uy, uz, psi1, psi2)
+                        d["psi1"] if old else None, d["psi2"] if old else None, dst1, dst2)
+    so = O.source_fast(go, adt, rho0, rho, ux, uy, uz, psi1, psi2, st1, st2)
     assert rel(to_host(s), so) < 1e-13
     fs = T.FastPoissonSolver(g)
     p = torch_gpu.zeros_like(s)
@@ -151,17 +170,45 @@ def test_fast_source_and_projection_variable_density(torch_gpu, old_levels):
     po = O.fft_solve(go, so)
     assert rel(to_host(p), po) < 1e-12
     F.makeVelocityDivFreeFast(g, adt, rho0, d["rho"], p, d["ux"], d["uy"], d["uz"],
-                              d["psi1"] if old_levels else None, d["psi2"] if old_levels else None)
-    uxo, uyo, uzo = O.project_fast(go, adt, rho0, rho, po, psi1, psi2, ux, uy, uz)
+                              d["psi1"] if old else None, d["psi2"] if old else None, dst1, dst2)
+    uxo, uyo, uzo = O.project_fast(go, adt, rho0, rho, po, psi1, psi2, ux, uy, uz, st1, st2)
     for dev_a, ref in ((d["ux"], uxo), (d["uy"], uyo), (d["uz"], uzo)):
         assert rel(to_host(dev_a), ref) < 1e-12
     assert np.all(to_host(d["uy"])[:, 0, :] == 0) and np.all(to_host(d["uy"])[:, -1, :] == 0)
+    # the step leaves the velocity divergence free whatever the old levels are
+    assert F.max_divergence(g, d["ux"], d["uy"], d["uz"]) <= 1e-10 * np.abs(O.divergence(go, ux, uy, uz)).max()
+    fs.close()
+
+
+def test_update_pressure_kernel(torch_gpu):
+    """updatePressure + clean_st_prev_bc (poissonEqn.f90:374-459) on the device against
+    the oracle: p = psi, both rotations, old st levels zero on the wall faces only."""
+    n = (48, 20, 12)
+    print("case update pressure", n, flush=True)
+    g, go = grids(n, 1)
+    psi, psi1, psi2 = (rng.standard_normal(n) for _ in range(3))
+    st, st1, st2 = random_st(n), random_st(n), random_st(n)
+    dv = lambda v: tuple(to_dev(a, torch_gpu) for a in v)  # noqa: E731
+    dpsi, d1, d2 = (to_dev(a, torch_gpu) for a in (psi, psi1, psi2))
+    dp = torch_gpu.zeros_like(dpsi)
+    dst, dst1, dst2 = dv(st), dv(st1), dv(st2)
+    F.updatePressure(g, dpsi, dp, d1, d2, dst, dst1, dst2)
+    p, n1, n2, s1, s2 = O.update_pressure(go, psi, psi1, psi2, st, st1, st2)
+    for got, want in ((dp, p), (d1, n1), (d2, n2)):
+        assert np.array_equal(to_host(got), want)
+    for got, want in zip(dst1 + dst2, s1 + s2):
+        assert np.array_equal(to_host(got), want)
+    # without surface tension levels only the pressure levels move
+    F.updatePressure(g, dpsi, None, d1, d2)
+    assert np.array_equal(to_host(d2), psi) and np.array_equal(to_host(d1), psi)
 
 
 def test_poisson_eqn_fast_steps_rotate_old_levels(torch_gpu):
-    """Three FAST_MODE pressure steps through PoissonEqn (source with the old levels
-    psi^n, psi^(n-1), solveFPS, projection, updatePressure), against the oracle
-    doing the same sequence."""
+    """Four FAST_MODE pressure steps through PoissonEqn (source with the old levels
+    psi^n, psi^(n-1), st^n, st^(n-1), solveFPS, projection, updatePressure with the
+    surface-tension rotation and wall clean-up), started from given old levels as
+    poissonEqnCTOR and the VOF constructor set them, against the oracle doing the
+    same sequence."""
     n = (32, 24, 16)
     print("case PoissonEqn FAST steps", n, flush=True)
     g, go = grids(n, 1)
@@ -169,20 +216,27 @@ def test_poisson_eqn_fast_steps_rotate_old_levels(torch_gpu):
     rho_l, rho_g, adt = 1.0, 1.0 / 20.0, 1e-3
     eq = T.PoissonEqn(g, "FAST", rho_l, rho_g)
     drho = to_dev(rho, torch_gpu)
-    psi = torch_gpu.zeros_like(drho)
+    o1, o2 = rng.standard_normal(n), rng.standard_normal(n)
+    t1, t2 = random_st(n), random_st(n)   # as read from the restart files: walls not cleaned yet
+    dv = lambda v: tuple(to_dev(a, torch_gpu) for a in v)  # noqa: E731
+    eq.setOldLevels(to_dev(o1, torch_gpu), to_dev(o2, torch_gpu), dv(t1), dv(t2))
+    psi = to_dev(o1, torch_gpu)
     p = torch_gpu.zeros_like(drho)
-    o1, o2 = np.zeros(n), np.zeros(n)
-    for step in range(3):
+    for step in range(4):
         ux, uy, uz = F.make_velocity(g, 2000 + step)
         dux, duy, duz = (to_dev(a, torch_gpu) for a in (ux, uy, uz))
         eq.solvePoissonEqn(psi, drho, dux, duy, duz, adt)
         eq.makeVelocityDivFree(psi, drho, dux, duy, duz, adt)
-        so = O.source_fast(go, adt, rho_g, rho, ux, uy, uz, o1, o2)
+        so = O.source_fast(go, adt, rho_g, rho, ux, uy, uz, o1, o2, t1, t2)
         po = O.fft_solve(go, so)
-        uxo, uyo, uzo = O.project_fast(go, adt, rho_g, rho, po, o1, o2, ux, uy, uz)
+        uxo, uyo, uzo = O.project_fast(go, adt, rho_g, rho, po, o1, o2, ux, uy, uz, t1, t2)
         assert rel(to_host(psi), po) < 1e-12, step
         assert rel(to_host(dux), uxo) < 1e-12 and rel(to_host(duy), uyo) < 1e-12, step
-        eq.updatePressure(psi, p)
-        o2, o1 = o1, po.copy()
+        assert F.max_divergence(g, dux, duy, duz) <= 1e-10 * np.abs(O.divergence(go, ux, uy, uz)).max(), step
+        st = random_st(n)   # this step's surface tension (computeSurfaceTension, out of this path)
+        eq.updatePressure(psi, p, dv(st))
+        _, o1, o2, t1, t2 = O.update_pressure(go, po, o1, o2, st, t1, t2)
         assert np.array_equal(to_host(p), to_host(psi))
+        for got, want in zip(eq.st_old_[0] + eq.st_old_[1], t1 + t2):
+            assert np.array_equal(to_host(got), want), step
     eq.close()

@@ -298,9 +298,25 @@ def test_pcg_mixed_vs_fp64_oracle(torch_gpu, n, stretch, nb):
         out = to_host(psi)
         true_res = np.linalg.norm(q - O.apply_op(go, beta, out)) / np.linalg.norm(q)
         assert true_res < 2 * tol
-        # error against the FP64 oracle is bounded by the residual times a
-        # modest condition factor
-        assert rel(mean_free(out), mean_free(ref)) < 1e3 * tol
+        err = rel(mean_free(out), mean_free(ref))
+        bound = fp64_error_at(go, beta, q, ref, tol)
+        print("  tol", tol, "mixed error", err, "FP64 oracle error at the same tolerance", bound, flush=True)
+        # as accurate as an FP64 solve stopped at the same tolerance
+        assert err < 5 * bound + 1e-12
+
+
+_FP64_ERR = {}
+
+
+def fp64_error_at(go, beta, q, ref, tol, **prm):
+    """Relative L2 error (mean free) of the oracle's FP64 solve stopped at tol against
+    ref: what the tolerance alone costs, the yardstick for the mixed forms."""
+    key = (tuple(go.n), tol, float(beta.sum()), float(q.sum()))
+    if key not in _FP64_ERR:
+        psi, _, _, rc = O.pcg_solve(go, O.params(n_levels=4, tol=tol, max_iter=300, flexible=1, **prm), beta, q)
+        assert rc == 0
+        _FP64_ERR[key] = rel(mean_free(psi), mean_free(ref))
+    return _FP64_ERR[key]
 
 
 def bf16_engaged(g, **kw):
@@ -338,7 +354,7 @@ def test_pcg_mixed_bf16_vs_fp64_oracle(torch_gpu, monkeypatch, n, stretch, nb):
         true_res = np.linalg.norm(q - O.apply_op(go, beta, out)) / np.linalg.norm(q)
         print("  tol", tol, "iterations", its, "true residual", true_res, flush=True)
         assert true_res < 2 * tol
-        assert rel(mean_free(out), mean_free(ref)) < 1e3 * tol
+        assert rel(mean_free(out), mean_free(ref)) < 5 * fp64_error_at(go, beta, q, ref, tol) + 1e-12
         assert its["mixed-bf16"] <= its["mixed"] + 2
 
 
@@ -423,14 +439,19 @@ def test_projection_step_divergence_free(torch_gpu):
     s.solvePCG(psi, dbeta, dq)
     F.makeVelocityDivFree(g, adt, drho, psi, dux, duy, duz)
     md = F.max_divergence(g, dux, duy, duz)
+    print("  max|div u| %.3e raw, %.3e times dt (computeContinuityError)" % (md, md * adt), flush=True)
     assert md <= 2e-12 * np.linalg.norm(q), (md, np.linalg.norm(q))   # |r|_max <= ||r||_2 <= tol ||q||_2
+    # the continuity error as the reference reports it (auxiliaryRoutines.f90:82: max|div u| * dt)
+    assert F.computeContinuityError(g, dux, duy, duz, adt) <= 1e-10
 
 
 def test_config1_bubbles_128(torch_gpu):
     """Config 1: 128^3 channel (Lx=2pi, Ly=2, Lz=pi, uniform), 18 bubbles D=0.3
     (seed 1001, nref 10 for the sub-sampling), rho_g/rho_l = 1/20, beta = alpha dt / rho,
-    q = div u of the seeded velocity; 4 levels, 2+2 sweeps.  FP64 to 1e-12 and mixed to
-    1e-8 against the oracle solved to 1e-12."""
+    q = div u of the seeded velocity; 4 levels, 2+2 sweeps.  Against the oracle solved
+    to 1e-13: FP64 and mixed-bf16 solved to 1e-13 agree to 1e-10 (relative L2, mean
+    free); the mixed forms at 1e-8 are within 5x the error of the oracle's own FP64
+    solve stopped at 1e-8."""
     n = (128, 128, 128)
     print("case config1", n, flush=True)
     g, go = grids(n, 0)
@@ -438,17 +459,20 @@ def test_config1_bubbles_128(torch_gpu):
     ux, uy, uz = F.make_velocity(g, 1001)
     q = O.divergence(go, ux, uy, uz)
     beta = O.beta_from_rho(go, 1e-3, rho)
-    ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=4, tol=1e-12, max_iter=200, coarse_mode=0, flexible=1),
+    ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=4, tol=1e-13, max_iter=200, coarse_mode=0, flexible=1),
                                    beta, q)
     assert rc == 0
     assert bf16_engaged(g, nLevels=4)
-    for precision, tol, bound in (("fp64", 1e-12, 1e-9), ("mixed", 1e-8, 1e-6), ("mixed-bf16", 1e-8, 1e-6)):
+    e8 = fp64_error_at(go, beta, q, ref, 1e-8)
+    for precision, tol, bound in (("fp64", 1e-13, 1e-10), ("mixed", 1e-8, 5 * e8), ("mixed-bf16", 1e-8, 5 * e8),
+                                  ("mixed-bf16", 1e-13, 1e-10)):
         s = T.PCG(g, nLevels=4, tolPCG=tol, maxIterPCG=200, precision=precision, flexible=1)
         psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
         s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
-        print("  ", precision, "iterations", s.iterations, "oracle", it, flush=True)
+        err = rel(mean_free(to_host(psi)), mean_free(ref))
+        print("  ", precision, "tol", tol, "iterations", s.iterations, "oracle", it, "rel L2 error", err, flush=True)
         assert s.converged
-        assert rel(mean_free(to_host(psi)), mean_free(ref)) < bound
+        assert err < bound
 
 
 @pytest.mark.parametrize("precision,max_it", [("fp64", 2), ("mixed", 4)])
@@ -569,6 +593,53 @@ def test_config2_full_size_properties(torch_gpu):
     F.computeSource(g, ux, uy, uz, d)
     torch_gpu.cuda.synchronize()
     assert float(torch_gpu.linalg.norm(d)) <= 1.05e-8 * div0
+
+
+def test_config2_full_size_matches_oracle(torch_gpu):
+    """The bench workload at its full size (256x192x256, tanh y with gamma 1.5, 72
+    bubbles of seed 1002, rho_g/rho_l = 1/20, alpha dt = 1e-3) against the oracle's
+    FP64 solve of the same right-hand side (flexible PCG with the same V-cycle
+    continued below the coarsest level, to 1e-12): the GPU FP64 pressure to 1e-12
+    agrees to 1e-10 (relative L2, mean free) with the same iteration count within
+    one; the headline's mixed-bf16 solve to 1e-12 agrees to 1e-10 as well; at the
+    headline's 1e-8 the mixed-bf16 error is within 5x that of the GPU FP64 solve
+    stopped at 1e-8; and with the pressures to 1e-12 the projected velocity meets a
+    continuity error of 1e-10 in the reference's form (max|div u| * dt; the raw
+    max|div u| is printed beside it)."""
+    n = (256, 192, 256)
+    print("case config2 full size vs oracle", n, flush=True)
+    g, go = grids(n, 1)
+    rho, _ = F.make_bubble_density(g, 72, 0.3, 1002, nref=10)
+    ux, uy, uz = F.make_velocity(g, 1002)
+    q = O.divergence(go, ux, uy, uz)
+    beta = O.beta_from_rho(go, 1e-3, rho)
+    ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=4, tol=1e-12, max_iter=300, coarse_mode=1, coarse_cycles=1,
+                                                flexible=1), beta, q)
+    assert rc == 0
+    ref = mean_free(ref)
+    print("  oracle: %d iterations, residual %.2e" % (it, res), flush=True)
+    dq, dbeta = to_dev(q, torch_gpu), to_dev(beta, torch_gpu)
+    errs = {}
+    for precision, tol in (("fp64", 1e-12), ("mixed-bf16", 1e-12), ("fp64", 1e-8), ("mixed-bf16", 1e-8)):
+        s = T.PCG(g, nLevels=4, tolPCG=tol, maxIterPCG=300, precision=precision, flexible=1, coarse_max_iter=0)
+        psi = torch_gpu.zeros_like(dq)
+        s.solvePCG(psi, dbeta, dq)
+        assert s.converged and s.res_ <= tol
+        errs[(precision, tol)] = rel(mean_free(to_host(psi)), ref)
+        print("  %s tol %.0e: %d iterations, rel L2 error vs oracle %.3e"
+              % (precision, tol, s.iterations, errs[(precision, tol)]), flush=True)
+        if (precision, tol) == ("fp64", 1e-12):
+            assert abs(s.iterations - it) <= 1, (s.iterations, it)
+        dux, duy, duz = (to_dev(a, torch_gpu) for a in (ux, uy, uz))
+        F.makeVelocityDivFree(g, 1e-3, to_dev(rho, torch_gpu), psi, dux, duy, duz)
+        md = F.max_divergence(g, dux, duy, duz)
+        print("  projected: max|div u| %.3e raw, %.3e times dt" % (md, md * 1e-3), flush=True)
+        if tol == 1e-12:
+            assert md * 1e-3 <= 1e-10
+        s.close()
+    assert errs[("fp64", 1e-12)] < 1e-10
+    assert errs[("mixed-bf16", 1e-12)] < 1e-10
+    assert errs[("mixed-bf16", 1e-8)] < 5 * errs[("fp64", 1e-8)]
 
 
 def test_fused_operator_edge_cases(torch_gpu, monkeypatch):
