@@ -50,7 +50,9 @@ def config3_instances(count=64, seed=1003):
 
 class Instance:
     """One channel instance: uniform grid (Lx = 2 pi, Ly = 2, Lz = pi), the
-    seeded velocity's divergence as right-hand side, and its solver."""
+    seeded velocity's divergence as right-hand side, and its solver.
+    precision_used records the precision the solve runs in: the requested one
+    for PCG + MG instances, always "fp64" for FFT instances."""
 
     def __init__(self, index, n, kind, nbubbles, seed=1003, precision="mixed", tol=1e-8, adt=1e-3):
         import torch
@@ -71,6 +73,7 @@ class Instance:
             # 2.4-3.6x faster than the FP32 one with FP64 refinement at every
             # instance size (0.059 vs 0.214 ms at 96x64x96), and exact
             self.solver = FastPoissonSolver(self.grid, precision="fp64")
+            self.precision_used = "fp64"
             self.beta = None
         else:
             rho, _ = F.make_bubble_density(self.grid, nbubbles, 0.3, iseed, nref=8)
@@ -79,6 +82,7 @@ class Instance:
             F.computeBeta(self.grid, adt, self.rho, self.beta)
             self.solver = PCG(self.grid, nLevels=4, tolPCG=tol, maxIterPCG=200, precision=precision, flexible=1,
                               coarse_max_iter=0 if precision == "mixed" else -1)
+            self.precision_used = precision
         self.iterations = 0
         torch.cuda.synchronize()
 

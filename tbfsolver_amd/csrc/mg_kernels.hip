@@ -421,6 +421,125 @@ __global__ void k_residual_init(LevelDev<double> L, const double* __restrict__ b
     block_partial<2>(v, red);
 }
 
+// The same residual with k_op_march's plane march (psi and beta of a plane
+// staged once in LDS with a one-cell ring, two plane steps of loads in
+// flight), so in-plane neighbours cost no HBM traffic: the column form above
+// re-reads them from HBM when they miss in L2.
+template <int TX, int NT, class TR>
+__global__ void __launch_bounds__(NT) k_resid_march(LevelDev<double> L, const double* __restrict__ beta,
+                                                     const double* __restrict__ q, const double* __restrict__ psi,
+                                                     double* __restrict__ r, TR* __restrict__ r32, Reducer red,
+                                                     int kc, int tiles_x, int ntiles)
+{
+    using Sh = OpTile<TX, NT>;
+    constexpr int TY = Sh::TY, OX = Sh::OX, OS = Sh::OS, ON = Sh::ON;
+    constexpr int PD = 2;
+    __shared__ double sD[4 * OS];
+    __shared__ double sB[4 * OS];
+    __shared__ double sTz[2 * kOpMaxChunk];
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    unsigned id = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) id = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const int tile = (int)(id % (unsigned)ntiles);
+    const int chunk = (int)(id / (unsigned)ntiles);
+    const int k0 = min(chunk * kc, nz);
+    const int k1 = min(nz, k0 + kc);
+    const int tx0 = (tile % tiles_x) * TX;
+    const int ty0 = (tile / tiles_x) * TY;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int gx = tx0 + tx, gy = ty0 + ty;  // TX divides nx, TY divides ny
+    for (int t = threadIdx.x; t < k1 - k0; t += NT) {
+        sTz[t] = L.a[2].ip[k0 + t];
+        sTz[kOpMaxChunk + t] = L.a[2].im[k0 + t];
+    }
+    int off[ON];
+#pragma unroll
+    for (int m = 0; m < ON; ++m) {
+        const int t = min((int)threadIdx.x + m * NT, OS - 1);
+        const int yy = t / OX, xx = t - yy * OX;
+        const int jj = ty0 + yy - 1, ii = tx0 + xx - 1;
+        off[m] = nx * (jj < 0 ? 0 : (jj >= ny ? ny - 1 : jj)) + (ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii));
+    }
+    const size_t plane = (size_t)nx * ny;
+    const size_t own = (size_t)nx * gy + gx;
+    auto modz = [&](int k) { return ((k % nz) + nz) % nz; };
+    auto slot = [&](int k) { return (k - k0 + 4) & 3; };
+    auto fetch = [&](double(&rd)[ON], double(&rb)[ON], double& rq, int kp) {
+        kp = min(kp, k1);  // past the chunk's last plane: plane k1 again (a cache hit, unused)
+        const size_t o = plane * modz(kp);
+#pragma unroll
+        for (int m = 0; m < ON; ++m) {
+            rd[m] = psi[o + off[m]];
+            rb[m] = beta[o + off[m]];
+        }
+        rq = q[plane * modz(kp - 1) + own];  // q of this thread's cell on the plane before
+    };
+    auto stash = [&](const double(&rd)[ON], const double(&rb)[ON], int k) {
+        double* D = sD + slot(k) * OS;
+        double* B = sB + slot(k) * OS;
+#pragma unroll
+        for (int m = 0; m < ON; ++m) {
+            const int t = min((int)threadIdx.x + m * NT, OS - 1);
+            D[t] = rd[m];
+            B[t] = rb[m];
+        }
+    };
+    double acc[2] = {0.0, 0.0};
+    if (k0 < k1) {
+        const double ipx = L.a[0].ip[gx], imx = L.a[0].im[gx];
+        const double ipy = L.a[1].ip[gy], imy = L.a[1].im[gy];
+        const int ic = (ty + 1) * OX + tx + 1;
+        {
+            double rd[ON], rb[ON], rq;
+            fetch(rd, rb, rq, k0 - 1);
+            stash(rd, rb, k0 - 1);
+            fetch(rd, rb, rq, k0);
+            stash(rd, rb, k0);
+        }
+        double rds[PD][ON], rbs[PD][ON], rqs[PD];
+#pragma unroll
+        for (int u = 0; u < PD; ++u) {
+            fetch(rds[u], rbs[u], rqs[u], k0 + 1 + u);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        auto step = [&](int k, double(&rd)[ON], double(&rb)[ON], double& rq) {
+            stash(rd, rb, k + 1);
+            const double qc = rq;  // q of the own cell on plane k
+            fetch(rd, rb, rq, k + 1 + PD);
+            __syncthreads();
+            if (k < k1) {
+                const double* D0 = sD + slot(k) * OS;
+                const double* Dp = sD + slot(k + 1) * OS;
+                const double* Dm = sD + slot(k - 1) * OS;
+                const double* B0 = sB + slot(k) * OS;
+                const double* Bp = sB + slot(k + 1) * OS;
+                const double* Bm = sB + slot(k - 1) * OS;
+                // coefficients and (A psi)_c as apply_cell (rbgs.f90:300-315, pcg.f90:171-176)
+                const double bc = B0[ic];
+                const double aR = 0.5 * (B0[ic + 1] + bc) * ipx;
+                const double aL = 0.5 * (bc + B0[ic - 1]) * imx;
+                const double aT = 0.5 * (B0[ic + OX] + bc) * ipy;
+                const double aB = 0.5 * (bc + B0[ic - OX]) * imy;
+                const double aF = 0.5 * (Bp[ic] + bc) * sTz[k - k0];
+                const double aK = 0.5 * (bc + Bm[ic]) * sTz[kOpMaxChunk + k - k0];
+                const double dc = D0[ic];
+                const double v = aR * (D0[ic + 1] - dc) - aL * (dc - D0[ic - 1]) + aT * (D0[ic + OX] - dc) -
+                                 aB * (dc - D0[ic - OX]) + aF * (Dp[ic] - dc) - aK * (dc - Dm[ic]);
+                const double rc = qc - v;
+                r[plane * k + own] = rc;
+                if (r32) r32[plane * k + own] = store_as<TR>((float)rc);
+                acc[0] += rc * rc;
+                acc[1] += qc * qc;
+            }
+        };
+        for (int k = k0; k < k1; k += PD) {
+#pragma unroll
+            for (int u = 0; u < PD; ++u) step(k + u, rds[u], rbs[u], rqs[u]);
+        }
+    }
+    block_partial<2>(acc, red);
+}
+
 __global__ void k_fin_init(unsigned nblocks, PcgState* st, Reducer red)
 {
     double tot[2];
@@ -1803,9 +1922,23 @@ int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, c
 }
 
 template <class TR>
-int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
-                         double* r, TR* r32, PcgState* st, const Reducer& red, void* stream)
+int launch_residual_init(const LevelDev<double>& L, const OpPlan& m, const double* beta, const double* q,
+                         const double* psi, double* r, TR* r32, PcgState* st, const Reducer& red, void* stream)
 {
+    if (m.tx && 2 * (size_t)m.nblocks <= reduction_slots(L, m)) {  // the operator's plane-march tiles
+#define TBFG_RSM(TX_, NT_)                                                                                        \
+    if (m.tx == TX_ && m.nt == NT_)                                                                               \
+        hipLaunchKernelGGL((k_resid_march<TX_, NT_, TR>), dim3(m.nblocks), dim3(NT_), 0, S(stream), L, beta, q,   \
+                           psi, r, r32, red, m.kc, m.tiles_x, m.ntiles);
+        TBFG_RSM(128, 512) else TBFG_RSM(64, 512) else TBFG_RSM(32, 256) else TBFG_RSM(64, 256) else {
+            set_error("launch_residual_init: no operator tile %dx%d", m.tx, m.nt);
+            return TBFG_EINVAL;
+        }
+#undef TBFG_RSM
+        hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(256), 0, S(stream), m.nblocks, st, red);
+        TBFG_HIP(hipGetLastError());
+        return TBFG_OK;
+    }
     dim3 blk = block2d(L.nx);
     dim3 grd = red_grid3(L, blk);
     hipLaunchKernelGGL(k_residual_init<TR>, grd, blk, 0, S(stream), L, beta, q, psi, r, r32, st, red);
@@ -1967,10 +2100,10 @@ template int launch_prolong_add<float>(const LevelDev<float>&, const float*, flo
 template int launch_convert<double, float>(size_t, const double*, float*, void*);
 template int launch_convert<float, double>(size_t, const float*, double*, void*);
 template int launch_convert<double, bf16s>(size_t, const double*, bf16s*, void*);
-template int launch_residual_init<float>(const LevelDev<double>&, const double*, const double*, const double*,
-                                         double*, float*, PcgState*, const Reducer&, void*);
-template int launch_residual_init<bf16s>(const LevelDev<double>&, const double*, const double*, const double*,
-                                         double*, bf16s*, PcgState*, const Reducer&, void*);
+template int launch_residual_init<float>(const LevelDev<double>&, const OpPlan&, const double*, const double*,
+                                         const double*, double*, float*, PcgState*, const Reducer&, void*);
+template int launch_residual_init<bf16s>(const LevelDev<double>&, const OpPlan&, const double*, const double*,
+                                         const double*, double*, bf16s*, PcgState*, const Reducer&, void*);
 template int launch_pcg_update<float>(size_t, double*, double*, float*, const double*, const double*, PcgState*,
                                       const Reducer&, void*);
 template int launch_pcg_update<bf16s>(size_t, double*, double*, bf16s*, const double*, const double*, PcgState*,

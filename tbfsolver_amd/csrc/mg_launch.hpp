@@ -28,10 +28,11 @@ int launch_op_dir(const LevelDev<double>& L, const OpPlan& plan, const double* b
                   const KernelEvents* kev = nullptr);
 // psi += alpha d of the last iteration that ran (d in d0 or d1 by its parity)
 int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, const PcgState* st, void* stream);
-// r32 (optional): the multigrid's copy of r, TR = float or bf16s
+// r = q - A psi, ||r||^2, ||q||^2; r32 (optional): the multigrid's copy of r,
+// TR = float or bf16s.  With plan.tx the operator's plane-march tiles.
 template <class TR>
-int launch_residual_init(const LevelDev<double>& L, const double* beta, const double* q, const double* psi,
-                         double* r, TR* r32, PcgState* st, const Reducer& red, void* stream);
+int launch_residual_init(const LevelDev<double>& L, const OpPlan& plan, const double* beta, const double* q,
+                         const double* psi, double* r, TR* r32, PcgState* st, const Reducer& red, void* stream);
 template <class TR>
 int launch_pcg_update(size_t n, double* psi, double* r, TR* r32, const double* d, const double* qv,
                       PcgState* st, const Reducer& red, void* stream);

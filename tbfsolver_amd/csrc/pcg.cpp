@@ -772,7 +772,7 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
         }
         return TBFG_OK;
     };
-    TBFG_TRY(launch_residual_init(L0, beta, q, psi, r, r32, st, red, stream));
+    TBFG_TRY(launch_residual_init(L0, h->opp, beta, q, psi, r, r32, st, red, stream));
     TBFG_TRY(precondition(1));
     // one iteration (pcg.f90:142-290): d = (z - m) + gamma d, qv = A d, alpha,
     // psi += alpha d, r -= alpha qv, the stop test, z = M r, the dot products.

@@ -7,7 +7,7 @@ last pressure, the projection, the old-level rotation: poissonEqn.f90:347-459
 and momentumEqn.f90:795-1127 through PoissonEqn).  The reference runs one
 instance per process and has no request loop; this module is the engine's own.
 
-  serving_instances(count, seed)     instance list: config-3 sizes, 70 % PCG + MG, 30 % FFT
+  serving_instances(count, seed)     instance list: sizes up to 256x192x256, 70 % PCG + MG, 30 % FFT
   poisson_arrivals(rate, duration)   open-loop arrival times (exponential gaps, seeded)
   HbmAdmission(budget)               residency accounting by device bytes, LRU eviction
   Resident                           one instance on the device: fields, solver handle, step()
@@ -42,14 +42,24 @@ from .solvers import channel_grid
 ALPHA_DT = 1e-3
 
 
+# sizes of the served instances: the batched workload's sets plus the largest
+# channel, up to 256 x 192 x 256
+SERVE_NXZ = NXZ + (256,)
+SERVE_NY = NY + (192,)
+
+
 def serving_instances(count=256, seed=1004, p_mg=0.7):
-    """[(index, (nx, ny, nz), kind, bubbles)]: nx, nz from {32, 64, 96, 128, 192},
-    ny from {32, 64, 96, 128}; kind "MG" (4..24 bubbles) with probability p_mg,
-    else "FAST"."""
+    """[(index, (nx, ny, nz), kind, bubbles)]: nx, nz from {32, 64, 96, 128, 192, 256},
+    ny from {32, 64, 96, 128, 192}, except instance 0, which is the largest channel
+    (256 x 192 x 256) so that every list spans the whole size range; kind "MG"
+    (4..24 bubbles) with probability p_mg, else "FAST"."""
     out = []
     for i in range(count):
         u = [_uniform01(seed, 5 * i + m) for m in range(5)]
-        n = (NXZ[int(u[0] * len(NXZ))], NY[int(u[1] * len(NY))], NXZ[int(u[2] * len(NXZ))])
+        n = (SERVE_NXZ[int(u[0] * len(SERVE_NXZ))], SERVE_NY[int(u[1] * len(SERVE_NY))],
+             SERVE_NXZ[int(u[2] * len(SERVE_NXZ))])
+        if i == 0:
+            n = (SERVE_NXZ[-1], SERVE_NY[-1], SERVE_NXZ[-1])
         kind = "MG" if u[3] < p_mg else "FAST"
         nb = 4 + int(u[4] * 21) if kind == "MG" else 0
         out.append((i, n, kind, nb))

@@ -23,29 +23,41 @@ def test_config3_instance_list():
 
 
 @pytest.mark.gpu
-def test_concurrent_instances_match_solo_solves(torch_gpu):
-    picks = [x for x in Bt.config3_instances(64, 1003) if x[1][0] * x[1][1] * x[1][2] <= 96 * 96 * 96][:8]
-    print("case batch", [x[1] for x in picks], flush=True)
-    insts = [Bt.Instance(*x) for x in picks]
-    Bt.run_batch(insts, workers=4)
+def test_config3_all_instances_concurrent_equal_solo_and_oracle(torch_gpu):
+    """All 64 instances of the batched workload (every size it draws, both kinds),
+    solved together on 8 streams: each equals its solo solve bit for bit (same
+    iteration count), and each matches the oracle -- the FFT instances (FP64) to
+    1e-12, the PCG + MG instances (mixed precision, here to 1e-11) to 1e-9
+    (relative L2, mean free) against the oracle's FP64 solve to 1e-12."""
+    specs = Bt.config3_instances(64, 1003)
+    sizes = sorted({x[1] for x in specs})
+    print("case batch all 64", len(sizes), "sizes, largest", max(sizes, key=lambda n: n[0] * n[1] * n[2]), flush=True)
+    insts = [Bt.Instance(*x, tol=1e-11) for x in specs]
+    assert {i.precision_used for i in insts if i.kind == "FAST"} == {"fp64"}
+    Bt.run_batch(insts, workers=8)
     together = [(i.iterations, to_host(i.psi)) for i in insts]
     for inst, (it, psi) in zip(insts, together):
         inst.solve()
         torch_gpu.cuda.synchronize()
-        assert inst.iterations == it
-        assert np.array_equal(to_host(inst.psi), psi)
-    # two of them against the oracle
-    for inst in insts[:2]:
+        assert inst.iterations == it, inst.index
+        assert np.array_equal(to_host(inst.psi), psi), inst.index
+    worst = {"FAST": 0.0, "MG": 0.0}
+    for inst in insts:
         go = O.desc(inst.n)
         q = to_host(inst.q)
+        a = to_host(inst.psi)
         if inst.kind == "FAST":
             ref = O.fft_solve(go, q)
-            assert np.linalg.norm(to_host(inst.psi) - ref) / np.linalg.norm(ref) < 1e-10
+            e = np.linalg.norm(a - ref) / np.linalg.norm(ref)
+            assert e < 1e-12, (inst.index, inst.n, e)
         else:
             b = to_host(inst.beta)
-            ref, _, _, rc = O.pcg_solve(go, O.params(n_levels=4, tol=1e-12, max_iter=200, flexible=1), b, q)
-            a = to_host(inst.psi)
+            ref, _, _, rc = O.pcg_solve(go, O.params(n_levels=4, tol=1e-12, max_iter=300, flexible=1), b, q)
+            assert rc == 0
             a, ref = a - a.mean(), ref - ref.mean()
-            assert np.linalg.norm(a - ref) / np.linalg.norm(ref) < 1e-6
+            e = np.linalg.norm(a - ref) / np.linalg.norm(ref)
+            assert e < 1e-9, (inst.index, inst.n, e)
+        worst[inst.kind] = max(worst[inst.kind], e)
+    print("  worst rel L2 error vs oracle:", worst, flush=True)
     for inst in insts:
         inst.close()

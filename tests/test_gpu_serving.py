@@ -11,19 +11,22 @@ from tbfsolver_amd import serving as S
 
 pytestmark = pytest.mark.gpu
 
-SPECS = [
-    (0, (32, 32, 32), "MG", 6),
-    (1, (64, 32, 32), "FAST", 0),
-    (2, (32, 64, 64), "MG", 9),
-    (3, (64, 32, 64), "MG", 4),
-    (4, (32, 32, 64), "FAST", 0),
-    (5, (64, 64, 32), "MG", 12),
-    (6, (96, 32, 32), "FAST", 0),
-    (7, (32, 96, 32), "MG", 5),
-]
+def pick_specs():
+    """Ten instances of the serving workload's own list (serving_instances(256, 1004)):
+    its first, the 256x192x256 channel, another of at least 192^3 cells, and the
+    eight smallest others (both kinds), renumbered 0..9 for the request stream."""
+    specs = S.serving_instances(256, 1004)
+    big = [x for x in specs[1:] if x[1][0] * x[1][1] * x[1][2] >= 192 ** 3][:1]
+    rest = sorted((x for x in specs[1:] if x not in big), key=lambda x: (x[1][0] * x[1][1] * x[1][2], x[0]))
+    small = [x for x in rest if x[2] == "FAST"][:3] + [x for x in rest if x[2] == "MG"][:5]
+    chosen = [specs[0]] + big + small
+    return [(k, n, kind, nb) for k, (_, n, kind, nb) in enumerate(chosen)]
 
 
-def serve(workers, budget=200e9, rate=4000.0):
+SPECS = pick_specs()
+
+
+def serve(workers, budget=200e9, rate=2000.0):
     reqs = S.poisson_arrivals(rate, 40 / rate, len(SPECS), seed=11)
     loop = S.ServingLoop(SPECS, precision="mixed", workers=workers, hbm_budget=budget, keep_results=True)
     out = loop.run(reqs)
@@ -35,7 +38,8 @@ def serve(workers, budget=200e9, rate=4000.0):
 
 
 def test_results_in_the_mix_equal_solo(torch_gpu):
-    print("case serving mix vs solo", flush=True)
+    print("case serving mix vs solo", [x[1:3] for x in SPECS], flush=True)
+    assert SPECS[0][1] == (256, 192, 256) and max(x[1][0] * x[1][1] * x[1][2] for x in SPECS[1:]) >= 192 ** 3
     reqs, out_mix, mix, divs, _ = serve(workers=4)
     assert out_mix["requests"] == len(reqs) and len(reqs) >= 20
     assert all(l is not None and l > 0 for l in out_mix["latency_ms"])
@@ -54,7 +58,9 @@ def test_results_in_the_mix_equal_solo(torch_gpu):
 
 def test_evictions_keep_results(torch_gpu):
     print("case serving with evictions", flush=True)
-    budget = 3 * S.estimate_footprint((64, 64, 32), "MG", "mixed")
+    # room for the largest instance and a few small ones: the small ones are evicted and re-admitted
+    budget = S.estimate_footprint((256, 192, 256), "MG", "mixed") + 3 * S.estimate_footprint((64, 64, 32), "MG",
+                                                                                               "mixed")
     reqs, _, evicted, _, ev = serve(workers=2, budget=budget)
     assert ev > 0
     _, _, solo, _, _ = serve(workers=1)

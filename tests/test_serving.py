@@ -10,7 +10,9 @@ from tbfsolver_amd.batch import NXZ, NY
 def test_instance_list_sizes_and_mix():
     specs = S.serving_instances(256, seed=1004)
     assert [s[0] for s in specs] == list(range(256))
-    assert all(s[1][0] in NXZ and s[1][2] in NXZ and s[1][1] in NY for s in specs)
+    assert all(s[1][0] in S.SERVE_NXZ and s[1][2] in S.SERVE_NXZ and s[1][1] in S.SERVE_NY for s in specs)
+    assert specs[0][1] == (256, 192, 256)   # the workload spans up to the largest channel
+    assert set(NXZ) < set(S.SERVE_NXZ) and set(NY) < set(S.SERVE_NY)
     mg = [s for s in specs if s[2] == "MG"]
     assert 0.6 < len(mg) / 256 < 0.8  # 70 % PCG + MG
     assert all(4 <= s[3] <= 24 for s in mg)
