@@ -3,11 +3,15 @@
 the result of the same instance's requests run alone, also when the HBM
 budget forces evictions and re-admissions; projected velocities are
 divergence free."""
+import math
+
 import numpy as np
 import pytest
 
 from tbfsolver_amd import fields as F
 from tbfsolver_amd import serving as S
+from oracle import pyoracle as O
+from conftest import to_host
 
 pytestmark = pytest.mark.gpu
 
@@ -26,12 +30,23 @@ def pick_specs():
 SPECS = pick_specs()
 
 
+def div_bound(r):
+    """Bound on max|div u| after an instance's last projection.  The grids are uniform, so
+    for the PCG + MG instances (relative residual <= tolPCG = 1e-8) max|div u| <= ||div u||_2
+    <= 1e-8 ||div u*||_2, with u* the velocity the last step projected (twice that here for
+    round-off); the FFT instances are exact to round-off, 1e-10 max|div u*|."""
+    c = math.sin(0.7 * (r.steps - 1) + 0.3 * r.index)
+    ustar = [to_host(ub + c * up) for ub, up in zip(r.ub, r.up)]
+    d = O.divergence(O.desc(r.n), *ustar)
+    return 2e-8 * np.linalg.norm(d) if r.kind == "MG" else 1e-10 * np.abs(d).max()
+
+
 def serve(workers, budget=200e9, rate=2000.0):
     reqs = S.poisson_arrivals(rate, 40 / rate, len(SPECS), seed=11)
     loop = S.ServingLoop(SPECS, precision="mixed", workers=workers, hbm_budget=budget, keep_results=True)
     out = loop.run(reqs)
     res = {k: v.cpu() for k, v in loop.results.items()}
-    divs = {i: F.max_divergence(r.grid, *r.u) for i, r in loop.res.items()}
+    divs = {i: (F.max_divergence(r.grid, *r.u), div_bound(r)) for i, r in loop.res.items()}
     ev = loop.evictions
     loop.close()
     return reqs, out, res, divs, ev
@@ -52,8 +67,8 @@ def test_results_in_the_mix_equal_solo(torch_gpu):
     for _, i in reqs:
         steps[i] = steps.get(i, 0) + 1
     assert all((i, steps[i] - 1) in mix for i in steps)
-    for i, d in divs.items():
-        assert np.isfinite(d) and d < 1e-5, (i, d)  # |div u| after projection (u ~ 1, h ~ 0.1)
+    for i, (d, bound) in divs.items():
+        assert np.isfinite(d) and d <= bound, (i, d, bound)
 
 
 def test_evictions_keep_results(torch_gpu):
