@@ -257,6 +257,17 @@ int tbfg_update_pressure_dev(const tbfg_grid_desc* grid, const double* psi, doub
 int tbfg_max_divergence_dev(const tbfg_grid_desc* grid, const double* ux, const double* uy, const double* uz,
                             double* out, void* stream);
 
+/* No counterpart in the reference: tbfg_max_divergence_dev for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  The
+ * same result, bit for bit, without a hipMalloc or hipFree: the max word and the grid
+ * metrics live in a static device array of 196,864 bytes per GPU, held until the
+ * process ends, so every axis may have at most 4096 cells (more: TBFG_EINVAL).  Every
+ * call on a GPU shares that array; calls are serialised per GPU and each returns with
+ * its stream drained, so calls from several threads do not overlap (overlapping use of
+ * the array would race).  Blocking; *out is a host double. */
+int tbfg_max_divergence_noalloc_dev(const tbfg_grid_desc* grid, const double* ux, const double* uy,
+                                    const double* uz, double* out, void* stream);
+
 /* updateMaterialProps + smoothVF (VOF.f90:197-262): two passes of cellToVertex /
  * vertexToCell (interpolation.f90:188-276) smooth the volume fraction c into cs
  * (ghosts: periodic x, z; the wall row in y), then rho = rho_g cs + (1-cs) rho_l

@@ -93,6 +93,7 @@ SIGNATURES = {
     "tbfg_beta_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, _P, _P, _P]),
     "tbfg_project_mg_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, _P, _P, _P, _P, _P, _P]),
     "tbfg_max_divergence_dev": (C.c_int, [C.POINTER(GridDesc), _P, _P, _P, _D, _P]),
+    "tbfg_max_divergence_noalloc_dev": (C.c_int, [C.POINTER(GridDesc), _P, _P, _P, _D, _P]),
     "tbfg_source_fast_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, _P, _P, _P, _P, _P, _P, _P3,
                                        _P3, _P, _P]),
     "tbfg_project_fast_dev": (C.c_int, [C.POINTER(GridDesc), C.c_double, C.c_double, _P, _P, _P, _P, _P3, _P3, _P,

@@ -369,6 +369,12 @@ int tbfg_max_divergence_dev(const tbfg_grid_desc* grid, const double* ux, const 
     ABI_GUARD(return max_divergence_dev(grid, ux, uy, uz, out, stream);)
 }
 
+int tbfg_max_divergence_noalloc_dev(const tbfg_grid_desc* grid, const double* ux, const double* uy,
+                                    const double* uz, double* out, void* stream)
+{
+    ABI_GUARD(return max_divergence_noalloc_dev(grid, ux, uy, uz, out, stream);)
+}
+
 int64_t tbfg_material_work_doubles(const tbfg_grid_desc* grid)
 {
     return material_work_doubles(grid);

@@ -349,6 +349,103 @@ int max_divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* 
     return TBFG_OK;
 }
 
+// ----------------------------------------------- continuity error without allocation
+//
+// The same kernel as max_divergence_dev with its scratch -- the max word and the
+// per-axis metrics -- in one file-scope device array instead of a hipMalloc per call
+// and the geometry cache: [max word, padding to 256 B | df, dc(i+1) of x | of y | of z].
+// Each device holds its own copy; its address comes from hipGetSymbolAddress on the
+// first call on that device (the array's name is not a device address in host code).
+constexpr int kNoAllocMaxAxis = 4096;   // cells per axis
+constexpr size_t kNoAllocHead = 32;     // doubles before the metrics
+constexpr size_t kNoAllocDoubles = kNoAllocHead + 2 * 3 * (size_t)kNoAllocMaxAxis;  // 196,864 bytes
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+struct NoAllocScratch {
+    std::mutex mu;           // held for a whole call: the call ends with its stream drained
+    double* ptr = nullptr;   // this device's copy of `scratch`
+};
+
+static std::mutex g_noalloc_mu;
+static std::map<int, std::unique_ptr<NoAllocScratch>> g_noalloc;  // by device
+
+static int noalloc_scratch(int dev, NoAllocScratch** out)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    std::unique_ptr<NoAllocScratch>& slot = g_noalloc[dev];
+    if (!slot) {
+        void* p = nullptr;
+        TBFG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        std::unique_ptr<NoAllocScratch> d(new NoAllocScratch());
+        d->ptr = static_cast<double*>(p);
+        slot = std::move(d);
+    }
+    *out = slot.get();
+    return TBFG_OK;
+}
+
+int max_divergence_noalloc_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, const double* uz,
+                               double* out, void* stream)
+{
+    if (!ux || !uy || !uz || !out) {
+        set_error("tbfg_max_divergence_noalloc_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    TBFG_TRY(validate_grid(g));
+    for (int a = 0; a < 3; ++a)
+        if (g->n[a] > kNoAllocMaxAxis) {
+            set_error("tbfg_max_divergence_noalloc_dev: axis %d has %d cells, at most %d fit its scratch", a,
+                      g->n[a], kNoAllocMaxAxis);
+            return TBFG_EINVAL;
+        }
+    int dev = 0;
+    TBFG_HIP(hipGetDevice(&dev));
+    NoAllocScratch* d = nullptr;
+    TBFG_TRY(noalloc_scratch(dev, &d));
+    std::lock_guard<std::mutex> lk(d->mu);
+
+    // host image of the used part of the scratch: a zero max word, then the metrics
+    // (the values get_geo uploads, so both entry points compute the same result)
+    const HostLevel L = make_level(g, 0);
+    std::vector<double> h(kNoAllocHead, 0.0);
+    FieldGeo G;
+    G.nx = L.n[0];
+    G.ny = L.n[1];
+    G.nz = L.n[2];
+    size_t off[3][2];
+    for (int a = 0; a < 3; ++a) {
+        off[a][0] = h.size();
+        for (int i = 1; i <= L.n[a]; ++i) h.push_back(L.ax[a].df(i));
+        off[a][1] = h.size();
+        for (int i = 1; i <= L.n[a]; ++i) h.push_back(L.ax[a].dc(i + 1));
+    }
+    for (int a = 0; a < 3; ++a) {
+        G.df[a] = d->ptr + off[a][0];
+        G.dcn[a] = d->ptr + off[a][1];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long bits = 0;
+    hipError_t e = hipMemcpyAsync(d->ptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        const dim3 blk = row_block(G.nx);
+        hipLaunchKernelGGL(k_maxdiv, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, s, G, ux, uy, uz,
+                           reinterpret_cast<unsigned long long*>(d->ptr));
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&bits, d->ptr, sizeof bits, hipMemcpyDeviceToHost, s);
+    // drained on every path: `h` and `bits` live on this frame, and the next call reuses the scratch
+    const hipError_t es = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) {
+        set_error("tbfg_max_divergence_noalloc_dev: %s", hipGetErrorString(e));
+        return TBFG_EDEVICE;
+    }
+    double v;
+    std::memcpy(&v, &bits, sizeof v);
+    *out = v;
+    return TBFG_OK;
+}
+
 int beta_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, double* beta, void* stream)
 {
     TBFG_TRY(validate_grid(g));

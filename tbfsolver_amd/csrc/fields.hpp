@@ -9,6 +9,8 @@ int divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, 
                    void* stream);
 int max_divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, const double* uz, double* out,
                        void* stream);
+int max_divergence_noalloc_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, const double* uz,
+                               double* out, void* stream);
 int beta_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, double* beta, void* stream);
 int source_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, const double* rho, const double* ux,
                     const double* uy, const double* uz, const double* psi1, const double* psi2,

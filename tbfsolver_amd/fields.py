@@ -88,6 +88,14 @@ def max_divergence(grid, ux, uy, uz, stream=None):
     return float(out.value)
 
 
+def max_divergence_noalloc(grid, ux, uy, uz, stream=None):
+    """max_divergence without any device allocation (static scratch, axes <= 4096 cells)."""
+    out = C.c_double(0.0)
+    check(lib().tbfg_max_divergence_noalloc_dev(C.byref(grid), ptr(ux), ptr(uy), ptr(uz), C.byref(out),
+                                                stream_ptr(stream)))
+    return float(out.value)
+
+
 def computeContinuityError(grid, ux, uy, uz, dt, stream=None):
     """max|div u| * dt, as the reference prints it."""
     return max_divergence(grid, ux, uy, uz, stream) * dt
