@@ -159,12 +159,25 @@ def batch_leg(torch, reps=3, workers=8):
         errs.append(float(torch.linalg.norm(a - b) / torch.linalg.norm(b)))
         ref.close()
     cells = sum(n[0] * n[1] * n[2] for _, n, _, _ in specs)
+    # the same pass as one graph launch (every solve's kernels captured once)
+    t0 = time.perf_counter()
+    bg = Bt.BatchGraph(insts, branches=12)
+    capture_s = time.perf_counter() - t0
+    bg.run(1)  # warm-up replay
+    gpass = min(bg.run(1) for _ in range(reps))
+    unconverged = bg.harvest()
+    bg.close()
     for inst in insts:
         inst.close()
-    log("batch: 64 instances (%.1f M cells) %.1f ms per pass on %d streams, %.1f ms on one; max error %.1e"
-        % (cells / 1e6, conc * 1e3, workers, seq * 1e3, max(errs)))
+    log("batch: 64 instances (%.1f M cells) %.1f ms per pass on %d streams, %.1f ms on one, %.1f ms as one graph "
+        "launch; max error %.1e" % (cells / 1e6, conc * 1e3, workers, seq * 1e3, gpass * 1e3, max(errs)))
     return {"instances": 64, "cells_total": cells, "solves_per_s": round(64 / conc, 2),
             "ms_per_pass": round(conc * 1e3, 3), "streams": workers,
+            "graph": {"launches_per_pass": 1, "ms_per_pass": round(gpass * 1e3, 3),
+                      "solves_per_s": round(64 / gpass, 2), "branches": 12,
+                      "capture_s": round(capture_s, 2), "unconverged": unconverged,
+                      "what": "every solve of the pass captured in one HIP graph (PCG iterations queued on the "
+                              "device, stop test on the device), replayed by one launch"},
             "one_stream_solves_per_s": round(64 / seq, 2), "max_rel_error_vs_fp64_1e-12": max(errs),
             "error_sample": "every third instance", "setup_s": round(setup, 1),
             "tile_shapes": "tuned per size (%d sizes, tbfsolver_amd/tuning/mi355x_config3_mixed.json)" % tuned

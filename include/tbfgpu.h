@@ -159,6 +159,21 @@ int tbfg_pcg_solve(tbfg_pcg* h, double* psi, const int64_t psi_lb[3], const int6
 int tbfg_pcg_solve_dev(tbfg_pcg* h, double* psi_dev, const double* beta_dev, const double* q_dev,
                        int32_t* iters, double* rel_res, void* stream);
 
+/* No counterpart in the reference: solvePCG (pcg.f90:103-140) enqueued without any
+ * host synchronisation, for callers that capture a whole batch of solves in one HIP
+ * graph.  Exactly `iterations` PCG iterations are queued (1..max_iter); the device's
+ * stop test still ends the solve, and iterations queued after it return at once, so
+ * the result equals tbfg_pcg_solve_dev's when `iterations` is at least the count that
+ * solve needs.  Asynchronous; the stream carries only kernels.  Not for a handle whose
+ * solve must synchronise (tolMG loop, coarsest level larger than the tail kernel takes,
+ * no pre-smoothing, kernel profiling): TBFG_EINVAL. */
+int tbfg_pcg_solve_queued_dev(tbfg_pcg* h, double* psi_dev, const double* beta_dev, const double* q_dev,
+                              int32_t iterations, void* stream);
+/* The outcome of the last queued solve, read after its work has finished: iterations,
+ * relative residual, and TBFG_OK, TBFG_ENOTCONVERGED (max_iter reached, or not converged
+ * within the iterations queued) or TBFG_ENUMERIC.  Blocking. */
+int tbfg_pcg_result(tbfg_pcg* h, int32_t* iters, double* res);
+
 /* Applies the handle's multigrid preconditioner once: z = M r (device arrays). */
 int tbfg_pcg_precondition_dev(tbfg_pcg* h, const double* beta_dev, const double* r_dev, double* z_dev,
                               void* stream);

@@ -80,6 +80,8 @@ SIGNATURES = {
     "tbfg_pcg_solve": (C.c_int, [_P, _D, _I64x3, _I64x3, _D, _I64x3, _I64x3, _D, _I64x3, _I64x3,
                                  C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "tbfg_pcg_solve_dev": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_int32), C.POINTER(C.c_double), _P]),
+    "tbfg_pcg_solve_queued_dev": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P]),
+    "tbfg_pcg_result": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "tbfg_pcg_precondition_dev": (C.c_int, [_P, _P, _P, _P, _P]),
     "tbfg_pcg_footprint": (C.c_int64, [_P]),
     "tbfg_pcg_destroy": (C.c_int, [_P]),

@@ -11,6 +11,15 @@ interpreter lock during every call, and distinct handles are independent.
   config3_instances(count, seed)  the instance list (sizes, kind, bubbles)
   Instance                        one instance: grid, inputs, handle, solve()
   run_batch(instances, workers)   all solves from a shared queue, wall time
+  BatchGraph(instances, branches) every solve of a pass captured in one HIP
+                                  graph: one launch per pass (replay())
+
+One launch per pass: a persistent kernel over dependent phases would need a
+grid-wide barrier between them, and on this part a device-scope barrier costs
+3-20 us (it grows with the workgroups taking part) against ~2.5 us for a
+dependent kernel launch.  So the pass is one graph instead: each instance's
+solve is a chain of kernels in it (its PCG iterations queued on the device,
+the stop test on the device), the chains spread over parallel branches.
 """
 import concurrent.futures as cf
 import time
@@ -95,6 +104,16 @@ class Instance:
             self.iterations = self.solver.iterations
         return self.psi
 
+    def solve_queued(self, iterations, stream=None):
+        """The solve without host synchronisation (the PCG with `iterations`
+        iterations queued): capturable in a graph."""
+        self.psi.zero_()
+        if self.kind == "FAST":
+            self.solver.solveFPS(self.psi, self.q, stream=stream)
+        else:
+            self.solver.solvePCG_queued(self.psi, self.beta, self.q, iterations, stream=stream)
+        return self.psi
+
     def close(self):
         self.solver.close()
 
@@ -138,3 +157,72 @@ def run_batch(instances, workers=8):
         list(ex.map(work, range(len(streams))))
     torch.cuda.synchronize()
     return time.perf_counter() - t0
+
+
+class BatchGraph:
+    """Every instance's solve in one HIP graph, replayed by one launch per pass.
+
+    Each PCG + MG instance is solved once on the host's usual path first; its
+    iteration count is what its chain in the graph queues (plus `margin`), and the
+    device's stop test still ends the solve, so a replay gives that solve bit for
+    bit.  The chains are spread over `branches` parallel branches, largest work
+    first onto the least loaded branch.  harvest() reads every PCG outcome after a
+    replay and returns the instances that did not converge (none while the inputs
+    are the ones the counts came from)."""
+
+    def __init__(self, instances, branches=12, margin=0):
+        import torch
+
+        self.instances = instances
+        for inst in instances:
+            inst.solve()
+        torch.cuda.synchronize()
+        self.queue = {inst.index: inst.iterations + margin for inst in instances if inst.kind == "MG"}
+        load = [0] * max(1, branches)
+        self.branches = [[] for _ in load]
+        for i in sorted(range(len(instances)), key=lambda i: -_cost(instances[i])):
+            b = load.index(min(load))
+            self.branches[b].append(instances[i])
+            load[b] += _cost(instances[i]) * (self.queue.get(instances[i].index, 1))
+        self.graph = torch.cuda.CUDAGraph()
+        cap = torch.cuda.Stream()
+        side = [torch.cuda.Stream() for _ in self.branches]
+        torch.cuda.synchronize()
+        with torch.cuda.graph(self.graph, stream=cap):
+            for chain, s in zip(self.branches, side):
+                s.wait_stream(cap)
+                with torch.cuda.stream(s):
+                    for inst in chain:
+                        inst.solve_queued(self.queue.get(inst.index, 0), stream=s)
+            for s in side:
+                cap.wait_stream(s)
+        torch.cuda.synchronize()
+
+    def replay(self):
+        """One pass: a single graph launch (asynchronous on the current stream)."""
+        self.graph.replay()
+
+    def harvest(self):
+        """After a replay has finished: every PCG instance's iterations, residual and
+        convergence from its device state; returns the instances not converged."""
+        bad = []
+        for inst in self.instances:
+            if inst.kind == "MG":
+                if not inst.solver.result():
+                    bad.append(inst.index)
+                inst.iterations = inst.solver.iterations
+        return bad
+
+    def run(self, reps=1):
+        """Wall time in seconds of `reps` passes (replays), from an idle device."""
+        import torch
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            self.replay()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    def close(self):
+        self.graph.reset()

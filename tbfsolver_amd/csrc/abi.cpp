@@ -217,6 +217,17 @@ int tbfg_pcg_solve_dev(tbfg_pcg* h, double* psi_dev, const double* beta_dev, con
     ABI_GUARD(return pcg_solve_dev(h, psi_dev, beta_dev, q_dev, iters, rel_res, stream);)
 }
 
+int tbfg_pcg_solve_queued_dev(tbfg_pcg* h, double* psi_dev, const double* beta_dev, const double* q_dev,
+                              int32_t iterations, void* stream)
+{
+    ABI_GUARD(return pcg_solve_queued_dev(h, psi_dev, beta_dev, q_dev, iterations, stream);)
+}
+
+int tbfg_pcg_result(tbfg_pcg* h, int32_t* iters, double* res)
+{
+    ABI_GUARD(return pcg_result(h, iters, res);)
+}
+
 int tbfg_pcg_precondition_dev(tbfg_pcg* h, const double* beta_dev, const double* r_dev, double* z_dev, void* stream)
 {
     ABI_GUARD({

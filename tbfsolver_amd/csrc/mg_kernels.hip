@@ -540,6 +540,19 @@ __global__ void __launch_bounds__(NT) k_resid_march(LevelDev<double> L, const do
     block_partial<2>(acc, red);
 }
 
+// a solve's state before its first kernel: every field zero but the parameters
+// (a kernel rather than a copy from host memory, so that a solve can be captured
+// in a graph and replayed)
+__global__ void k_state_init(PcgState* st, double tol, int32_t max_iter, int32_t flexible)
+{
+    if (threadIdx.x != 0) return;
+    PcgState z{};
+    z.tol = tol;
+    z.max_iter = max_iter;
+    z.flexible = flexible;
+    *st = z;
+}
+
 __global__ void k_fin_init(unsigned nblocks, PcgState* st, Reducer red)
 {
     double tot[2];
@@ -1913,6 +1926,13 @@ template int launch_op_dir<float>(const LevelDev<double>&, const OpPlan&, const 
 template int launch_op_dir<double>(const LevelDev<double>&, const OpPlan&, const double*, const double*,
                                    const double*, double*, double*, int, double*, PcgState*, const Reducer&, void*,
                                    const KernelEvents*);
+
+int launch_state_init(PcgState* st, double tol, int32_t max_iter, int32_t flexible, void* stream)
+{
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, S(stream), st, tol, max_iter, flexible);
+    TBFG_HIP(hipGetLastError());
+    return TBFG_OK;
+}
 
 int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, const PcgState* st, void* stream)
 {

@@ -28,6 +28,8 @@ int launch_op_dir(const LevelDev<double>& L, const OpPlan& plan, const double* b
                   const KernelEvents* kev = nullptr);
 // psi += alpha d of the last iteration that ran (d in d0 or d1 by its parity)
 int launch_psi_last(size_t n, double* psi, const double* d0, const double* d1, const PcgState* st, void* stream);
+// the state of a new solve: zero but tol, max_iter and flexible
+int launch_state_init(PcgState* st, double tol, int32_t max_iter, int32_t flexible, void* stream);
 // r = q - A psi, ||r||^2, ||q||^2; r32 (optional): the multigrid's copy of r,
 // TR = float or bf16s.  With plan.tx the operator's plane-march tiles.
 template <class TR>

@@ -723,23 +723,58 @@ static int vcycle(tbfg_pcg* h, const T0* r0, T** z, const T0* beta0, const int* 
     return TBFG_OK;
 }
 
+// The outcome of the last solve from the host copy of its state (iterations,
+// residual, status; the cycle counts of fixed-cycle applications).
+static int finish_state(tbfg_pcg* h, bool fft, int32_t* iters, double* res)
+{
+    if (fft) {
+        h->hst->mg_cyc_total = h->hst->mg_cyc_max = 0;
+    } else if (!(h->prm.mg_tol > 0.0)) {
+        // fixed cycles: the first application and one after every update but
+        // the last (whose stop test comes before it; the reference applies
+        // the preconditioner once more before testing, pcg.f90:122-136)
+        h->hst->mg_cyc_total = h->hst->iter * h->prm.mg_cycles;
+        h->hst->mg_cyc_max = h->hst->iter > 0 ? h->prm.mg_cycles : 0;
+    }
+    if (iters) *iters = h->hst->iter;
+    if (res) *res = h->hst->res;
+    if (h->hst->status == TBFG_ENUMERIC) {
+        set_error("PCG: non-finite value (res=%g) after %d iterations", h->hst->res, h->hst->iter);
+        return TBFG_ENUMERIC;
+    }
+    if (h->hst->status == TBFG_ENOTCONVERGED) {
+        set_error("EXIT PCG iterations: max iter reached (%d, res=%g)", h->hst->iter, h->hst->res);
+        return TBFG_ENOTCONVERGED;
+    }
+    if (!h->hst->done) {
+        set_error("PCG: not converged within the %d iterations queued (res=%g)", h->hst->iter, h->hst->res);
+        return TBFG_ENOTCONVERGED;
+    }
+    return TBFG_OK;
+}
+
 // T: the multigrid's precision; FFT: the preconditioner is the FFT solve of
 // the constant-coefficient operator (z in FP64) instead of the V-cycle; T0:
-// the storage type of the finest level's multigrid q and beta (T or bf16s)
+// the storage type of the finest level's multigrid q and beta (T or bf16s).
+// queue > 0: enqueue exactly that many iterations and return at once, with no
+// host synchronisation and no host memory in the stream (a solve a caller can
+// capture in a graph); the device's stop test still ends the solve (iterations
+// queued after it return at once) and pcg_result reads the outcome.
 template <class T, bool FFT = false, class T0 = T>
 static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t* iters, double* res,
-                   void* stream)
+                   void* stream, int32_t queue = 0)
 {
     using TZ = typename std::conditional<FFT, double, T>::type;
     hipStream_t s = (hipStream_t)stream;
     PcgState* st = h->st.as<PcgState>();
     const int* done = &st->done;
     Reducer red{h->partials.as<double>()};
-    std::memset(h->hst, 0, sizeof(PcgState));
-    h->hst->tol = h->prm.tol;
-    h->hst->max_iter = h->prm.max_iter;
-    h->hst->flexible = h->prm.flexible;
-    TBFG_HIP(hipMemcpyAsync(st, h->hst, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    if (queue > 0 && (h->prm.mg_tol > 0.0 || h->kin >= 0 || h->prof_kernel >= 0 || h->prm.n_pre == 0)) {
+        set_error("tbfg_pcg_solve_queued_dev: this handle's solve synchronises with the host (tolMG loop, "
+                  "coarsest level outside the tail kernel, no pre-smoothing, or kernel profiling)");
+        return TBFG_EINVAL;
+    }
+    TBFG_TRY(launch_state_init(st, h->prm.tol, h->prm.max_iter, h->prm.flexible, stream));
     if (!FFT) TBFG_TRY(setup_levels<T, T0>(h, beta, stream));
     const T0* beta0;
     if constexpr (sizeof(T) == 4)
@@ -810,6 +845,12 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
         TBFG_HIP(hipEventRecord(h->hev[slot], s));
         return TBFG_OK;
     };
+    if (queue > 0) {
+        for (int32_t it = 1; it <= queue; ++it) TBFG_TRY(iteration(it == 1 ? 1 : 0));
+        if (h->fuse_op) TBFG_TRY(launch_psi_last(n, psi, dbuf[0], dbuf[1], st, stream));
+        h->queued_fft = FFT;
+        return TBFG_OK;
+    }
     // The host examines the state after iteration i while iteration i+1 is
     // already queued, so the device does not idle through the stop test; the
     // kernels of an iteration queued after convergence return at once.
@@ -826,26 +867,7 @@ static int solve_t(tbfg_pcg* h, double* psi, const double* beta, const double* q
     TBFG_HIP(hipMemcpyAsync(h->hst, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     TBFG_HIP(hipStreamSynchronize(s));
     if (h->prof_kernel >= 0) prof_harvest(h);
-    if (FFT) {
-        h->hst->mg_cyc_total = h->hst->mg_cyc_max = 0;
-    } else if (!(h->prm.mg_tol > 0.0)) {
-        // fixed cycles: the first application and one after every update but
-        // the last (whose stop test comes before it; the reference applies
-        // the preconditioner once more before testing, pcg.f90:122-136)
-        h->hst->mg_cyc_total = h->hst->iter * h->prm.mg_cycles;
-        h->hst->mg_cyc_max = h->hst->iter > 0 ? h->prm.mg_cycles : 0;
-    }
-    if (iters) *iters = h->hst->iter;
-    if (res) *res = h->hst->res;
-    if (h->hst->status == TBFG_ENUMERIC) {
-        set_error("PCG: non-finite value (res=%g) after %d iterations", h->hst->res, h->hst->iter);
-        return TBFG_ENUMERIC;
-    }
-    if (h->hst->status == TBFG_ENOTCONVERGED) {
-        set_error("EXIT PCG iterations: max iter reached (%d, res=%g)", h->hst->iter, h->hst->res);
-        return TBFG_ENOTCONVERGED;
-    }
-    return TBFG_OK;
+    return finish_state(h, FFT, iters, res);
 }
 
 int pcg_solve_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t* iters, double* res,
@@ -860,6 +882,34 @@ int pcg_solve_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q,
     if (h->b16) return solve_t<float, false, bf16s>(h, psi, beta, q, iters, res, stream);
     return h->mixed ? solve_t<float>(h, psi, beta, q, iters, res, stream)
                     : solve_t<double>(h, psi, beta, q, iters, res, stream);
+}
+
+int pcg_solve_queued_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t iterations,
+                         void* stream)
+{
+    if (!h || !psi || !beta || !q) {
+        set_error("tbfg_pcg_solve_queued_dev: NULL argument");
+        return TBFG_EINVAL;
+    }
+    if (iterations < 1 || iterations > h->prm.max_iter) {
+        set_error("tbfg_pcg_solve_queued_dev: %d iterations queued (expected 1..max_iter = %d)", iterations,
+                  h->prm.max_iter);
+        return TBFG_EINVAL;
+    }
+    if (h->fps) return solve_t<double, true>(h, psi, beta, q, nullptr, nullptr, stream, iterations);
+    if (h->b16) return solve_t<float, false, bf16s>(h, psi, beta, q, nullptr, nullptr, stream, iterations);
+    return h->mixed ? solve_t<float>(h, psi, beta, q, nullptr, nullptr, stream, iterations)
+                    : solve_t<double>(h, psi, beta, q, nullptr, nullptr, stream, iterations);
+}
+
+int pcg_result(tbfg_pcg* h, int32_t* iters, double* res)
+{
+    if (!h) {
+        set_error("tbfg_pcg_result: NULL handle");
+        return TBFG_EINVAL;
+    }
+    TBFG_HIP(hipMemcpy(h->hst, h->st.p, sizeof(PcgState), hipMemcpyDeviceToHost));
+    return finish_state(h, h->queued_fft, iters, res);
 }
 
 template <class T, class T0 = T>

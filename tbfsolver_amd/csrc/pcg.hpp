@@ -30,6 +30,7 @@ struct tbfg_pcg {
     size_t tail_lds = 0;                // dynamic LDS of the LDS-resident tail kernel (0: HBM form)
     tbfg::OpPlan opp;                   // launch of the finest level's operator product
     bool fuse_op = false;               // direction + operator + deferred psi update in one pass (d2: the pair)
+    bool queued_fft = false;            // the last queued solve used the FFT preconditioner (pcg_result)
     tbfg::DevBuf r, d, d2, qv;          // CG vectors (FP64)
     tbfg::DevBuf r32;                   // FP32 (b16: bf16) copy of r, the multigrid's right-hand side (mixed)
     tbfg::DevBuf st, partials, counter; // PcgState + reduction scratch
@@ -63,6 +64,9 @@ namespace tbfg {
 int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm, tbfg_pcg** out);
 int pcg_solve_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t* iters, double* res,
                   void* stream);
+int pcg_solve_queued_dev(tbfg_pcg* h, double* psi, const double* beta, const double* q, int32_t iterations,
+                         void* stream);
+int pcg_result(tbfg_pcg* h, int32_t* iters, double* res);
 int pcg_precondition_dev(tbfg_pcg* h, const double* beta, const double* r, double* z, void* stream);
 int pcg_profile_begin(tbfg_pcg* h, int kernel);
 int pcg_profile_end(tbfg_pcg* h, int64_t* launches, double* total_ms, double* bytes_per_launch);

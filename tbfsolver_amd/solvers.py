@@ -193,6 +193,28 @@ class PCG:
         self.iter_ = self.iterations + 1 if self.converged else self.iterations
         return psi
 
+    def solvePCG_queued(self, psi, beta, q, iterations, stream=None):
+        """solvePCG with `iterations` iterations enqueued and no host synchronisation
+        (device tensors; capturable in a graph).  The device's stop test still ends the
+        solve; result() reads the outcome once the stream's work has finished."""
+        for x in (psi, beta, q):
+            _check_dev(x, self.grid.cells)
+        check(lib().tbfg_pcg_solve_queued_dev(self._h, ptr(psi), ptr(beta), ptr(q), int(iterations),
+                                              stream_ptr(stream)))
+        return psi
+
+    def result(self):
+        """The outcome of the last queued solve: sets iterations, res_, converged, iter_."""
+        it = C.c_int32(0)
+        res = C.c_double(0.0)
+        rc = lib().tbfg_pcg_result(self._h, C.byref(it), C.byref(res))
+        check(rc, allow=(_lib.TBFG_ENOTCONVERGED,))
+        self.iterations = int(it.value)
+        self.res_ = float(res.value)
+        self.converged = rc == _lib.TBFG_OK
+        self.iter_ = self.iterations + 1 if self.converged else self.iterations
+        return self.converged
+
     def precondition(self, beta, r, z, stream=None):
         """z = M r, one application of the multigrid preconditioner (device tensors)."""
         for x in (beta, r, z):

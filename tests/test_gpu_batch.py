@@ -62,3 +62,38 @@ def test_config3_all_instances_concurrent_equal_solo_and_oracle(torch_gpu):
     print("  worst rel L2 error vs oracle:", worst, flush=True)
     for inst in insts:
         inst.close()
+
+
+@pytest.mark.gpu
+def test_config3_one_graph_launch_per_pass_equals_solo(torch_gpu):
+    """All 64 instances of the batched workload captured in one HIP graph (one
+    launch per pass, 8 parallel branches; each PCG's iterations queued on the
+    device, its stop test on the device).  Three replays, with eager work that
+    overwrites every output and synchronisations between them: each replay equals
+    the instances solved one at a time on the host-driven path, bit for bit, with
+    the same iteration counts."""
+    specs = Bt.config3_instances(64, 1003)
+    print("case batch graph, 64 instances", flush=True)
+    insts = [Bt.Instance(*x) for x in specs]
+    solo = []
+    for inst in insts:
+        inst.solve()
+        torch_gpu.cuda.synchronize()
+        solo.append((inst.iterations, to_host(inst.psi)))
+    bg = Bt.BatchGraph(insts, branches=8)
+    assert len(bg.queue) == 32 and all(k > 0 for k in bg.queue.values())
+    for rep in range(3):
+        for inst in insts:
+            inst.psi.fill_(float("nan"))
+        torch_gpu.cuda.synchronize()
+        bg.replay()
+        torch_gpu.cuda.synchronize()
+        assert bg.harvest() == [], rep
+        for inst, (it, psi) in zip(insts, solo):
+            if inst.kind == "MG":
+                assert inst.iterations == it, (rep, inst.index)
+            assert np.array_equal(to_host(inst.psi), psi), (rep, inst.index)
+        print("  replay", rep, "equal", flush=True)
+    bg.close()
+    for inst in insts:
+        inst.close()
