@@ -227,6 +227,39 @@ def serving_leg(torch, rates=(50, 100, 200, 500, 1000, 2000), workers=8, fp64_ra
     return out
 
 
+def streamed_bytes_per_iteration(dims, precision, npre=2, npost=2, tail_cells=4096):
+    """Bytes one PCG iteration's kernels must move (each input read once, each
+    output written once), kernel by kernel: the operator pass (z, d_old, beta,
+    psi in; d, qv, psi out), the residual update (r, qv in; r out; the mixed
+    forms' copy of r), and per multigrid level above the tail kernel the sweeps
+    (the first from zero; the finest level's last one also reads r and qv for the
+    dot products), the residual-restriction and the prolongation.  Levels of at
+    most tail_cells cells live in LDS inside one kernel and are left out."""
+    n = float(np.prod(dims))
+    if precision == "fp64":
+        bp, bq, bz, bcopy = 8.0, 8.0, 8.0, 0.0
+    elif precision == "mixed":
+        bp, bq, bz, bcopy = 4.0, 4.0, 4.0, 4.0
+    else:  # mixed-bf16: q and beta of the finest level in bf16
+        bp, bq, bz, bcopy = 4.0, 2.0, 4.0, 2.0
+    total = (bz + 48.0) * n + (24.0 + bcopy) * n  # operator pass; update
+
+    def level(cells, bpl, bql, finest):
+        first = 2 * bql + bpl
+        plain = 2 * bql + 2 * bpl
+        sweeps = first + (npre - 1) * plain + npost * plain + (16.0 if finest else 0.0)
+        rr = 2 * bql + bpl + bpl / 8.0
+        prolong = 2 * bpl + bpl / 8.0
+        return cells * (sweeps + rr + prolong)
+
+    total += level(n, bp, bq, True)
+    cells = n / 8.0
+    while cells > tail_cells:
+        total += level(cells, bp, bp, False)
+        cells /= 8.0
+    return total
+
+
 def fast_leg(torch, T, g, like, q, reps=20):
     """FAST_MODE solve (FFT x/z + tridiagonal y) of s = q on the headline grid: FP64, and
     FP32 with two FP64 refinement sweeps; mean wall time per solve over reps solves."""
@@ -530,6 +563,24 @@ def main():
                acc.get("fp64_solves_per_s")))
         out["speedup_mixed_vs_fp64"] = round(value / v64, 3)
         out["error_vs_fp64_1e-12"] = {"mixed_rel_l2": err_l2, "mixed_rel_max": err_max, "fp64_rel_l2": err64}
+        # each form at the smoothing that suits it best (the headline keeps the configured 2 + 2)
+        best = {"mixed_bf16": {"nu": [2, 2], "solves_per_s": round(value, 3), "iterations": out["iterations"]},
+                "fp64": {"nu": [2, 2], "solves_per_s": round(v64, 3), "iterations": int(np.median(it64))}}
+        for prec, key, nus in (("mixed-bf16", "mixed_bf16", ((3, 3), (4, 4))), ("fp64", "fp64", ((3, 2),))):
+            for nu in nus:
+                sv = T.PCG(g, nLevels=4, nPreSweep=nu[0], nPostSweep=nu[1], tolPCG=1e-8, maxIterPCG=200,
+                           precision=prec, flexible=1, coarse_max_iter=0)
+                kt = max(3, args.steps // 3)
+                tt, _, itt, _ = timed_solves(torch, sv, torch.zeros_like(drho), beta, q, kt, 1)
+                sv.close()
+                if kt / tt > best[key]["solves_per_s"]:
+                    best[key] = {"nu": list(nu), "solves_per_s": round(kt / tt, 3), "iterations": int(np.median(itt))}
+        best["ratio"] = round(best["mixed_bf16"]["solves_per_s"] / best["fp64"]["solves_per_s"], 3)
+        best["tried"] = "mixed-bf16 nu (2,2), (3,3), (4,4); fp64 (2,2), (3,2); to 1e-8 like the headline"
+        out["own_optimum"] = best
+        log("own optimum: mixed nu %s %.2f solves/s (%d it), fp64 nu %s %.2f solves/s (%d it), ratio %.3f"
+            % (best["mixed_bf16"]["nu"], best["mixed_bf16"]["solves_per_s"], best["mixed_bf16"]["iterations"],
+               best["fp64"]["nu"], best["fp64"]["solves_per_s"], best["fp64"]["iterations"], best["ratio"]))
         log("fp64: %.2f solves/s, iterations %s; mixed err %.2e (L2) %.2e (max); reference algorithm %.2f/s"
             % (v64, sorted(set(it64)), err_l2, err_max, 2 / totr))
         # the constant-coefficient FFT solve as the preconditioner (FP32 transforms)
@@ -602,8 +653,16 @@ def main():
         mg = sum(n0 / 8 ** l for l in range(1, 12)) * (4 * 16 + 29.0) + n0 * (8 + 3 * 12 + 23.0)
         per_it = 128.0 * n0 + mg
         solve_gbs = out["iterations"] * per_it / (out["p50_ms"] * 1e-3) / 1e9
+        streamed = streamed_bytes_per_iteration(dims, "mixed-bf16")
+        str_gbs = out["iterations"] * streamed / (out["p50_ms"] * 1e-3) / 1e9
         out["solve_roofline"] = {"bytes_per_iteration": per_it, "achieved": round(solve_gbs, 1), "unit": "GB/s",
-                                 "frac": round(solve_gbs / HBM_PEAK_GBS, 4)}
+                                 "frac": round(solve_gbs / HBM_PEAK_GBS, 4),
+                                 "model": "the survey's algorithmic model (unfused CG vectors 128 N)",
+                                 "streamed_bytes_per_iteration": streamed,
+                                 "streamed_achieved": round(str_gbs, 1),
+                                 "streamed_frac": round(str_gbs / HBM_PEAK_GBS, 4),
+                                 "streamed_model": "bytes the fused kernels of this form must move, kernel by "
+                                                   "kernel (bench.streamed_bytes_per_iteration)"}
 
     if not args.no_batch:
         out["config3"] = batch_leg(torch)
