@@ -63,6 +63,25 @@ static int get_geo(const tbfg_grid_desc* g, FieldGeo& out)
     return TBFG_OK;
 }
 
+// (i, j, k) of a thread in a row-wise launch made as a 1-D grid of
+// ceil(nx / blockDim.x) * ny * nz workgroups.  Workgroups go round-robin to the 8
+// XCDs, each with its own L2, so consecutive ones are renumbered to fall on one
+// XCD: each XCD then works through a contiguous range of rows and planes, and the
+// y and z neighbours a workgroup reads were read into that XCD's L2 by its
+// neighbours (a 3-D grid numbered rows first sends neighbouring rows to different
+// XCDs, and each of them fetches the shared faces from HBM again).
+__device__ __forceinline__ void row_cell(int nx, int ny, int& i, int& j, int& k)
+{
+    const unsigned nb = gridDim.x;
+    unsigned id = blockIdx.x;
+    if ((nb & 7u) == 0) id = (id & 7u) * (nb >> 3) + (id >> 3);
+    const unsigned gx = ((unsigned)nx + blockDim.x - 1) / blockDim.x;
+    const unsigned bx = id % gx, r = id / gx;
+    i = (int)(bx * blockDim.x + threadIdx.x);
+    j = (int)(r % (unsigned)ny);
+    k = (int)(r / (unsigned)ny);
+}
+
 __device__ __forceinline__ double div_cell(const FieldGeo& G, const double* __restrict__ ux,
                                            const double* __restrict__ uy, const double* __restrict__ uz, int i, int j,
                                            int k)
@@ -83,9 +102,8 @@ __device__ __forceinline__ double div_cell(const FieldGeo& G, const double* __re
 __global__ void k_divergence(FieldGeo G, const double* __restrict__ ux, const double* __restrict__ uy,
                              const double* __restrict__ uz, double* __restrict__ div)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    row_cell(G.nx, G.ny, i, j, k);
     if (i >= G.nx) return;
     div[i + (size_t)G.nx * (j + (size_t)G.ny * k)] = div_cell(G, ux, uy, uz, i, j, k);
 }
@@ -94,9 +112,8 @@ __global__ void k_divergence(FieldGeo G, const double* __restrict__ ux, const do
 __global__ void k_maxdiv(FieldGeo G, const double* __restrict__ ux, const double* __restrict__ uy,
                          const double* __restrict__ uz, unsigned long long* out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    row_cell(G.nx, G.ny, i, j, k);
     double v = 0.0;
     if (i < G.nx) v = fabs(div_cell(G, ux, uy, uz, i, j, k));
     if (!(v == v)) v = __longlong_as_double(0x7ff0000000000000LL);  // NaN -> +inf
@@ -126,9 +143,8 @@ __global__ void k_project_mg(FieldGeo G, double alpha_dt, const double* __restri
                              const double* __restrict__ psi, double* __restrict__ ux, double* __restrict__ uy,
                              double* __restrict__ uz)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    row_cell(G.nx, G.ny, i, j, k);
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     if (i >= nx) return;
     const size_t c = i + (size_t)nx * (j + (size_t)ny * k);
@@ -186,9 +202,8 @@ __global__ void k_source_fast(FieldGeo G, double r_adt, double rho0, const doubl
                               const double* __restrict__ uz, const double* __restrict__ psi1,
                               const double* __restrict__ psi2, StLevels st, double* __restrict__ s)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    row_cell(G.nx, G.ny, i, j, k);
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     if (i >= nx) return;
     const size_t plane = (size_t)nx * ny;
@@ -232,9 +247,8 @@ __global__ void k_project_fast(FieldGeo G, double alpha_dt, double rho0, const d
                                const double* __restrict__ psi2, StLevels st, double* __restrict__ ux,
                                double* __restrict__ uy, double* __restrict__ uz)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int k = blockIdx.z;
+    int i, j, k;
+    row_cell(G.nx, G.ny, i, j, k);
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     if (i >= nx) return;
     const size_t plane = (size_t)nx * ny;
@@ -316,7 +330,7 @@ int divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* uy, 
     FieldGeo G;
     TBFG_TRY(get_geo(g, G));
     dim3 blk = row_block(G.nx);
-    hipLaunchKernelGGL(k_divergence, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G, ux, uy,
+    hipLaunchKernelGGL(k_divergence, dim3(ceil_div(G.nx, blk.x) * G.ny * G.nz), blk, 0, (hipStream_t)stream, G, ux, uy,
                        uz, div);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
@@ -336,7 +350,7 @@ int max_divergence_dev(const tbfg_grid_desc* g, const double* ux, const double* 
     hipStream_t s = (hipStream_t)stream;
     TBFG_HIP(hipMemsetAsync(res.p, 0, sizeof(unsigned long long), s));
     dim3 blk = row_block(G.nx);
-    hipLaunchKernelGGL(k_maxdiv, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, s, G, ux, uy, uz,
+    hipLaunchKernelGGL(k_maxdiv, dim3(ceil_div(G.nx, blk.x) * G.ny * G.nz), blk, 0, s, G, ux, uy, uz,
                        res.as<unsigned long long>());
     TBFG_HIP(hipGetLastError());
     unsigned long long bits = 0;
@@ -428,7 +442,7 @@ int max_divergence_noalloc_dev(const tbfg_grid_desc* g, const double* ux, const 
     hipError_t e = hipMemcpyAsync(d->ptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
         const dim3 blk = row_block(G.nx);
-        hipLaunchKernelGGL(k_maxdiv, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, s, G, ux, uy, uz,
+        hipLaunchKernelGGL(k_maxdiv, dim3(ceil_div(G.nx, blk.x) * G.ny * G.nz), blk, 0, s, G, ux, uy, uz,
                            reinterpret_cast<unsigned long long*>(d->ptr));
         e = hipGetLastError();
     }
@@ -486,7 +500,7 @@ int source_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, const
     }
     const StLevels st = st_levels(st1, st2);
     dim3 blk = row_block(G.nx);
-    hipLaunchKernelGGL(k_source_fast, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G,
+    hipLaunchKernelGGL(k_source_fast, dim3(ceil_div(G.nx, blk.x) * G.ny * G.nz), blk, 0, (hipStream_t)stream, G,
                        1.0 / alpha_dt, rho0, rho, ux, uy, uz, psi1, psi2, st, s);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
@@ -512,7 +526,7 @@ int project_fast_dev(const tbfg_grid_desc* g, double alpha_dt, double rho0, cons
     }
     const StLevels st = st_levels(st1, st2);
     dim3 blk = row_block(G.nx);
-    hipLaunchKernelGGL(k_project_fast, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G,
+    hipLaunchKernelGGL(k_project_fast, dim3(ceil_div(G.nx, blk.x) * G.ny * G.nz), blk, 0, (hipStream_t)stream, G,
                        alpha_dt, rho0, rho, psi, psi1, psi2, st, ux, uy, uz);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
@@ -564,7 +578,7 @@ int project_mg_dev(const tbfg_grid_desc* g, double alpha_dt, const double* rho, 
     FieldGeo G;
     TBFG_TRY(get_geo(g, G));
     dim3 blk = row_block(G.nx);
-    hipLaunchKernelGGL(k_project_mg, dim3(ceil_div(G.nx, blk.x), G.ny, G.nz), blk, 0, (hipStream_t)stream, G,
+    hipLaunchKernelGGL(k_project_mg, dim3(ceil_div(G.nx, blk.x) * G.ny * G.nz), blk, 0, (hipStream_t)stream, G,
                        alpha_dt, rho, psi, ux, uy, uz);
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
