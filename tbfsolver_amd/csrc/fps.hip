@@ -258,6 +258,18 @@ __global__ void __launch_bounds__(kFftThreads)
     }
 }
 
+// A 1-D grid's workgroup index with consecutive workgroups on one XCD
+// (workgroups go round-robin to the 8 XCDs, each with its own L2): the z passes
+// number the y chunks of one kx consecutively, so the chunks that write (or
+// read) the neighbouring runs of a column share an L2 instead of each XCD
+// moving partial lines.
+__device__ __forceinline__ unsigned xcd_block()
+{
+    const unsigned nb = gridDim.x;
+    const unsigned b = blockIdx.x;
+    return (nb & 7u) == 0 ? (b & 7u) * (nb >> 3) + (b >> 3) : b;
+}
+
 // K2: A[kx][j0..j0+JT-1][:] -> B[kx][kz][j]
 template <class T>
 __global__ void __launch_bounds__(kFftThreads)
@@ -269,8 +281,9 @@ __global__ void __launch_bounds__(kFftThreads)
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
     const int LS = nz + 1;  // padded line stride (fft_lines)
     C2<T>* twl = buf + JT * LS;
-    const int m = blockIdx.y;
-    const int j0 = blockIdx.x * JT;
+    const unsigned id = xcd_block(), gj = (unsigned)(ny / JT);
+    const int m = (int)(id / gj);
+    const int j0 = (int)(id % gj) * JT;
     load_twiddles(twl, tw, nz);
     const C2<T>* src = A + ((size_t)m * ny + j0) * nz;
     for (int t0 = threadIdx.x; t0 < JT * nz; t0 += kStageBatch * kFftThreads) {
@@ -305,8 +318,9 @@ __global__ void __launch_bounds__(kFftThreads)
     C2<T>* buf = reinterpret_cast<C2<T>*>(smem);
     const int LS = nz + 1;  // padded line stride (fft_lines)
     C2<T>* twl = buf + JT * LS;
-    const int m = blockIdx.y;
-    const int j0 = blockIdx.x * JT;
+    const unsigned id = xcd_block(), gj = (unsigned)(ny / JT);
+    const int m = (int)(id / gj);
+    const int j0 = (int)(id % gj) * JT;
     load_twiddles(twl, tw, nz);
     for (int t0 = threadIdx.x; t0 < JT * nz; t0 += kStageBatch * kFftThreads) {
         C2<T> v[kStageBatch];
@@ -879,7 +893,8 @@ static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipSt
     const size_t ldz = ((size_t)h->JT * (nz + 1) + nz) * sizeof(C2<T>);
     hipLaunchKernelGGL((k_xfwd<T, TIN>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px,
                        twx, in, A);
-    hipLaunchKernelGGL((k_zfwd<T>), dim3(ny / h->JT, nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz, A, B);
+    hipLaunchKernelGGL((k_zfwd<T>), dim3((ny / h->JT) * nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz, A,
+                       B);
     switch (h->MR) {
         case 1: launch_ysolve<T, 1>(h, B, s); break;
         case 2: launch_ysolve<T, 2>(h, B, s); break;
@@ -887,7 +902,8 @@ static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipSt
         case 8: launch_ysolve<T, 8>(h, B, s); break;
         default: launch_ysolve<T, 16>(h, B, s); break;
     }
-    hipLaunchKernelGGL((k_zinv<T>), dim3(ny / h->JT, nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz, B, A);
+    hipLaunchKernelGGL((k_zinv<T>), dim3((ny / h->JT) * nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz, B,
+                       A);
     hipLaunchKernelGGL((k_xinv<T>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px, twx, A,
                        p, mode, h->mean.as<double>());
     TBFG_HIP(hipGetLastError());
