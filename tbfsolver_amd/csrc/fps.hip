@@ -204,7 +204,10 @@ __device__ __forceinline__ void load_twiddles(C2<T>* dst, const C2<T>* __restric
 
 // Staging loops issue the loads of kStageBatch elements per thread before the
 // first LDS store, so the HBM latency is paid once per batch rather than once
-// per element (a rolled loop of load-then-store waits on every load).
+// per element (a rolled loop of load-then-store waits on every load).  The
+// loads are unconditional (a thread past the end re-reads the last element)
+// and only the LDS stores are guarded: a guarded load is compiled as a branch
+// that waits for its own load before the next one is issued.
 template <class T>
 constexpr int stage_batch() { return sizeof(T) == 8 ? 4 : 8; }
 
@@ -227,13 +230,11 @@ __global__ void __launch_bounds__(kFftThreads)
         TIN v0[kStageBatch], v1[kStageBatch];
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
-            const int t = t0 + u * kFftThreads;
-            if (t < L * nx) {
-                const int l = t / nx, i = t - l * nx;
-                const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
-                v0[u] = s[o0];
-                v1[u] = s[o0 + (size_t)nx * ny];
-            }
+            const int t = min(t0 + u * kFftThreads, L * nx - 1);
+            const int l = t / nx, i = t - l * nx;
+            const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
+            v0[u] = s[o0];
+            v1[u] = s[o0 + (size_t)nx * ny];
         }
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
@@ -287,16 +288,20 @@ __global__ void __launch_bounds__(kFftThreads)
     load_twiddles(twl, tw, nz);
     const C2<T>* src = A + ((size_t)m * ny + j0) * nz;
     for (int t0 = threadIdx.x; t0 < JT * nz; t0 += kStageBatch * kFftThreads) {
-        C2<T> v[kStageBatch];
+        // real and imaginary parts in scalar arrays: an array of C2<double> is
+        // kept in scratch memory, each element loaded, waited for and spilled
+        T vr[kStageBatch], vi[kStageBatch];
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
-            const int t = t0 + u * kFftThreads;
-            if (t < JT * nz) v[u] = src[t];
+            const int t = min(t0 + u * kFftThreads, JT * nz - 1);
+            const C2<T> x = src[t];
+            vr[u] = x.x;
+            vi[u] = x.y;
         }
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
             const int t = t0 + u * kFftThreads;
-            if (t < JT * nz) buf[t + t / nz] = v[u];
+            if (t < JT * nz) buf[t + t / nz] = C2<T>{vr[u], vi[u]};
         }
     }
     __syncthreads();
@@ -323,21 +328,21 @@ __global__ void __launch_bounds__(kFftThreads)
     const int j0 = (int)(id % gj) * JT;
     load_twiddles(twl, tw, nz);
     for (int t0 = threadIdx.x; t0 < JT * nz; t0 += kStageBatch * kFftThreads) {
-        C2<T> v[kStageBatch];
+        T vr[kStageBatch], vi[kStageBatch];  // scalar arrays, as in k_zfwd
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
-            const int t = t0 + u * kFftThreads;
-            if (t < JT * nz) {
-                const int kz = t / JT, l = t - kz * JT;
-                v[u] = B[((size_t)m * nz + kz) * ny + j0 + l];
-            }
+            const int t = min(t0 + u * kFftThreads, JT * nz - 1);
+            const int kz = t / JT, l = t - kz * JT;
+            const C2<T> x = B[((size_t)m * nz + kz) * ny + j0 + l];
+            vr[u] = x.x;
+            vi[u] = x.y;
         }
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
             const int t = t0 + u * kFftThreads;
             if (t < JT * nz) {
                 const int kz = t / JT, l = t - kz * JT;
-                buf[l * LS + kz] = v[u];
+                buf[l * LS + kz] = C2<T>{vr[u], vi[u]};
             }
         }
     }
@@ -368,23 +373,24 @@ __global__ void __launch_bounds__(kFftThreads)
     // DC and Nyquist keep their real parts only, as a c2r transform does.
     constexpr int kB = kStageBatch / 2;  // two complex loads per element
     for (int t0 = threadIdx.x; t0 < nh * L; t0 += kB * kFftThreads) {
-        C2<T> v1[kB], v2[kB];
+        T x1r[kB], x1i[kB], x2r[kB], x2i[kB];  // scalar arrays, as in k_zfwd
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            const int t = t0 + u * kFftThreads;
-            if (t < nh * L) {
-                const int m = t / L, l = t - m * L;
-                const size_t o = ((size_t)m * ny + j) * nz + k0 + 2 * l;
-                v1[u] = A[o];
-                v2[u] = A[o + 1];
-            }
+            const int t = min(t0 + u * kFftThreads, nh * L - 1);
+            const int m = t / L, l = t - m * L;
+            const size_t o = ((size_t)m * ny + j) * nz + k0 + 2 * l;
+            const C2<T> a = A[o], b = A[o + 1];
+            x1r[u] = a.x;
+            x1i[u] = a.y;
+            x2r[u] = b.x;
+            x2i[u] = b.y;
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
             const int t = t0 + u * kFftThreads;
             if (t < nh * L) {
                 const int m = t / L, l = t - m * L;
-                const C2<T> x1 = v1[u], x2 = v2[u];
+                const C2<T> x1{x1r[u], x1i[u]}, x2{x2r[u], x2i[u]};
                 if (m == 0 || 2 * m == nx) {
                     buf[l * LS + m] = C2<T>{x1.x, x2.x};
                 } else {
