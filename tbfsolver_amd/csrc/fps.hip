@@ -58,6 +58,7 @@ struct FftPlan {
     int n;
     int nrad;
     int rad[16];
+    int swz;           // XOR mask of the LDS line swizzle (lds_pos), 0 when n % 8 != 0
     uint32_t mnb[16];  // ceil(2^32 / (n / rad[s]))
     uint32_t mns[16];  // ceil(2^32 / Ns), Ns = rad[0] * ... * rad[s-1]
 };
@@ -81,6 +82,7 @@ static int make_plan(int n, FftPlan& p)
         p.rad[p.nrad++] = 3;
         m /= 3;
     }
+    p.swz = n % 8 == 0 ? 6 : 0;
     int Ns = 1;
     for (int st = 0; st < p.nrad; ++st) {
         p.mnb[st] = div_magic(n / p.rad[st]);
@@ -131,22 +133,35 @@ __device__ __forceinline__ void dft3(C2<T>* v)
     v[2] = csub(t2, t3);
 }
 
-// Stockham autosort FFT of `nlines` lines of length N held in LDS (line l at
-// buf + l*(N+1): one element of padding per line, so that reading or writing
-// the same index of consecutive lines -- the transposed side of the passes'
-// loads and stores -- falls in different LDS banks); tw[t] = exp(-2 pi i t / N).
-// Each pass reads all its butterfly inputs into registers, synchronises, then
-// writes.
+// LDS position of element i of line l.  Lines are N+1 elements apart, so the
+// same index of consecutive lines -- the transposed side of the passes' loads
+// and stores -- falls in different banks.  Within a line, index bits 1-2 are
+// XORed with (bit 4 ^ bit 5) * swz: a permutation inside each aligned group of
+// 8 (contiguous accesses keep their banks) that takes the bank conflicts out of
+// the Stockham writes, whose lanes step by R*Ns elements while Ns < 8 (gfx950
+// serves 16-byte accesses 8 lanes at a time over 64 four-byte banks).
+__device__ __forceinline__ int lds_pos(int l, int LS, int i, int swz)
+{
+    return l * LS + (i ^ (((i >> 4) ^ (i >> 5)) & 1) * swz);
+}
+
+// Stockham autosort FFT of `nlines` lines of length N held in LDS at lds_pos.
+// tw holds each pass's twiddles contiguously: pass s with sub-transform length
+// Ns and radix R uses tw[Ns - 1 + (r-1)*Ns + k] = exp(-2 pi i r k / (Ns R)),
+// r = 1..R-1, k = 0..Ns-1 (N-1 entries in all), so consecutive lanes read
+// consecutive entries.  Each pass reads all its butterfly inputs into
+// registers, synchronises, then writes.
 template <class T, int SIGN>
 __device__ void fft_lines(C2<T>* buf, int N, int nlines, const C2<T>* tw, const FftPlan& plan)
 {
     const int LS = N + 1;
+    const int swz = plan.swz;
     int Ns = 1;
     for (int s = 0; s < plan.nrad; ++s) {
         const int R = plan.rad[s];
         const int nb = N / R;
         const int total = nlines * nb;
-        const int stride = N / (Ns * R);
+        const C2<T>* tws = tw + Ns - 1;
         const uint32_t mnb = plan.mnb[s], mns = plan.mns[s];
         constexpr int kMaxBfly = max_bfly<T>();
         C2<T> v[kMaxBfly][4];
@@ -156,13 +171,12 @@ __device__ void fft_lines(C2<T>* buf, int N, int nlines, const C2<T>* tw, const 
             if (b < total) {
                 const int line = fast_div(b, nb, mnb), j = b - line * nb;
                 const int k = j - fast_div(j, Ns, mns) * Ns;
-                const C2<T>* src = buf + line * LS + j;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     if (r < R) {
-                        C2<T> x = src[r * nb];
+                        C2<T> x = buf[lds_pos(line, LS, j + r * nb, swz)];
                         if (r > 0 && k > 0) {
-                            C2<T> w = tw[r * k * stride];
+                            C2<T> w = tws[(r - 1) * Ns + k];
                             if (SIGN > 0) w.y = -w.y;
                             x = cmul(x, w);
                         }
@@ -185,10 +199,10 @@ __device__ void fft_lines(C2<T>* buf, int N, int nlines, const C2<T>* tw, const 
                     dft2<T, SIGN>(v[u]);
                 else
                     dft3<T, SIGN>(v[u]);
-                C2<T>* dst = buf + line * LS + jq * Ns * R + k;
+                const int d0 = jq * Ns * R + k;
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    if (r < R) dst[r * Ns] = v[u][r];
+                    if (r < R) buf[lds_pos(line, LS, d0 + r * Ns, swz)] = v[u][r];
             }
         }
         __syncthreads();
@@ -239,7 +253,10 @@ __global__ void __launch_bounds__(kFftThreads)
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
             const int t = t0 + u * kFftThreads;
-            if (t < L * nx) buf[t + t / nx] = C2<T>{(T)v0[u], (T)v1[u]};
+            if (t < L * nx) {
+                const int l = t / nx;
+                buf[lds_pos(l, LS, t - l * nx, plan.swz)] = C2<T>{(T)v0[u], (T)v1[u]};
+            }
         }
     }
     __syncthreads();
@@ -248,8 +265,8 @@ __global__ void __launch_bounds__(kFftThreads)
     for (int t = threadIdx.x; t < nh * KT; t += kFftThreads) {
         const int m = t / KT, kk = t - m * KT;
         const int l = kk >> 1;
-        const C2<T> Z = buf[l * LS + m];
-        const C2<T> Zn = buf[l * LS + (m == 0 ? 0 : nx - m)];
+        const C2<T> Z = buf[lds_pos(l, LS, m, plan.swz)];
+        const C2<T> Zn = buf[lds_pos(l, LS, m == 0 ? 0 : nx - m, plan.swz)];
         C2<T> val;
         if ((kk & 1) == 0)
             val = C2<T>{(T)0.5 * (Z.x + Zn.x), (T)0.5 * (Z.y - Zn.y)};  // (Z + conj Zn)/2
@@ -301,14 +318,17 @@ __global__ void __launch_bounds__(kFftThreads)
 #pragma unroll
         for (int u = 0; u < kStageBatch; ++u) {
             const int t = t0 + u * kFftThreads;
-            if (t < JT * nz) buf[t + t / nz] = C2<T>{vr[u], vi[u]};
+            if (t < JT * nz) {
+                const int l = t / nz;
+                buf[lds_pos(l, LS, t - l * nz, plan.swz)] = C2<T>{vr[u], vi[u]};
+            }
         }
     }
     __syncthreads();
     fft_lines<T, -1>(buf, nz, JT, twl, plan);
     for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) {
         const int kz = t / JT, l = t - kz * JT;
-        B[((size_t)m * nz + kz) * ny + j0 + l] = buf[l * LS + kz];
+        B[((size_t)m * nz + kz) * ny + j0 + l] = buf[lds_pos(l, LS, kz, plan.swz)];
     }
 }
 
@@ -342,14 +362,17 @@ __global__ void __launch_bounds__(kFftThreads)
             const int t = t0 + u * kFftThreads;
             if (t < JT * nz) {
                 const int kz = t / JT, l = t - kz * JT;
-                buf[l * LS + kz] = C2<T>{vr[u], vi[u]};
+                buf[lds_pos(l, LS, kz, plan.swz)] = C2<T>{vr[u], vi[u]};
             }
         }
     }
     __syncthreads();
     fft_lines<T, +1>(buf, nz, JT, twl, plan);
     C2<T>* dst = A + ((size_t)m * ny + j0) * nz;
-    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) dst[t] = buf[t + t / nz];
+    for (int t = threadIdx.x; t < JT * nz; t += kFftThreads) {
+        const int l = t / nz;
+        dst[t] = buf[lds_pos(l, LS, t - l * nz, plan.swz)];
+    }
 }
 
 // K5: A[kx][j][k0..k0+KT-1] -> p(:, j, k) scaled by 1/(nx nz).
@@ -392,10 +415,10 @@ __global__ void __launch_bounds__(kFftThreads)
                 const int m = t / L, l = t - m * L;
                 const C2<T> x1{x1r[u], x1i[u]}, x2{x2r[u], x2i[u]};
                 if (m == 0 || 2 * m == nx) {
-                    buf[l * LS + m] = C2<T>{x1.x, x2.x};
+                    buf[lds_pos(l, LS, m, plan.swz)] = C2<T>{x1.x, x2.x};
                 } else {
-                    buf[l * LS + m] = C2<T>{x1.x - x2.y, x1.y + x2.x};
-                    buf[l * LS + nx - m] = C2<T>{x1.x + x2.y, x2.x - x1.y};
+                    buf[lds_pos(l, LS, m, plan.swz)] = C2<T>{x1.x - x2.y, x1.y + x2.x};
+                    buf[lds_pos(l, LS, nx - m, plan.swz)] = C2<T>{x1.x + x2.y, x2.x - x1.y};
                 }
             }
         }
@@ -408,8 +431,9 @@ __global__ void __launch_bounds__(kFftThreads)
         for (int t = threadIdx.x; t < L * nx; t += kFftThreads) {
             const int l = t / nx, i = t - l * nx;
             const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
-            p[o0] = (double)buf[t + l].x * scale;
-            p[o0 + (size_t)nx * ny] = (double)buf[t + l].y * scale;
+            const C2<T> z = buf[lds_pos(l, LS, i, plan.swz)];
+            p[o0] = (double)z.x * scale;
+            p[o0 + (size_t)nx * ny] = (double)z.y * scale;
         }
         return;
     }
@@ -432,8 +456,9 @@ __global__ void __launch_bounds__(kFftThreads)
             if (t < L * nx) {
                 const int l = t / nx, i = t - l * nx;
                 const size_t o0 = i + (size_t)nx * (j + (size_t)ny * (k0 + 2 * l));
-                p[o0] = a0[u] + (double)buf[t + l].x * scale - sh;
-                p[o0 + (size_t)nx * ny] = a1[u] + (double)buf[t + l].y * scale - sh;
+                const C2<T> z = buf[lds_pos(l, LS, i, plan.swz)];
+                p[o0] = a0[u] + (double)z.x * scale - sh;
+                p[o0 + (size_t)nx * ny] = a1[u] + (double)z.y * scale - sh;
             }
         }
     }
@@ -844,16 +869,27 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
     h->ot.ap = h->tt.ap;
     h->ot.vol_y = tb + o_vy;
     h->inv_vg = 1.0 / (ly * L.ax[0].df(1) * nx * L.ax[2].df(1) * nz);
-    // twiddles exp(-2 pi i t / n) in FP64 and rounded to FP32
+    // per-pass twiddle tables (fft_lines): entry Ns-1 + (r-1)*Ns + k of the pass
+    // with sub-transform length Ns and radix R is exp(-2 pi i r k / (Ns R)), in
+    // FP64 and rounded to FP32; entry n-1 is unused
     auto tw = [&](int n, DevBuf& d64, DevBuf& d32) -> int {
-        std::vector<double> w(2 * (size_t)n);
-        std::vector<float> wf(2 * (size_t)n);
-        for (int q = 0; q < n; ++q) {
-            w[2 * q] = std::cos(2.0 * pi * q / n);
-            w[2 * q + 1] = -std::sin(2.0 * pi * q / n);
-            wf[2 * q] = (float)w[2 * q];
-            wf[2 * q + 1] = (float)w[2 * q + 1];
+        FftPlan pl;
+        make_plan(n, pl);
+        std::vector<double> w(2 * (size_t)n, 0.0);
+        std::vector<float> wf(2 * (size_t)n, 0.0f);
+        int Ns = 1;
+        for (int st = 0; st < pl.nrad; ++st) {
+            const int R = pl.rad[st];
+            for (int r = 1; r < R; ++r)
+                for (int k = 0; k < Ns; ++k) {
+                    const size_t e = (size_t)Ns - 1 + (size_t)(r - 1) * Ns + k;
+                    const int q = r * k * (n / (Ns * R));  // exponent in units of 2 pi / n
+                    w[2 * e] = std::cos(2.0 * pi * q / n);
+                    w[2 * e + 1] = -std::sin(2.0 * pi * q / n);
+                }
+            Ns *= R;
         }
+        for (size_t q = 0; q < w.size(); ++q) wf[q] = (float)w[q];
         TBFG_TRY(upload(d64, w.data(), w.size() * 8));
         TBFG_TRY(upload(d32, wf.data(), wf.size() * 4));
         return TBFG_OK;
