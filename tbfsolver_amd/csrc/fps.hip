@@ -464,6 +464,315 @@ __global__ void __launch_bounds__(kFftThreads)
     }
 }
 
+// ------------------------------------------- two-level passes for N = 256
+//
+// Lines of N = 256 = 16 x 16 skip the Stockham passes in LDS.  With
+// n = 16 n1 + n2 and k = k1 + 16 k2,
+//   X[k1 + 16 k2] = sum_n2 W16^(n2 k2) [ W256^(n2 k1) sum_n1 x[16 n1 + n2] W16^(n1 k1) ].
+// Step A: thread n2 of a line holds x[16 n1 + n2], n1 = 0..15, in registers
+// (loaded from HBM, or from the pre-processed line in LDS), runs the inner
+// 16-point DFT, multiplies by W256^(n2 k1) and writes Y[k1][n2] to LDS.
+// Step B: thread k1 reads Y[k1][0..15] and runs the outer 16-point DFT, which
+// leaves X[k1 + 16 k2] in its registers, stored straight to HBM (z passes) or
+// through one more LDS line (x passes, whose r2c/c2r packing pairs X[m] with
+// X[N-m]).  One LDS exchange per line replaces the staging, log4(N) passes and
+// output round trips of fft_lines.  8 lines per 128-thread workgroup; Y rows of
+// 17 elements and lines of 273 (= 1 mod 16), so that step A's stores
+// (consecutive n2, or consecutive lines) and step B's loads (consecutive k1, or
+// consecutive lines) fall in different banks.  Each step orders its lanes the
+// way its side of HBM is laid out: along the line where the line is contiguous,
+// across lines where the pass reads or writes the transposed layout.
+constexpr int kTlLines = 8;
+constexpr int kTlThreads = kTlLines * 16;
+constexpr int kTlRow = 17;
+constexpr int kTlLine = 16 * kTlRow + 1;          // 273 elements
+constexpr int kTlLds = kTlLines * kTlLine + 256;  // Y (or the x passes' 8 lines of 257) + twiddles
+
+// a * W16^E; W16 = exp(SIGN 2 pi i / 16), E = 0..9
+template <class T, int SIGN, int E>
+__device__ __forceinline__ C2<T> tw16_mul(C2<T> a)
+{
+    constexpr double c[10] = {1.0, 0.923879532511286756128, 0.707106781186547524401, 0.382683432365089771728, 0.0,
+                              -0.382683432365089771728, -0.707106781186547524401, -0.923879532511286756128, -1.0,
+                              -0.923879532511286756128};
+    constexpr double s[10] = {0.0, 0.382683432365089771728, 0.707106781186547524401, 0.923879532511286756128, 1.0,
+                              0.923879532511286756128, 0.707106781186547524401, 0.382683432365089771728, 0.0,
+                              -0.382683432365089771728};
+    if constexpr (E == 0) {
+        return a;
+    } else if constexpr (E == 4) {
+        return SIGN < 0 ? C2<T>{a.y, -a.x} : C2<T>{-a.y, a.x};  // a * (-i) or a * (+i)
+    } else {
+        const T wr = (T)c[E], wi = (T)(SIGN * s[E]);
+        return C2<T>{a.x * wr - a.y * wi, a.x * wi + a.y * wr};
+    }
+}
+
+// 16-point DFT in registers, natural order in and out (4 x 4 Cooley-Tukey)
+template <class T, int SIGN>
+__device__ __forceinline__ void dft16(C2<T> (&v)[16])
+{
+    C2<T> a[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        C2<T> t[4] = {v[q], v[4 + q], v[8 + q], v[12 + q]};
+        dft4<T, SIGN>(t);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[q][k] = t[k];
+    }
+    a[1][1] = tw16_mul<T, SIGN, 1>(a[1][1]);
+    a[1][2] = tw16_mul<T, SIGN, 2>(a[1][2]);
+    a[1][3] = tw16_mul<T, SIGN, 3>(a[1][3]);
+    a[2][1] = tw16_mul<T, SIGN, 2>(a[2][1]);
+    a[2][2] = tw16_mul<T, SIGN, 4>(a[2][2]);
+    a[2][3] = tw16_mul<T, SIGN, 6>(a[2][3]);
+    a[3][1] = tw16_mul<T, SIGN, 3>(a[3][1]);
+    a[3][2] = tw16_mul<T, SIGN, 6>(a[3][2]);
+    a[3][3] = tw16_mul<T, SIGN, 9>(a[3][3]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        C2<T> t[4] = {a[0][k], a[1][k], a[2][k], a[3][k]};
+        dft4<T, SIGN>(t);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[k + 4 * q] = t[q];
+    }
+}
+
+// v * w (SIGN = -1) or v * conj(w) (SIGN = +1)
+template <class T, int SIGN>
+__device__ __forceinline__ C2<T> tw_apply(C2<T> v, C2<T> w)
+{
+    if (SIGN > 0) w.y = -w.y;
+    return cmul(v, w);
+}
+
+// Step A's tail: twiddle W256^(n2 k1) and store Y[k1][n2] of line l
+template <class T, int SIGN>
+__device__ __forceinline__ void tl_store_y(C2<T>* Y, const C2<T>* twl, int l, int n2, const C2<T> (&v)[16])
+{
+    C2<T>* y = Y + l * kTlLine + n2;
+    y[0] = v[0];
+#pragma unroll
+    for (int k1 = 1; k1 < 16; ++k1) y[k1 * kTlRow] = tw_apply<T, SIGN>(v[k1], twl[k1 * 16 + n2]);
+}
+
+// K2 for nz = 256: A[kx][j][:] -> B[kx][kz][j]
+template <class T>
+__global__ void __launch_bounds__(kTlThreads)
+    k_zfwd_tl(int ny, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ A, C2<T>* __restrict__ B)
+{
+    constexpr int N = 256;
+    __shared__ C2<T> Y[kTlLds];
+    C2<T>* twl = Y + kTlLines * kTlLine;
+    const unsigned id = xcd_block(), gj = (unsigned)(ny / kTlLines);
+    const int m = (int)(id / gj);
+    const int j0 = (int)(id % gj) * kTlLines;
+    const int tid = threadIdx.x;
+    const C2<T> w0 = tw[tid], w1 = tw[tid + kTlThreads];  // first: waiting for them leaves the line loads in flight
+    {
+        const int l = tid >> 4, n2 = tid & 15;  // lanes along the line
+        const C2<T>* src = A + ((size_t)m * ny + j0 + l) * N + n2;
+        C2<T> v[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) v[n1] = src[16 * n1];
+        twl[tid] = w0;
+        twl[tid + kTlThreads] = w1;
+        __syncthreads();
+        dft16<T, -1>(v);
+        tl_store_y<T, -1>(Y, twl, l, n2, v);
+    }
+    __syncthreads();
+    {
+        const int l = tid & (kTlLines - 1), k1 = tid / kTlLines;  // lanes across lines
+        const C2<T>* y = Y + l * kTlLine + k1 * kTlRow;
+        C2<T> v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = y[n2];
+        dft16<T, -1>(v);
+        C2<T>* dst = B + ((size_t)m * N + k1) * ny + j0 + l;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) dst[(size_t)16 * k2 * ny] = v[k2];
+    }
+}
+
+// K4 for nz = 256: B[kx][kz][j] -> A[kx][j][:]
+template <class T>
+__global__ void __launch_bounds__(kTlThreads)
+    k_zinv_tl(int ny, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ B, C2<T>* __restrict__ A)
+{
+    constexpr int N = 256;
+    __shared__ C2<T> Y[kTlLds];
+    C2<T>* twl = Y + kTlLines * kTlLine;
+    const unsigned id = xcd_block(), gj = (unsigned)(ny / kTlLines);
+    const int m = (int)(id / gj);
+    const int j0 = (int)(id % gj) * kTlLines;
+    const int tid = threadIdx.x;
+    const C2<T> w0 = tw[tid], w1 = tw[tid + kTlThreads];
+    {
+        const int l = tid & (kTlLines - 1), n2 = tid / kTlLines;  // lanes across lines
+        const C2<T>* src = B + ((size_t)m * N + n2) * ny + j0 + l;
+        C2<T> v[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) v[n1] = src[(size_t)16 * n1 * ny];
+        twl[tid] = w0;
+        twl[tid + kTlThreads] = w1;
+        __syncthreads();
+        dft16<T, +1>(v);
+        tl_store_y<T, +1>(Y, twl, l, n2, v);
+    }
+    __syncthreads();
+    {
+        const int k1 = tid & 15, l = tid >> 4;  // lanes along the line
+        const C2<T>* y = Y + l * kTlLine + k1 * kTlRow;
+        C2<T> v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = y[n2];
+        dft16<T, +1>(v);
+        C2<T>* dst = A + ((size_t)m * ny + j0 + l) * N + k1;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) dst[16 * k2] = v[k2];
+    }
+}
+
+// K1 for nx = 256: s(:, j, k0..k0+15) -> A[kx][j][k], kx = 0..128 (line l packs
+// the real lines k0+2l and k0+2l+1 into one complex line, as k_xfwd)
+template <class T, class TIN>
+__global__ void __launch_bounds__(kTlThreads)
+    k_xfwd_tl(int ny, int nz, const C2<T>* __restrict__ tw, const TIN* __restrict__ s, C2<T>* __restrict__ A)
+{
+    constexpr int N = 256, KT = 2 * kTlLines, LS = N + 1, nh = N / 2 + 1;
+    __shared__ C2<T> Y[kTlLds];
+    C2<T>* twl = Y + kTlLines * kTlLine;
+    const int j = blockIdx.y, k0 = blockIdx.x * KT;
+    const int tid = threadIdx.x;
+    const C2<T> w0 = tw[tid], w1 = tw[tid + kTlThreads];
+    {
+        const int l = tid >> 4, n2 = tid & 15;  // lanes along the line
+        const size_t plane = (size_t)N * ny;
+        const TIN* s0 = s + n2 + (size_t)N * (j + (size_t)ny * (k0 + 2 * l));
+        C2<T> v[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) v[n1] = C2<T>{(T)s0[16 * n1], (T)s0[16 * n1 + plane]};
+        twl[tid] = w0;
+        twl[tid + kTlThreads] = w1;
+        __syncthreads();
+        dft16<T, -1>(v);
+        tl_store_y<T, -1>(Y, twl, l, n2, v);
+    }
+    __syncthreads();
+    {
+        const int k1 = tid & 15, l = tid >> 4;
+        const C2<T>* y = Y + l * kTlLine + k1 * kTlRow;
+        C2<T> v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) v[n2] = y[n2];
+        dft16<T, -1>(v);
+        __syncthreads();  // every Y row read before the lines overwrite them
+        C2<T>* x = Y + l * LS + k1;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) x[16 * k2] = v[k2];
+    }
+    __syncthreads();
+    for (int t = tid; t < nh * KT; t += kTlThreads) {
+        const int m = t / KT, kk = t - m * KT;
+        const int l = kk >> 1;
+        const C2<T> Z = Y[l * LS + m];
+        const C2<T> Zn = Y[l * LS + (m == 0 ? 0 : N - m)];
+        C2<T> val;
+        if ((kk & 1) == 0)
+            val = C2<T>{(T)0.5 * (Z.x + Zn.x), (T)0.5 * (Z.y - Zn.y)};  // (Z + conj Zn)/2
+        else
+            val = C2<T>{(T)0.5 * (Z.y + Zn.y), (T)0.5 * (Zn.x - Z.x)};  // (Z - conj Zn)/(2i)
+        A[((size_t)m * ny + j) * nz + k0 + kk] = val;
+    }
+}
+
+// K5 for nx = 256: A[kx][j][k0..k0+15] -> p(:, j, k) scaled by 1/(nx nz);
+// mode 0: p = val; mode 1: p = p + val - shift[0] (as k_xinv)
+template <class T>
+__global__ void __launch_bounds__(kTlThreads)
+    k_xinv_tl(int ny, int nz, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ A, double* __restrict__ p,
+              int mode, const double* __restrict__ shift)
+{
+    constexpr int N = 256, L = kTlLines, KT = 2 * L, LS = N + 1, nh = N / 2 + 1;
+    constexpr int kB = stage_batch<T>() / 2;  // two complex loads per element
+    __shared__ C2<T> Y[kTlLds];
+    C2<T>* twl = Y + kTlLines * kTlLine;
+    const int j = blockIdx.y, k0 = blockIdx.x * KT;
+    const int tid = threadIdx.x;
+    twl[tid] = tw[tid];
+    twl[tid + kTlThreads] = tw[tid + kTlThreads];
+    // Z_l[m] = X1[m] + i X2[m] with Hermitian extension, into the lines (as k_xinv)
+    for (int t0 = tid; t0 < nh * L; t0 += kB * kTlThreads) {
+        T x1r[kB], x1i[kB], x2r[kB], x2i[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int t = min(t0 + u * kTlThreads, nh * L - 1);
+            const int m = t / L, l = t - m * L;
+            const size_t o = ((size_t)m * ny + j) * nz + k0 + 2 * l;
+            const C2<T> a = A[o], b = A[o + 1];
+            x1r[u] = a.x;
+            x1i[u] = a.y;
+            x2r[u] = b.x;
+            x2i[u] = b.y;
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int t = t0 + u * kTlThreads;
+            if (t < nh * L) {
+                const int m = t / L, l = t - m * L;
+                const C2<T> x1{x1r[u], x1i[u]}, x2{x2r[u], x2i[u]};
+                if (m == 0 || 2 * m == N) {
+                    Y[l * LS + m] = C2<T>{x1.x, x2.x};
+                } else {
+                    Y[l * LS + m] = C2<T>{x1.x - x2.y, x1.y + x2.x};
+                    Y[l * LS + N - m] = C2<T>{x1.x + x2.y, x2.x - x1.y};
+                }
+            }
+        }
+    }
+    __syncthreads();
+    {
+        const int l = tid >> 4, n2 = tid & 15;
+        C2<T> v[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1) v[n1] = Y[l * LS + 16 * n1 + n2];
+        dft16<T, +1>(v);
+        __syncthreads();  // every line read before Y overwrites them
+        tl_store_y<T, +1>(Y, twl, l, n2, v);
+    }
+    __syncthreads();
+    const int k1 = tid & 15, l = tid >> 4;  // lanes along the line
+    const C2<T>* y = Y + l * kTlLine + k1 * kTlRow;
+    C2<T> v[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = y[n2];
+    dft16<T, +1>(v);
+    const double scale = 1.0 / ((double)N * (double)nz);
+    const size_t plane = (size_t)N * ny;
+    double* p0 = p + k1 + (size_t)N * (j + (size_t)ny * (k0 + 2 * l));
+    if (mode == 0) {
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) {
+            p0[16 * k2] = (double)v[k2].x * scale;
+            p0[16 * k2 + plane] = (double)v[k2].y * scale;
+        }
+        return;
+    }
+    const double sh = shift[0];
+    double a0[16], a1[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+        a0[k2] = p0[16 * k2];
+        a1[k2] = p0[16 * k2 + plane];
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+        p0[16 * k2] = a0[k2] + (double)v[k2].x * scale - sh;
+        p0[16 * k2 + plane] = a1[k2] + (double)v[k2].y * scale - sh;
+    }
+}
+
 // ------------------------------------------------------------ K3 y-solve
 
 template <class T>
@@ -742,6 +1051,7 @@ struct tbfg_fps {
     int precision = 0, refine = 0;
     int nx = 0, ny = 0, nz = 0, nh = 0;
     int KT = 0, JT = 0, MR = 1;
+    bool tlx = false, tlz = false;  // two-level passes (N = 256) along x / z
     FftPlan px{}, pz{};
     DevBuf A, B;               // spectral work arrays (complex, T)
     DevBuf tab;                // lx, lz, am, ap, dyf, vol_y (double)
@@ -869,16 +1179,30 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
     h->ot.ap = h->tt.ap;
     h->ot.vol_y = tb + o_vy;
     h->inv_vg = 1.0 / (ly * L.ax[0].df(1) * nx * L.ax[2].df(1) * nz);
-    // per-pass twiddle tables (fft_lines): entry Ns-1 + (r-1)*Ns + k of the pass
-    // with sub-transform length Ns and radix R is exp(-2 pi i r k / (Ns R)), in
-    // FP64 and rounded to FP32; entry n-1 is unused
-    auto tw = [&](int n, DevBuf& d64, DevBuf& d32) -> int {
+    // two-level passes where the line length is 256 and the workgroup's lines
+    // tile the other axis (TBFG_FFT_TWO_LEVEL=0 keeps fft_lines: an A/B switch)
+    const char* tl_env = std::getenv("TBFG_FFT_TWO_LEVEL");
+    const bool tl_on = !(tl_env && tl_env[0] == '0');
+    h->tlx = tl_on && nx == 256 && nz % (2 * kTlLines) == 0;
+    h->tlz = tl_on && nz == 256 && ny % kTlLines == 0;
+    // twiddles in FP64 and rounded to FP32.  fft_lines: entry Ns-1 + (r-1)*Ns + k
+    // of the pass with sub-transform length Ns and radix R is
+    // exp(-2 pi i r k / (Ns R)) (entry n-1 unused); two-level passes: entry
+    // 16 k1 + n2 is exp(-2 pi i n2 k1 / 256)
+    auto tw = [&](int n, bool two_level, DevBuf& d64, DevBuf& d32) -> int {
         FftPlan pl;
         make_plan(n, pl);
         std::vector<double> w(2 * (size_t)n, 0.0);
         std::vector<float> wf(2 * (size_t)n, 0.0f);
+        if (two_level)
+            for (int k1 = 0; k1 < 16; ++k1)
+                for (int n2 = 0; n2 < 16; ++n2) {
+                    const size_t e = (size_t)16 * k1 + n2;
+                    w[2 * e] = std::cos(2.0 * pi * n2 * k1 / n);
+                    w[2 * e + 1] = -std::sin(2.0 * pi * n2 * k1 / n);
+                }
         int Ns = 1;
-        for (int st = 0; st < pl.nrad; ++st) {
+        for (int st = 0; st < pl.nrad && !two_level; ++st) {
             const int R = pl.rad[st];
             for (int r = 1; r < R; ++r)
                 for (int k = 0; k < Ns; ++k) {
@@ -894,8 +1218,8 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
         TBFG_TRY(upload(d32, wf.data(), wf.size() * 4));
         return TBFG_OK;
     };
-    TBFG_TRY(tw(nx, h->twx, h->twxf));
-    TBFG_TRY(tw(nz, h->twz, h->twzf));
+    TBFG_TRY(tw(nx, h->tlx, h->twx, h->twxf));
+    TBFG_TRY(tw(nz, h->tlz, h->twz, h->twzf));
     const size_t es = h->precision == TBFG_PREC_MIXED ? 8 : 16;  // complex element size
     const size_t nspec = (size_t)h->nh * ny * nz;
     TBFG_TRY(h->A.alloc(nspec * es));
@@ -933,10 +1257,17 @@ static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipSt
     const C2<T>* twz = f32 ? h->twzf.as<C2<T>>() : h->twz.as<C2<T>>();
     const size_t ldx = ((size_t)(h->KT / 2) * (nx + 1) + nx) * sizeof(C2<T>);  // padded lines + twiddles
     const size_t ldz = ((size_t)h->JT * (nz + 1) + nz) * sizeof(C2<T>);
-    hipLaunchKernelGGL((k_xfwd<T, TIN>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px,
-                       twx, in, A);
-    hipLaunchKernelGGL((k_zfwd<T>), dim3((ny / h->JT) * nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz, A,
-                       B);
+    if (h->tlx)
+        hipLaunchKernelGGL((k_xfwd_tl<T, TIN>), dim3(nz / (2 * kTlLines), ny), dim3(kTlThreads), 0, s, ny, nz, twx, in,
+                           A);
+    else
+        hipLaunchKernelGGL((k_xfwd<T, TIN>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px,
+                           twx, in, A);
+    if (h->tlz)
+        hipLaunchKernelGGL((k_zfwd_tl<T>), dim3((ny / kTlLines) * nh), dim3(kTlThreads), 0, s, ny, twz, A, B);
+    else
+        hipLaunchKernelGGL((k_zfwd<T>), dim3((ny / h->JT) * nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz,
+                           A, B);
     switch (h->MR) {
         case 1: launch_ysolve<T, 1>(h, B, s); break;
         case 2: launch_ysolve<T, 2>(h, B, s); break;
@@ -944,10 +1275,17 @@ static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipSt
         case 8: launch_ysolve<T, 8>(h, B, s); break;
         default: launch_ysolve<T, 16>(h, B, s); break;
     }
-    hipLaunchKernelGGL((k_zinv<T>), dim3((ny / h->JT) * nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz, B,
-                       A);
-    hipLaunchKernelGGL((k_xinv<T>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px, twx, A,
-                       p, mode, h->mean.as<double>());
+    if (h->tlz)
+        hipLaunchKernelGGL((k_zinv_tl<T>), dim3((ny / kTlLines) * nh), dim3(kTlThreads), 0, s, ny, twz, B, A);
+    else
+        hipLaunchKernelGGL((k_zinv<T>), dim3((ny / h->JT) * nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz,
+                           B, A);
+    if (h->tlx)
+        hipLaunchKernelGGL((k_xinv_tl<T>), dim3(nz / (2 * kTlLines), ny), dim3(kTlThreads), 0, s, ny, nz, twx, A, p,
+                           mode, h->mean.as<double>());
+    else
+        hipLaunchKernelGGL((k_xinv<T>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px, twx,
+                           A, p, mode, h->mean.as<double>());
     TBFG_HIP(hipGetLastError());
     return TBFG_OK;
 }
