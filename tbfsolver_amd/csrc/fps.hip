@@ -825,19 +825,20 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
                 const TC cint = (r < mm - 1) ? c[r] : (TC)0;
                 const TC rg = (r == mm - 1) ? -c[r] : (TC)0;
                 const TC rb = (r == 0) ? -a[0] : (TC)0;
-                TC den;
+                // one division per row: its reciprocal scales the row's four results
                 if (r == 0) {
-                    den = b[0];
-                    al[0] = C2<TC>{d[0].x / den, d[0].y / den};
-                    be[0] = rb / den;
-                    ga[0] = rg / den;
+                    const TC inv = (TC)1 / b[0];
+                    al[0] = C2<TC>{d[0].x * inv, d[0].y * inv};
+                    be[0] = rb * inv;
+                    ga[0] = rg * inv;
+                    cp[0] = cint * inv;
                 } else {
-                    den = b[r] - a[r] * cp[r - 1];
-                    al[r] = C2<TC>{(d[r].x - a[r] * al[r - 1].x) / den, (d[r].y - a[r] * al[r - 1].y) / den};
-                    be[r] = (rb - a[r] * be[r - 1]) / den;
-                    ga[r] = (rg - a[r] * ga[r - 1]) / den;
+                    const TC inv = (TC)1 / (b[r] - a[r] * cp[r - 1]);
+                    al[r] = C2<TC>{(d[r].x - a[r] * al[r - 1].x) * inv, (d[r].y - a[r] * al[r - 1].y) * inv};
+                    be[r] = (rb - a[r] * be[r - 1]) * inv;
+                    ga[r] = (rg - a[r] * ga[r - 1]) * inv;
+                    cp[r] = cint * inv;
                 }
-                cp[r] = cint / den;
             }
         }
 #pragma unroll
@@ -879,38 +880,36 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
         nb_ = 0;
         ng = 0;
     }
-    TC A = ae * lb;
-    TC B = ae * lg + bee + ce * nb_;
-    TC C = ce * ng;
-    C2<TC> D = {de.x - ae * la.x - ce * na.x, de.y - ae * la.y - ce * na.y};
-    // parallel cyclic reduction over the 64 lanes
+    // the reduced row, scaled to a unit diagonal
+    const TC inv0 = (TC)1 / (ae * lg + bee + ce * nb_);
+    TC A = ae * lb * inv0;
+    TC C = ce * ng * inv0;
+    C2<TC> D = {(de.x - ae * la.x - ce * na.x) * inv0, (de.y - ae * la.y - ce * na.y) * inv0};
+    // parallel cyclic reduction over the 64 lanes; every row keeps a unit
+    // diagonal, so the eliminations need no division and B is never exchanged
 #pragma unroll
     for (int s = 1; s < 64; s <<= 1) {
-        TC Am = shfl_t(A, lane - s), Bm = shfl_t(B, lane - s), Cm = shfl_t(C, lane - s);
+        TC Am = shfl_t(A, lane - s), Cm = shfl_t(C, lane - s);
         C2<TC> Dm = {shfl_t(D.x, lane - s), shfl_t(D.y, lane - s)};
-        TC Ap = shfl_t(A, lane + s), Bp = shfl_t(B, lane + s), Cp = shfl_t(C, lane + s);
+        TC Ap = shfl_t(A, lane + s), Cp = shfl_t(C, lane + s);
         C2<TC> Dp = {shfl_t(D.x, lane + s), shfl_t(D.y, lane + s)};
         if (lane - s < 0) {
             Am = 0;
-            Bm = 1;
             Cm = 0;
             Dm = C2<TC>{0, 0};
         }
         if (lane + s > 63) {
             Ap = 0;
-            Bp = 1;
             Cp = 0;
             Dp = C2<TC>{0, 0};
         }
-        const TC k1 = A / Bm, k2 = C / Bp;
-        const TC A2 = -Am * k1, C2v = -Cp * k2;
-        const TC B2 = B - Cm * k1 - Ap * k2;
-        D = C2<TC>{D.x - Dm.x * k1 - Dp.x * k2, D.y - Dm.y * k1 - Dp.y * k2};
+        const TC inv = (TC)1 / ((TC)1 - Cm * A - Ap * C);
+        D = C2<TC>{(D.x - Dm.x * A - Dp.x * C) * inv, (D.y - Dm.y * A - Dp.y * C) * inv};
+        const TC A2 = -Am * A * inv, C2v = -Cp * C * inv;
         A = A2;
-        B = B2;
         C = C2v;
     }
-    const C2<TC> y = {D.x / B, D.y / B};
+    const C2<TC> y = D;
     C2<TC> ym = {shfl_t(y.x, lane - 1), shfl_t(y.y, lane - 1)};
     if (lane == 0) ym = C2<TC>{0, 0};
     C2<TC> x[MR];
