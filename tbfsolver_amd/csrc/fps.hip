@@ -1132,7 +1132,7 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
         return TBFG_EINVAL;
     }
     const int m = (ny + 63) / 64;
-    h->MR = m <= 1 ? 1 : (m <= 2 ? 2 : (m <= 4 ? 4 : (m <= 8 ? 8 : 16)));
+    h->MR = m <= 1 ? 1 : (m <= 2 ? 2 : (m <= 3 ? 3 : (m <= 4 ? 4 : (m <= 8 ? 8 : 16))));
     HostLevel L = make_level(g, 0);
     // fastPoissonSolver.f90:215-251 (eigenvalues), :255-276 (y coefficients)
     std::vector<double> t;
@@ -1270,6 +1270,7 @@ static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipSt
     switch (h->MR) {
         case 1: launch_ysolve<T, 1>(h, B, s); break;
         case 2: launch_ysolve<T, 2>(h, B, s); break;
+        case 3: launch_ysolve<T, 3>(h, B, s); break;
         case 4: launch_ysolve<T, 4>(h, B, s); break;
         case 8: launch_ysolve<T, 8>(h, B, s); break;
         default: launch_ysolve<T, 16>(h, B, s); break;
