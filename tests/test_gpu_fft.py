@@ -32,10 +32,12 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-# (the last two: the longest transform the solver takes, one FFT line pair per
-# workgroup, and the tallest y column, 16 rows per lane of the tridiagonal solve)
+# (256-point x or z lines run the two-level passes, the others the Stockham
+# passes: (256, 16, 32) and (32, 16, 256) mix the two.  The last two: the
+# longest transform the solver takes, one FFT line pair per workgroup, and the
+# tallest y column, 16 rows per lane of the tridiagonal solve)
 CASES = [((64, 64, 64), 0), ((32, 24, 16), 1), ((96, 32, 48), 1), ((256, 192, 256), 1), ((192, 128, 96), 0),
-         ((8, 6, 4), 1), ((1024, 8, 768), 0), ((16, 1024, 8), 1)]
+         ((256, 16, 32), 1), ((32, 16, 256), 0), ((8, 6, 4), 1), ((1024, 8, 768), 0), ((16, 1024, 8), 1)]
 
 
 @pytest.mark.parametrize("n,stretch", CASES)
@@ -64,7 +66,7 @@ def test_fft_solve_fp64(torch_gpu, n, stretch):
     assert abs(np.sum(V * out)) / V.sum() < 1e-13 * np.abs(out).max()
 
 
-@pytest.mark.parametrize("n,stretch", [((64, 64, 64), 0), ((96, 32, 48), 1), ((256, 192, 256), 1)])
+@pytest.mark.parametrize("n,stretch", [((64, 64, 64), 0), ((96, 32, 48), 1), ((256, 16, 32), 1), ((256, 192, 256), 1)])
 def test_fft_solve_mixed(torch_gpu, n, stretch):
     print("case fft mixed", n, stretch, flush=True)
     g, go = grids(n, stretch)
@@ -82,6 +84,32 @@ def test_fft_solve_mixed(torch_gpu, n, stretch):
         e = rel(to_host(p), to_host(pr))
         print("  sweeps", sweeps, "rel err", e, flush=True)
         assert e < bound
+
+
+@pytest.mark.parametrize("precision,bound", [("fp64", 1e-13), ("mixed", 1e-5)])
+def test_two_level_passes_equal_stockham_passes(torch_gpu, monkeypatch, precision, bound):
+    """256-point lines: the two-level passes against the Stockham passes of the
+    same lengths (TBFG_FFT_TWO_LEVEL=0, read when a handle is created), on the
+    same right-hand side; mixed precision without refinement compares the FP32
+    transforms themselves."""
+    n = (256, 48, 256)
+    print("case two-level vs stockham", precision, n, flush=True)
+    g, go = grids(n, 1)
+    V = vol(go)
+    s = rng.standard_normal(n)
+    s -= np.sum(V * s) / V.sum()
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("TBFG_FFT_TWO_LEVEL", flag)
+        fs = T.FastPoissonSolver(g, precision=precision, refine_sweeps=0)
+        p = torch_gpu.zeros(n[::-1], dtype=torch_gpu.float64, device="cuda")
+        fs.solveFPS(p, to_dev(s, torch_gpu))
+        torch_gpu.cuda.synchronize()
+        out[flag] = to_host(p)
+        fs.close()
+    e = rel(out["1"], out["0"])
+    print("  rel diff", e, flush=True)
+    assert e < bound
 
 
 def test_config0_projection(torch_gpu):
