@@ -171,7 +171,9 @@ int tbfg_pcg_solve_queued_dev(tbfg_pcg* h, double* psi_dev, const double* beta_d
                               int32_t iterations, void* stream);
 /* The outcome of the last queued solve, read after its work has finished: iterations,
  * relative residual, and TBFG_OK, TBFG_ENOTCONVERGED (max_iter reached, or not converged
- * within the iterations queued) or TBFG_ENUMERIC.  Blocking. */
+ * within the iterations queued) or TBFG_ENUMERIC.  Blocking.  The read is ordered with no
+ * stream (a library-internal non-blocking one, legal while other threads capture graphs):
+ * the caller synchronises the solve's stream first. */
 int tbfg_pcg_result(tbfg_pcg* h, int32_t* iters, double* res);
 
 /* Applies the handle's multigrid preconditioner once: z = M r (device arrays). */

@@ -2,6 +2,7 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <mutex>
 #include <cstdarg>
 #include <cstring>
 
@@ -125,11 +126,46 @@ void DevBuf::release()
     bytes = 0;
 }
 
+// one non-blocking stream per device (default priority), kept for the process
+static int copy_stream(hipStream_t* out)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static hipStream_t streams[kMaxDevices] = {};
+    int dev = 0;
+    TBFG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d beyond the copy streams' table", dev);
+        return TBFG_EDEVICE;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!streams[dev]) TBFG_HIP(hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking));
+    *out = streams[dev];
+    return TBFG_OK;
+}
+
+int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+{
+    hipStream_t s;
+    TBFG_TRY(copy_stream(&s));
+    TBFG_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+    TBFG_HIP(hipStreamSynchronize(s));
+    return TBFG_OK;
+}
+
+int memset_sync(void* dst, int value, size_t bytes)
+{
+    hipStream_t s;
+    TBFG_TRY(copy_stream(&s));
+    TBFG_HIP(hipMemsetAsync(dst, value, bytes, s));
+    TBFG_HIP(hipStreamSynchronize(s));
+    return TBFG_OK;
+}
+
 int upload(DevBuf& d, const void* src, size_t bytes)
 {
     TBFG_TRY(d.alloc(bytes));
-    TBFG_HIP(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
-    return TBFG_OK;
+    return copy_sync(d.p, src, bytes, hipMemcpyHostToDevice);
 }
 
 template <class T>

@@ -101,6 +101,13 @@ struct DevBuf {
 
 int upload(DevBuf& d, const void* src, size_t bytes);
 
+// Synchronous copies and fills of handle creation and result reads, on a
+// library-internal non-blocking stream of the current device (never ordered
+// with the legacy null stream, so they stay legal while another thread
+// captures a HIP graph on its own stream).
+int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+int memset_sync(void* dst, int value, size_t bytes);
+
 // Per-axis device tables of one level, stored in T.  For cell i (1-based):
 //   ip[i-1] = 1/(dc(i+1) df(i)), im[i-1] = 1/(dc(i) df(i))       (pcg.f90:448-463)
 // restriction to the next coarser level, per coarse index I (1-based):

@@ -1227,7 +1227,7 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
         TBFG_TRY(h->r.alloc((size_t)nx * ny * nz * 8));
         TBFG_TRY(h->partials.alloc((size_t)row_blocks(nx) * ny * nz * 8 + 64));
         TBFG_TRY(h->counter.alloc(64));
-        TBFG_HIP(hipMemset(h->counter.p, 0, 64));
+        TBFG_TRY(memset_sync(h->counter.p, 0, 64));
         TBFG_TRY(h->mean.alloc(64));
     }
     h->footprint = (int64_t)(h->A.bytes + h->B.bytes + h->tab.bytes + h->tabf.bytes + h->twx.bytes + h->twz.bytes +

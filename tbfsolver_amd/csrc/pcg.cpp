@@ -406,7 +406,7 @@ int pcg_create(const tbfg_grid_desc* g, const tbfg_pcg_params* prm_in, tbfg_pcg*
     if (h->fuse_op) TBFG_TRY(h->d2.alloc(h->n0 * 8));
     TBFG_TRY(h->partials.alloc(reduction_slots(h->tabd[0]->view, h->opp) * sizeof(double)));
     TBFG_TRY(h->counter.alloc(256));
-    TBFG_HIP(hipMemset(h->counter.p, 0, 256));
+    TBFG_TRY(memset_sync(h->counter.p, 0, 256));
     // dense bottom operator
     const HostLevel& B = h->hl[ntot - 1];
     const size_t nbc = B.cells();
@@ -908,7 +908,7 @@ int pcg_result(tbfg_pcg* h, int32_t* iters, double* res)
         set_error("tbfg_pcg_result: NULL handle");
         return TBFG_EINVAL;
     }
-    TBFG_HIP(hipMemcpy(h->hst, h->st.p, sizeof(PcgState), hipMemcpyDeviceToHost));
+    TBFG_TRY(copy_sync(h->hst, h->st.p, sizeof(PcgState), hipMemcpyDeviceToHost));
     return finish_state(h, h->queued_fft, iters, res);
 }
 
