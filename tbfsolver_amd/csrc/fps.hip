@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "fps.hpp"
@@ -467,79 +468,107 @@ __global__ void __launch_bounds__(kFftThreads)
     }
 }
 
-// ------------------------------------------- two-level passes for N = 256
+// ------------------------------------------- two-level passes, N = N1 x N2
 //
-// Lines of N = 256 = 16 x 16 skip the Stockham passes in LDS.  With
-// n = 16 n1 + n2 and k = k1 + 16 k2,
-//   X[k1 + 16 k2] = sum_n2 W16^(n2 k2) [ W256^(n2 k1) sum_n1 x[16 n1 + n2] W16^(n1 k1) ].
-// Step A: thread n2 of a line holds x[16 n1 + n2], n1 = 0..15, in registers
+// Lines of N = N1 * N2 points (N1, N2 in {4, 8, 12, 16}; tl_split) skip the
+// Stockham passes in LDS.  With n = N2 n1 + n2 and k = k1 + N1 k2,
+//   X[k1 + N1 k2] = sum_n2 W_N2^(n2 k2) [ W_N^(n2 k1) sum_n1 x[N2 n1 + n2] W_N1^(n1 k1) ].
+// Step A: thread n2 of a line holds x[N2 n1 + n2], n1 = 0..N1-1, in registers
 // (loaded from HBM, or from the pre-processed line in LDS), runs the inner
-// 16-point DFT, multiplies by W256^(n2 k1) and writes Y[k1][n2] to LDS.
-// Step B: thread k1 reads Y[k1][0..15] and runs the outer 16-point DFT, which
-// leaves X[k1 + 16 k2] in its registers, stored straight to HBM (z passes) or
-// through one more LDS line (x passes, whose r2c/c2r packing pairs X[m] with
-// X[N-m]).  One LDS exchange per line replaces the staging, log4(N) passes and
-// output round trips of fft_lines.  8 lines per 128-thread workgroup; Y rows of
-// 17 elements and lines of 273 (= 1 mod 16), so that step A's stores
-// (consecutive n2, or consecutive lines) and step B's loads (consecutive k1, or
-// consecutive lines) fall in different banks.  Each step orders its lanes the
-// way its side of HBM is laid out: along the line where the line is contiguous,
-// across lines where the pass reads or writes the transposed layout.
+// N1-point DFT, multiplies by W_N^(n2 k1) and writes Y[k1][n2] to LDS.
+// Step B: thread k1 reads Y[k1][0..N2-1] and runs the outer N2-point DFT,
+// which leaves X[k1 + N1 k2] in its registers, stored straight to HBM (z
+// passes) or through one more LDS line (x passes, whose r2c/c2r packing pairs
+// X[m] with X[N-m]).  One LDS exchange per line replaces the staging, the
+// Stockham passes and the output round trip of fft_lines.  8 lines per
+// workgroup of 8 max(N1, N2) threads (the threads of the smaller factor idle
+// through the other step); Y rows of N2 + 1 elements and lines of
+// N1 (N2 + 1) + 1, so that step A's stores (consecutive n2, or consecutive
+// lines) and step B's loads (consecutive k1, or consecutive lines) fall in
+// different banks.  Each step orders its lanes the way its side of HBM is
+// laid out: along the line where the line is contiguous, across lines where
+// the pass reads or writes the transposed layout.
 constexpr int kTlLines = 8;
-constexpr int kTlThreads = kTlLines * 16;
-constexpr int kTlRow = 17;
-constexpr int kTlLine = 16 * kTlRow + 1;          // 273 elements
-constexpr int kTlLds = kTlLines * kTlLine + 256;  // Y (or the x passes' 8 lines of 257) + twiddles
 
-// a * W16^E; W16 = exp(SIGN 2 pi i / 16), E = 0..9
-template <class T, int SIGN, int E>
-__device__ __forceinline__ C2<T> tw16_mul(C2<T> a)
+template <int N1, int N2>
+struct TlShape {
+    static constexpr int N = N1 * N2;
+    static constexpr int NT = kTlLines * (N1 > N2 ? N1 : N2);  // threads
+    static constexpr int ROW = N2 + 1;
+    static constexpr int LINE = N1 * ROW + 1;
+    static constexpr int LDS = kTlLines * LINE + N;            // Y (or the x passes' lines) + twiddles
+    static_assert(LINE >= N + 1, "the x passes' lines of N + 1 fit in a Y line");
+};
+
+// cos and sin of 2 pi e / 48 for e = 0..12 (a quarter turn; the W_R of every
+// supported factor R divides 48)
+constexpr double kW48c[13] = {1.0, 0.991444861373810411144558, 0.965925826289068286749743,
+                              0.923879532511286756128183, 0.866025403784438646763723, 0.793353340291235164579777,
+                              0.707106781186547524400844, 0.608761429008720639416098, 0.5,
+                              0.382683432365089771728460, 0.258819045102520762348899, 0.130526192220051591548406,
+                              0.0};
+
+// a * W_R^e, W_R = exp(SIGN 2 pi i / R); R divides 48; e a constant once the
+// caller's loops are unrolled, so the table reads and branches fold away
+template <class T, int SIGN, int R>
+__device__ __forceinline__ C2<T> twr_mul(C2<T> a, int e)
 {
-    constexpr double c[10] = {1.0, 0.923879532511286756128, 0.707106781186547524401, 0.382683432365089771728, 0.0,
-                              -0.382683432365089771728, -0.707106781186547524401, -0.923879532511286756128, -1.0,
-                              -0.923879532511286756128};
-    constexpr double s[10] = {0.0, 0.382683432365089771728, 0.707106781186547524401, 0.923879532511286756128, 1.0,
-                              0.923879532511286756128, 0.707106781186547524401, 0.382683432365089771728, 0.0,
-                              -0.382683432365089771728};
-    if constexpr (E == 0) {
-        return a;
-    } else if constexpr (E == 4) {
-        return SIGN < 0 ? C2<T>{a.y, -a.x} : C2<T>{-a.y, a.x};  // a * (-i) or a * (+i)
-    } else {
-        const T wr = (T)c[E], wi = (T)(SIGN * s[E]);
-        return C2<T>{a.x * wr - a.y * wi, a.x * wi + a.y * wr};
+    const int e48 = (e % R) * (48 / R);
+    const int q = e48 / 12, r = e48 % 12;
+    if (e48 == 0) return a;
+    if (r == 0) {  // a quarter, half or three-quarter turn
+        if (q == 2) return C2<T>{-a.x, -a.y};
+        const bool minus_i = (q == 1) == (SIGN < 0);  // W = -i: forward quarter, backward three-quarter
+        return minus_i ? C2<T>{a.y, -a.x} : C2<T>{-a.y, a.x};
     }
+    // cos and sin of the angle q * 90 + r * 7.5 degrees
+    const double c = q == 0 ? kW48c[r] : (q == 1 ? -kW48c[12 - r] : (q == 2 ? -kW48c[r] : kW48c[12 - r]));
+    const double s = q == 0 ? kW48c[12 - r] : (q == 1 ? kW48c[r] : (q == 2 ? -kW48c[12 - r] : -kW48c[r]));
+    const T wr = (T)c, wi = (T)(SIGN * s);
+    return C2<T>{a.x * wr - a.y * wi, a.x * wi + a.y * wr};
 }
 
-// 16-point DFT in registers, natural order in and out (4 x 4 Cooley-Tukey)
+// R-point DFT in registers, natural order in and out: R = 2, 3, 4 directly,
+// R = 8, 12, 16 as 4 x (R/4) Cooley-Tukey
+template <class T, int SIGN, int R>
+struct DftReg {
+    static __device__ __forceinline__ void run(C2<T>* v)
+    {
+        static_assert(R % 4 == 0 && R > 4 && R <= 16, "supported: 2, 3, 4, 8, 12, 16");
+        constexpr int P = 4, Q = R / 4;  // n = Q n1 + n2, k = k1 + P k2
+        C2<T> a[Q][P];
+#pragma unroll
+        for (int n2 = 0; n2 < Q; ++n2) {
+            C2<T> t[P];
+#pragma unroll
+            for (int n1 = 0; n1 < P; ++n1) t[n1] = v[Q * n1 + n2];
+            dft4<T, SIGN>(t);
+#pragma unroll
+            for (int k1 = 0; k1 < P; ++k1) a[n2][k1] = twr_mul<T, SIGN, R>(t[k1], n2 * k1);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) {
+            C2<T> t[Q];
+#pragma unroll
+            for (int n2 = 0; n2 < Q; ++n2) t[n2] = a[n2][k1];
+            DftReg<T, SIGN, Q>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < Q; ++k2) v[k1 + P * k2] = t[k2];
+        }
+    }
+};
 template <class T, int SIGN>
-__device__ __forceinline__ void dft16(C2<T> (&v)[16])
-{
-    C2<T> a[4][4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        C2<T> t[4] = {v[q], v[4 + q], v[8 + q], v[12 + q]};
-        dft4<T, SIGN>(t);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) a[q][k] = t[k];
-    }
-    a[1][1] = tw16_mul<T, SIGN, 1>(a[1][1]);
-    a[1][2] = tw16_mul<T, SIGN, 2>(a[1][2]);
-    a[1][3] = tw16_mul<T, SIGN, 3>(a[1][3]);
-    a[2][1] = tw16_mul<T, SIGN, 2>(a[2][1]);
-    a[2][2] = tw16_mul<T, SIGN, 4>(a[2][2]);
-    a[2][3] = tw16_mul<T, SIGN, 6>(a[2][3]);
-    a[3][1] = tw16_mul<T, SIGN, 3>(a[3][1]);
-    a[3][2] = tw16_mul<T, SIGN, 6>(a[3][2]);
-    a[3][3] = tw16_mul<T, SIGN, 9>(a[3][3]);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        C2<T> t[4] = {a[0][k], a[1][k], a[2][k], a[3][k]};
-        dft4<T, SIGN>(t);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[k + 4 * q] = t[q];
-    }
-}
+struct DftReg<T, SIGN, 2> {
+    static __device__ __forceinline__ void run(C2<T>* v) { dft2<T, SIGN>(v); }
+};
+template <class T, int SIGN>
+struct DftReg<T, SIGN, 3> {
+    static __device__ __forceinline__ void run(C2<T>* v) { dft3<T, SIGN>(v); }
+};
+template <class T, int SIGN>
+struct DftReg<T, SIGN, 4> {
+    static __device__ __forceinline__ void run(C2<T>* v) { dft4<T, SIGN>(v); }
+};
 
 // v * w (SIGN = -1) or v * conj(w) (SIGN = +1)
 template <class T, int SIGN>
@@ -549,134 +578,164 @@ __device__ __forceinline__ C2<T> tw_apply(C2<T> v, C2<T> w)
     return cmul(v, w);
 }
 
-// Step A's tail: twiddle W256^(n2 k1) and store Y[k1][n2] of line l
-template <class T, int SIGN>
-__device__ __forceinline__ void tl_store_y(C2<T>* Y, const C2<T>* twl, int l, int n2, const C2<T> (&v)[16])
+// The pass's twiddle table W_N^(n2 k1) (entry N2 k1 + n2) into LDS
+template <class T, int N1, int N2>
+__device__ __forceinline__ void tl_load_twiddles(C2<T>* twl, const C2<T>* __restrict__ tw)
 {
-    C2<T>* y = Y + l * kTlLine + n2;
+    using S = TlShape<N1, N2>;
+#pragma unroll
+    for (int u = 0; u < (S::N + S::NT - 1) / S::NT; ++u) {
+        const int t = (int)threadIdx.x + u * S::NT;
+        if (t < S::N) twl[t] = tw[t];
+    }
+}
+
+// Step A's tail: twiddle W_N^(n2 k1) and store Y[k1][n2] of line l
+template <class T, int SIGN, int N1, int N2>
+__device__ __forceinline__ void tl_store_y(C2<T>* Y, const C2<T>* twl, int l, int n2, const C2<T> (&v)[N1])
+{
+    using S = TlShape<N1, N2>;
+    C2<T>* y = Y + l * S::LINE + n2;
     y[0] = v[0];
 #pragma unroll
-    for (int k1 = 1; k1 < 16; ++k1) y[k1 * kTlRow] = tw_apply<T, SIGN>(v[k1], twl[k1 * 16 + n2]);
+    for (int k1 = 1; k1 < N1; ++k1) y[k1 * S::ROW] = tw_apply<T, SIGN>(v[k1], twl[k1 * N2 + n2]);
 }
 
-// K2 for nz = 256: A[kx][j][:] -> B[kx][kz][j]
-template <class T>
-__global__ void __launch_bounds__(kTlThreads)
+// K2, two-level: A[kx][j][:] -> B[kx][kz][j]
+template <class T, int N1, int N2>
+__global__ void __launch_bounds__((TlShape<N1, N2>::NT))
     k_zfwd_tl(int ny, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ A, C2<T>* __restrict__ B)
 {
-    constexpr int N = 256;
-    __shared__ C2<T> Y[kTlLds];
-    C2<T>* twl = Y + kTlLines * kTlLine;
+    using S = TlShape<N1, N2>;
+    constexpr int N = S::N;
+    __shared__ C2<T> Y[S::LDS];
+    C2<T>* twl = Y + kTlLines * S::LINE;
     const unsigned id = xcd_block(), gj = (unsigned)(ny / kTlLines);
     const int m = (int)(id / gj);
     const int j0 = (int)(id % gj) * kTlLines;
     const int tid = threadIdx.x;
-    const C2<T> w0 = tw[tid], w1 = tw[tid + kTlThreads];  // first: waiting for them leaves the line loads in flight
     {
-        const int l = tid >> 4, n2 = tid & 15;  // lanes along the line
-        const C2<T>* src = A + ((size_t)m * ny + j0 + l) * N + n2;
-        C2<T> v[16];
+        const int l = tid / N2, n2 = tid - l * N2;  // lanes along the line
+        const bool act = tid < kTlLines * N2;
+        C2<T> v[N1];
+        if (act) {
+            const C2<T>* src = A + ((size_t)m * ny + j0 + l) * N + n2;
 #pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) v[n1] = src[16 * n1];
-        twl[tid] = w0;
-        twl[tid + kTlThreads] = w1;
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[N2 * n1];
+        }
+        tl_load_twiddles<T, N1, N2>(twl, tw);
         __syncthreads();
-        dft16<T, -1>(v);
-        tl_store_y<T, -1>(Y, twl, l, n2, v);
+        if (act) {
+            DftReg<T, -1, N1>::run(v);
+            tl_store_y<T, -1, N1, N2>(Y, twl, l, n2, v);
+        }
     }
     __syncthreads();
-    {
+    if (tid < kTlLines * N1) {
         const int l = tid & (kTlLines - 1), k1 = tid / kTlLines;  // lanes across lines
-        const C2<T>* y = Y + l * kTlLine + k1 * kTlRow;
-        C2<T> v[16];
+        const C2<T>* y = Y + l * S::LINE + k1 * S::ROW;
+        C2<T> v[N2];
 #pragma unroll
-        for (int n2 = 0; n2 < 16; ++n2) v[n2] = y[n2];
-        dft16<T, -1>(v);
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = y[n2];
+        DftReg<T, -1, N2>::run(v);
         C2<T>* dst = B + ((size_t)m * N + k1) * ny + j0 + l;
 #pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) dst[(size_t)16 * k2 * ny] = v[k2];
+        for (int k2 = 0; k2 < N2; ++k2) dst[(size_t)N1 * k2 * ny] = v[k2];
     }
 }
 
-// K4 for nz = 256: B[kx][kz][j] -> A[kx][j][:]
-template <class T>
-__global__ void __launch_bounds__(kTlThreads)
+// K4, two-level: B[kx][kz][j] -> A[kx][j][:]
+template <class T, int N1, int N2>
+__global__ void __launch_bounds__((TlShape<N1, N2>::NT))
     k_zinv_tl(int ny, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ B, C2<T>* __restrict__ A)
 {
-    constexpr int N = 256;
-    __shared__ C2<T> Y[kTlLds];
-    C2<T>* twl = Y + kTlLines * kTlLine;
+    using S = TlShape<N1, N2>;
+    constexpr int N = S::N;
+    __shared__ C2<T> Y[S::LDS];
+    C2<T>* twl = Y + kTlLines * S::LINE;
     const unsigned id = xcd_block(), gj = (unsigned)(ny / kTlLines);
     const int m = (int)(id / gj);
     const int j0 = (int)(id % gj) * kTlLines;
     const int tid = threadIdx.x;
-    const C2<T> w0 = tw[tid], w1 = tw[tid + kTlThreads];
     {
         const int l = tid & (kTlLines - 1), n2 = tid / kTlLines;  // lanes across lines
-        const C2<T>* src = B + ((size_t)m * N + n2) * ny + j0 + l;
-        C2<T> v[16];
+        const bool act = tid < kTlLines * N2;
+        C2<T> v[N1];
+        if (act) {
+            const C2<T>* src = B + ((size_t)m * N + n2) * ny + j0 + l;
 #pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) v[n1] = src[(size_t)16 * n1 * ny];
-        twl[tid] = w0;
-        twl[tid + kTlThreads] = w1;
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[(size_t)N2 * n1 * ny];
+        }
+        tl_load_twiddles<T, N1, N2>(twl, tw);
         __syncthreads();
-        dft16<T, +1>(v);
-        tl_store_y<T, +1>(Y, twl, l, n2, v);
+        if (act) {
+            DftReg<T, +1, N1>::run(v);
+            tl_store_y<T, +1, N1, N2>(Y, twl, l, n2, v);
+        }
     }
     __syncthreads();
-    {
-        const int k1 = tid & 15, l = tid >> 4;  // lanes along the line
-        const C2<T>* y = Y + l * kTlLine + k1 * kTlRow;
-        C2<T> v[16];
+    if (tid < kTlLines * N1) {
+        const int l = tid / N1, k1 = tid - l * N1;  // lanes along the line
+        const C2<T>* y = Y + l * S::LINE + k1 * S::ROW;
+        C2<T> v[N2];
 #pragma unroll
-        for (int n2 = 0; n2 < 16; ++n2) v[n2] = y[n2];
-        dft16<T, +1>(v);
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = y[n2];
+        DftReg<T, +1, N2>::run(v);
         C2<T>* dst = A + ((size_t)m * ny + j0 + l) * N + k1;
 #pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) dst[16 * k2] = v[k2];
+        for (int k2 = 0; k2 < N2; ++k2) dst[N1 * k2] = v[k2];
     }
 }
 
-// K1 for nx = 256: s(:, j, k0..k0+15) -> A[kx][j][k], kx = 0..128 (line l packs
+// K1, two-level: s(:, j, k0..k0+15) -> A[kx][j][k], kx = 0..N/2 (line l packs
 // the real lines k0+2l and k0+2l+1 into one complex line, as k_xfwd)
-template <class T, class TIN>
-__global__ void __launch_bounds__(kTlThreads)
+template <class T, class TIN, int N1, int N2>
+__global__ void __launch_bounds__((TlShape<N1, N2>::NT))
     k_xfwd_tl(int ny, int nz, const C2<T>* __restrict__ tw, const TIN* __restrict__ s, C2<T>* __restrict__ A)
 {
-    constexpr int N = 256, KT = 2 * kTlLines, LS = N + 1, nh = N / 2 + 1;
-    __shared__ C2<T> Y[kTlLds];
-    C2<T>* twl = Y + kTlLines * kTlLine;
+    using S = TlShape<N1, N2>;
+    constexpr int N = S::N, KT = 2 * kTlLines, LS = N + 1, nh = N / 2 + 1;
+    __shared__ C2<T> Y[S::LDS];
+    C2<T>* twl = Y + kTlLines * S::LINE;
     const int j = blockIdx.y, k0 = blockIdx.x * KT;
     const int tid = threadIdx.x;
-    const C2<T> w0 = tw[tid], w1 = tw[tid + kTlThreads];
     {
-        const int l = tid >> 4, n2 = tid & 15;  // lanes along the line
-        const size_t plane = (size_t)N * ny;
-        const TIN* s0 = s + n2 + (size_t)N * (j + (size_t)ny * (k0 + 2 * l));
-        C2<T> v[16];
+        const int l = tid / N2, n2 = tid - l * N2;  // lanes along the line
+        const bool act = tid < kTlLines * N2;
+        C2<T> v[N1];
+        if (act) {
+            const size_t plane = (size_t)N * ny;
+            const TIN* s0 = s + n2 + (size_t)N * (j + (size_t)ny * (k0 + 2 * l));
 #pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) v[n1] = C2<T>{(T)s0[16 * n1], (T)s0[16 * n1 + plane]};
-        twl[tid] = w0;
-        twl[tid + kTlThreads] = w1;
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = C2<T>{(T)s0[N2 * n1], (T)s0[N2 * n1 + plane]};
+        }
+        tl_load_twiddles<T, N1, N2>(twl, tw);
         __syncthreads();
-        dft16<T, -1>(v);
-        tl_store_y<T, -1>(Y, twl, l, n2, v);
+        if (act) {
+            DftReg<T, -1, N1>::run(v);
+            tl_store_y<T, -1, N1, N2>(Y, twl, l, n2, v);
+        }
     }
     __syncthreads();
     {
-        const int k1 = tid & 15, l = tid >> 4;
-        const C2<T>* y = Y + l * kTlLine + k1 * kTlRow;
-        C2<T> v[16];
+        const int l = tid / N1, k1 = tid - l * N1;
+        const bool act = tid < kTlLines * N1;
+        C2<T> v[N2];
+        if (act) {
+            const C2<T>* y = Y + l * S::LINE + k1 * S::ROW;
 #pragma unroll
-        for (int n2 = 0; n2 < 16; ++n2) v[n2] = y[n2];
-        dft16<T, -1>(v);
+            for (int n2 = 0; n2 < N2; ++n2) v[n2] = y[n2];
+            DftReg<T, -1, N2>::run(v);
+        }
         __syncthreads();  // every Y row read before the lines overwrite them
-        C2<T>* x = Y + l * LS + k1;
+        if (act) {
+            C2<T>* x = Y + l * LS + k1;
 #pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) x[16 * k2] = v[k2];
+            for (int k2 = 0; k2 < N2; ++k2) x[N1 * k2] = v[k2];
+        }
     }
     __syncthreads();
-    for (int t = tid; t < nh * KT; t += kTlThreads) {
+    for (int t = tid; t < nh * KT; t += S::NT) {
         const int m = t / KT, kk = t - m * KT;
         const int l = kk >> 1;
         const C2<T> Z = Y[l * LS + m];
@@ -690,27 +749,27 @@ __global__ void __launch_bounds__(kTlThreads)
     }
 }
 
-// K5 for nx = 256: A[kx][j][k0..k0+15] -> p(:, j, k) scaled by 1/(nx nz);
+// K5, two-level: A[kx][j][k0..k0+15] -> p(:, j, k) scaled by 1/(nx nz);
 // mode 0: p = val; mode 1: p = p + val - shift[0] (as k_xinv)
-template <class T>
-__global__ void __launch_bounds__(kTlThreads)
+template <class T, int N1, int N2>
+__global__ void __launch_bounds__((TlShape<N1, N2>::NT))
     k_xinv_tl(int ny, int nz, const C2<T>* __restrict__ tw, const C2<T>* __restrict__ A, double* __restrict__ p,
               int mode, const double* __restrict__ shift)
 {
-    constexpr int N = 256, L = kTlLines, KT = 2 * L, LS = N + 1, nh = N / 2 + 1;
+    using S = TlShape<N1, N2>;
+    constexpr int N = S::N, L = kTlLines, KT = 2 * L, LS = N + 1, nh = N / 2 + 1;
     constexpr int kB = stage_batch<T>() / 2;  // two complex loads per element
-    __shared__ C2<T> Y[kTlLds];
-    C2<T>* twl = Y + kTlLines * kTlLine;
+    __shared__ C2<T> Y[S::LDS];
+    C2<T>* twl = Y + kTlLines * S::LINE;
     const int j = blockIdx.y, k0 = blockIdx.x * KT;
     const int tid = threadIdx.x;
-    twl[tid] = tw[tid];
-    twl[tid + kTlThreads] = tw[tid + kTlThreads];
+    tl_load_twiddles<T, N1, N2>(twl, tw);
     // Z_l[m] = X1[m] + i X2[m] with Hermitian extension, into the lines (as k_xinv)
-    for (int t0 = tid; t0 < nh * L; t0 += kB * kTlThreads) {
+    for (int t0 = tid; t0 < nh * L; t0 += kB * S::NT) {
         T x1r[kB], x1i[kB], x2r[kB], x2i[kB];
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            const int t = min(t0 + u * kTlThreads, nh * L - 1);
+            const int t = min(t0 + u * S::NT, nh * L - 1);
             const int m = t / L, l = t - m * L;
             const size_t o = ((size_t)m * ny + j) * nz + k0 + 2 * l;
             const C2<T> a = A[o], b = A[o + 1];
@@ -721,7 +780,7 @@ __global__ void __launch_bounds__(kTlThreads)
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            const int t = t0 + u * kTlThreads;
+            const int t = t0 + u * S::NT;
             if (t < nh * L) {
                 const int m = t / L, l = t - m * L;
                 const C2<T> x1{x1r[u], x1i[u]}, x2{x2r[u], x2i[u]};
@@ -736,43 +795,82 @@ __global__ void __launch_bounds__(kTlThreads)
     }
     __syncthreads();
     {
-        const int l = tid >> 4, n2 = tid & 15;
-        C2<T> v[16];
+        const int l = tid / N2, n2 = tid - l * N2;
+        const bool act = tid < kTlLines * N2;
+        C2<T> v[N1];
+        if (act) {
 #pragma unroll
-        for (int n1 = 0; n1 < 16; ++n1) v[n1] = Y[l * LS + 16 * n1 + n2];
-        dft16<T, +1>(v);
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = Y[l * LS + N2 * n1 + n2];
+            DftReg<T, +1, N1>::run(v);
+        }
         __syncthreads();  // every line read before Y overwrites them
-        tl_store_y<T, +1>(Y, twl, l, n2, v);
+        if (act) tl_store_y<T, +1, N1, N2>(Y, twl, l, n2, v);
     }
     __syncthreads();
-    const int k1 = tid & 15, l = tid >> 4;  // lanes along the line
-    const C2<T>* y = Y + l * kTlLine + k1 * kTlRow;
-    C2<T> v[16];
+    if (tid >= kTlLines * N1) return;
+    const int l = tid / N1, k1 = tid - l * N1;  // lanes along the line
+    const C2<T>* y = Y + l * S::LINE + k1 * S::ROW;
+    C2<T> v[N2];
 #pragma unroll
-    for (int n2 = 0; n2 < 16; ++n2) v[n2] = y[n2];
-    dft16<T, +1>(v);
+    for (int n2 = 0; n2 < N2; ++n2) v[n2] = y[n2];
+    DftReg<T, +1, N2>::run(v);
     const double scale = 1.0 / ((double)N * (double)nz);
     const size_t plane = (size_t)N * ny;
     double* p0 = p + k1 + (size_t)N * (j + (size_t)ny * (k0 + 2 * l));
     if (mode == 0) {
 #pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) {
-            p0[16 * k2] = (double)v[k2].x * scale;
-            p0[16 * k2 + plane] = (double)v[k2].y * scale;
+        for (int k2 = 0; k2 < N2; ++k2) {
+            p0[N1 * k2] = (double)v[k2].x * scale;
+            p0[N1 * k2 + plane] = (double)v[k2].y * scale;
         }
         return;
     }
     const double sh = shift[0];
-    double a0[16], a1[16];
+    double a0[N2], a1[N2];
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) {
-        a0[k2] = p0[16 * k2];
-        a1[k2] = p0[16 * k2 + plane];
+    for (int k2 = 0; k2 < N2; ++k2) {
+        a0[k2] = p0[N1 * k2];
+        a1[k2] = p0[N1 * k2 + plane];
     }
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) {
-        p0[16 * k2] = a0[k2] + (double)v[k2].x * scale - sh;
-        p0[16 * k2 + plane] = a1[k2] + (double)v[k2].y * scale - sh;
+    for (int k2 = 0; k2 < N2; ++k2) {
+        p0[N1 * k2] = a0[k2] + (double)v[k2].x * scale - sh;
+        p0[N1 * k2 + plane] = a1[k2] + (double)v[k2].y * scale - sh;
+    }
+}
+
+// (N1, N2) of the two-level passes for a line length, or {0, 0} for the
+// Stockham passes; N1 (the step-A DFT, held in registers per thread) takes the
+// larger factor
+struct TlSplit {
+    int n1, n2;
+};
+static TlSplit tl_split(int n)
+{
+    switch (n) {
+        case 32: return {8, 4};
+        case 64: return {8, 8};
+        case 96: return {12, 8};
+        case 128: return {16, 8};
+        case 192: return {16, 12};
+        case 256: return {16, 16};
+        default: return {0, 0};
+    }
+}
+
+// f(integral_constant<N1>, integral_constant<N2>) for a length tl_split accepts
+template <class F>
+static void tl_dispatch(int n, F&& f)
+{
+    using std::integral_constant;
+    switch (n) {
+        case 32: f(integral_constant<int, 8>{}, integral_constant<int, 4>{}); break;
+        case 64: f(integral_constant<int, 8>{}, integral_constant<int, 8>{}); break;
+        case 96: f(integral_constant<int, 12>{}, integral_constant<int, 8>{}); break;
+        case 128: f(integral_constant<int, 16>{}, integral_constant<int, 8>{}); break;
+        case 192: f(integral_constant<int, 16>{}, integral_constant<int, 12>{}); break;
+        case 256: f(integral_constant<int, 16>{}, integral_constant<int, 16>{}); break;
+        default: break;
     }
 }
 
@@ -1053,7 +1151,7 @@ struct tbfg_fps {
     int precision = 0, refine = 0;
     int nx = 0, ny = 0, nz = 0, nh = 0;
     int KT = 0, JT = 0, MR = 1;
-    bool tlx = false, tlz = false;  // two-level passes (N = 256) along x / z
+    bool tlx = false, tlz = false;  // two-level passes along x / z (tl_split)
     FftPlan px{}, pz{};
     DevBuf A, B;               // spectral work arrays (complex, T)
     DevBuf tab;                // lx, lz, am, ap, dyf, vol_y (double)
@@ -1181,28 +1279,31 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
     h->ot.ap = h->tt.ap;
     h->ot.vol_y = tb + o_vy;
     h->inv_vg = 1.0 / (ly * L.ax[0].df(1) * nx * L.ax[2].df(1) * nz);
-    // two-level passes where the line length is 256 and the workgroup's lines
-    // tile the other axis (TBFG_FFT_TWO_LEVEL=0 keeps fft_lines: an A/B switch)
+    // two-level passes where the line length splits (tl_split) and the
+    // workgroup's lines tile the other axis (TBFG_FFT_TWO_LEVEL=0 keeps
+    // fft_lines: an A/B switch)
     const char* tl_env = std::getenv("TBFG_FFT_TWO_LEVEL");
     const bool tl_on = !(tl_env && tl_env[0] == '0');
-    h->tlx = tl_on && nx == 256 && nz % (2 * kTlLines) == 0;
-    h->tlz = tl_on && nz == 256 && ny % kTlLines == 0;
+    h->tlx = tl_on && tl_split(nx).n1 > 0 && nz % (2 * kTlLines) == 0;
+    h->tlz = tl_on && tl_split(nz).n1 > 0 && ny % kTlLines == 0;
     // twiddles in FP64 and rounded to FP32.  fft_lines: entry Ns-1 + (r-1)*Ns + k
     // of the pass with sub-transform length Ns and radix R is
-    // exp(-2 pi i r k / (Ns R)) (entry n-1 unused); two-level passes: entry
-    // 16 k1 + n2 is exp(-2 pi i n2 k1 / 256)
+    // exp(-2 pi i r k / (Ns R)) (entry n-1 unused); two-level passes (n = N1 N2):
+    // entry N2 k1 + n2 is exp(-2 pi i n2 k1 / n)
     auto tw = [&](int n, bool two_level, DevBuf& d64, DevBuf& d32) -> int {
         FftPlan pl;
         make_plan(n, pl);
         std::vector<double> w(2 * (size_t)n, 0.0);
         std::vector<float> wf(2 * (size_t)n, 0.0f);
-        if (two_level)
-            for (int k1 = 0; k1 < 16; ++k1)
-                for (int n2 = 0; n2 < 16; ++n2) {
-                    const size_t e = (size_t)16 * k1 + n2;
+        if (two_level) {
+            const TlSplit sp = tl_split(n);
+            for (int k1 = 0; k1 < sp.n1; ++k1)
+                for (int n2 = 0; n2 < sp.n2; ++n2) {
+                    const size_t e = (size_t)sp.n2 * k1 + n2;
                     w[2 * e] = std::cos(2.0 * pi * n2 * k1 / n);
                     w[2 * e + 1] = -std::sin(2.0 * pi * n2 * k1 / n);
                 }
+        }
         int Ns = 1;
         for (int st = 0; st < pl.nrad && !two_level; ++st) {
             const int R = pl.rad[st];
@@ -1260,13 +1361,20 @@ static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipSt
     const size_t ldx = ((size_t)(h->KT / 2) * (nx + 1) + nx) * sizeof(C2<T>);  // padded lines + twiddles
     const size_t ldz = ((size_t)h->JT * (nz + 1) + nz) * sizeof(C2<T>);
     if (h->tlx)
-        hipLaunchKernelGGL((k_xfwd_tl<T, TIN>), dim3(nz / (2 * kTlLines), ny), dim3(kTlThreads), 0, s, ny, nz, twx, in,
-                           A);
+        tl_dispatch(nx, [&](auto a, auto b) {
+            constexpr int N1 = decltype(a)::value, N2 = decltype(b)::value;
+            hipLaunchKernelGGL((k_xfwd_tl<T, TIN, N1, N2>), dim3(nz / (2 * kTlLines), ny), dim3(TlShape<N1, N2>::NT), 0,
+                               s, ny, nz, twx, in, A);
+        });
     else
         hipLaunchKernelGGL((k_xfwd<T, TIN>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px,
                            twx, in, A);
     if (h->tlz)
-        hipLaunchKernelGGL((k_zfwd_tl<T>), dim3((ny / kTlLines) * nh), dim3(kTlThreads), 0, s, ny, twz, A, B);
+        tl_dispatch(nz, [&](auto a, auto b) {
+            constexpr int N1 = decltype(a)::value, N2 = decltype(b)::value;
+            hipLaunchKernelGGL((k_zfwd_tl<T, N1, N2>), dim3((ny / kTlLines) * nh), dim3(TlShape<N1, N2>::NT), 0, s, ny,
+                               twz, A, B);
+        });
     else
         hipLaunchKernelGGL((k_zfwd<T>), dim3((ny / h->JT) * nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz,
                            A, B);
@@ -1279,13 +1387,20 @@ static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipSt
         default: launch_ysolve<T, 16>(h, B, s); break;
     }
     if (h->tlz)
-        hipLaunchKernelGGL((k_zinv_tl<T>), dim3((ny / kTlLines) * nh), dim3(kTlThreads), 0, s, ny, twz, B, A);
+        tl_dispatch(nz, [&](auto a, auto b) {
+            constexpr int N1 = decltype(a)::value, N2 = decltype(b)::value;
+            hipLaunchKernelGGL((k_zinv_tl<T, N1, N2>), dim3((ny / kTlLines) * nh), dim3(TlShape<N1, N2>::NT), 0, s, ny,
+                               twz, B, A);
+        });
     else
         hipLaunchKernelGGL((k_zinv<T>), dim3((ny / h->JT) * nh), dim3(kFftThreads), ldz, s, ny, nz, h->JT, h->pz, twz,
                            B, A);
     if (h->tlx)
-        hipLaunchKernelGGL((k_xinv_tl<T>), dim3(nz / (2 * kTlLines), ny), dim3(kTlThreads), 0, s, ny, nz, twx, A, p,
-                           mode, h->mean.as<double>());
+        tl_dispatch(nx, [&](auto a, auto b) {
+            constexpr int N1 = decltype(a)::value, N2 = decltype(b)::value;
+            hipLaunchKernelGGL((k_xinv_tl<T, N1, N2>), dim3(nz / (2 * kTlLines), ny), dim3(TlShape<N1, N2>::NT), 0, s,
+                               ny, nz, twx, A, p, mode, h->mean.as<double>());
+        });
     else
         hipLaunchKernelGGL((k_xinv<T>), dim3(nz / h->KT, ny), dim3(kFftThreads), ldx, s, nx, ny, nz, h->KT, h->px, twx,
                            A, p, mode, h->mean.as<double>());
