@@ -8,9 +8,9 @@
 //                  + parallel cyclic reduction across the 64 lanes (:280-333)
 //   K4 z-inverse : B -> A                                           (:366-367)
 //   K5 x-inverse : A -> real lines, scaled by 1/(nx nz)             (:370-374)
-// K1/K2/K4/K5 run two-level kernels (k_*_tl: 16-point DFTs in registers, one
-// LDS exchange per line) for 256-point lines and LDS-staged Stockham passes
-// (fft_lines) for the other lengths.
+// K1/K2/K4/K5 run two-level kernels (k_*_tl: in-register DFTs of the two
+// factors of the length, one LDS exchange per line) for lines of 32-256 points
+// (tl_split) and LDS-staged Stockham passes (fft_lines) for the other lengths.
 // The pencil transposes of pencilDec.f90 become the layout changes folded into
 // the loads and stores of K1/K2/K4/K5.  The volume-weighted mean removal
 // (:388-423) is done exactly in spectral space on the (0,0) column, where

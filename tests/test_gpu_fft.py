@@ -32,8 +32,9 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-# (256-point x or z lines run the two-level passes, the others the Stockham
-# passes: (256, 16, 32) and (32, 16, 256) mix the two.  The last two: the
+# (lines of 32, 64, 96, 128, 192 or 256 points run the two-level passes where
+# the other axis tiles into their workgroups, the others the Stockham passes:
+# (32, 24, 16), (96, 32, 48) and (192, 128, 96) mix the two.  The last two: the
 # longest transform the solver takes, one FFT line pair per workgroup, and the
 # tallest y column, 16 rows per lane of the tridiagonal solve)
 CASES = [((64, 64, 64), 0), ((32, 24, 16), 1), ((96, 32, 48), 1), ((256, 192, 256), 1), ((192, 128, 96), 0),
@@ -111,13 +112,13 @@ def test_fft_solve_full_size_against_oracle(torch_gpu):
         fs.close()
 
 
+@pytest.mark.parametrize("n", [(256, 48, 256), (128, 32, 192), (96, 16, 64), (32, 8, 32)])
 @pytest.mark.parametrize("precision,bound", [("fp64", 1e-13), ("mixed", 1e-5)])
-def test_two_level_passes_equal_stockham_passes(torch_gpu, monkeypatch, precision, bound):
-    """256-point lines: the two-level passes against the Stockham passes of the
-    same lengths (TBFG_FFT_TWO_LEVEL=0, read when a handle is created), on the
-    same right-hand side; mixed precision without refinement compares the FP32
-    transforms themselves."""
-    n = (256, 48, 256)
+def test_two_level_passes_equal_stockham_passes(torch_gpu, monkeypatch, n, precision, bound):
+    """Every two-level split (16x16, 16x8 and 16x12, 12x8 and 8x8, 8x4) against
+    the Stockham passes of the same lengths (TBFG_FFT_TWO_LEVEL=0, read when a
+    handle is created), on the same right-hand side; mixed precision without
+    refinement compares the FP32 transforms themselves."""
     print("case two-level vs stockham", precision, n, flush=True)
     g, go = grids(n, 1)
     V = vol(go)
