@@ -79,7 +79,9 @@ typedef struct tbfg_fps_opts {
 
 /* Replaces fastPoissonSolverCTOR (src/fastPoissonSolver/fastPoissonSolver.f90:123-156):
  * eigenvalues, y coefficients and device work buffers.  x and z must be uniform,
- * nx and nz in {2^a 3^b}, nx even, n <= 1024 along x and z, ny <= 1024. */
+ * nx and nz in {2^a 3^b}, even, at most 4096 (beyond 1024 points only lengths whose every FFT pass
+ * has at most 1024 butterflies in FP64, 2048 in mixed precision: n / r <= that for each radix r of
+ * n = 4^a 2^b 3^c, so every length up to 2048, and 3072 and 4096), ny <= 1024. */
 int tbfg_fps_create(const tbfg_grid_desc* grid, const tbfg_fps_opts* opts, tbfg_fps** out);
 
 /* Replaces solveFPS (fastPoissonSolver.f90:337-384): solves lap(p) = s with the

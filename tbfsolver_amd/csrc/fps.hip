@@ -51,6 +51,7 @@ constexpr int kFftThreads = 256;
 template <class T>
 constexpr int max_bfly() { return sizeof(T) == 8 ? 4 : 8; }
 constexpr int kLdsElems = 4096;    // complex elements (lines + twiddles) per block: 64 KB in FP64
+constexpr int kLdsElemsMax = 10240;  // lines longer than 1024 points, one per block: 160 KB in FP64
 
 // Radices of each Stockham pass, and per pass the multipliers that divide by
 // the pass's butterfly count N/R and its sub-transform length Ns with one
@@ -1170,8 +1171,17 @@ namespace tbfg {
 static unsigned row_threads(int nx) { return nx >= 256 ? 256u : 64u; }
 static unsigned row_blocks(int nx) { return ceil_div(nx, row_threads(nx)); }
 
-static int choose_lines(int n, int count, int pairs, int bfly)
+static int choose_lines(int n, int count, int pairs, int bfly, const FftPlan& plan)
 {
+    // lines longer than 1024 points: one line (pair) per workgroup, its LDS
+    // beyond the default 64 KB (set_stockham_lds), if the plan's largest pass
+    // (n over its smallest radix) fits the workgroup's butterflies
+    if (n > 1024) {
+        int nb = 0;
+        for (int st = 0; st < plan.nrad; ++st) nb = std::max(nb, n / plan.rad[st]);
+        if (2 * n + 1 > kLdsElemsMax || nb > bfly * kFftThreads) return 0;
+        return count % (pairs ? 2 : 1) == 0 ? 1 : 0;
+    }
     // largest L with L*(n+1)+n <= kLdsElems (padded lines plus twiddles) and L*n/2 <=
     // bfly * threads (the butterflies of a radix-2 pass), (pairs ? 2L : L)
     // dividing count, L <= 64 (preferring whole 128-byte lines in the z passes'
@@ -1185,6 +1195,34 @@ static int choose_lines(int n, int count, int pairs, int bfly)
         if (count % span == 0) best = L;
     }
     return best;
+}
+
+// Dynamic LDS of the Stockham passes (padded lines plus twiddles) in bytes
+template <class T>
+static size_t stockham_lds_x(const tbfg_fps* h) { return ((size_t)(h->KT / 2) * (h->nx + 1) + h->nx) * sizeof(C2<T>); }
+template <class T>
+static size_t stockham_lds_z(const tbfg_fps* h) { return ((size_t)h->JT * (h->nz + 1) + h->nz) * sizeof(C2<T>); }
+
+// Lines longer than 1024 points take more than the default 64 KB of dynamic
+// LDS: raise the kernels' limit (gfx950 gives a workgroup up to 160 KB)
+template <class T, class TIN>
+static int set_stockham_lds(const tbfg_fps* h)
+{
+    const size_t lx = stockham_lds_x<T>(h), lz = stockham_lds_z<T>(h);
+    auto raise = [](const void* f, size_t bytes) -> int {
+        if (bytes <= 65536) return TBFG_OK;
+        TBFG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+        return TBFG_OK;
+    };
+    if (!h->tlx) {
+        TBFG_TRY(raise(reinterpret_cast<const void*>(&k_xfwd<T, TIN>), lx));
+        TBFG_TRY(raise(reinterpret_cast<const void*>(&k_xinv<T>), lx));
+    }
+    if (!h->tlz) {
+        TBFG_TRY(raise(reinterpret_cast<const void*>(&k_zfwd<T>), lz));
+        TBFG_TRY(raise(reinterpret_cast<const void*>(&k_zinv<T>), lz));
+    }
+    return TBFG_OK;
 }
 
 int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
@@ -1208,8 +1246,8 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
         return TBFG_EINVAL;
     }
     const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
-    if (nx % 2 || nz % 2 || nx > 1024 || nz > 1024 || ny > 1024 || nx < 2 || nz < 2) {
-        set_error("FFT solver needs even nx, nz <= 1024 and ny <= 1024 (got %d x %d x %d)", nx, ny, nz);
+    if (nx % 2 || nz % 2 || nx > 4096 || nz > 4096 || ny > 1024 || nx < 2 || nz < 2) {
+        set_error("FFT solver needs even nx, nz <= 4096 and ny <= 1024 (got %d x %d x %d)", nx, ny, nz);
         return TBFG_EINVAL;
     }
     if (make_plan(nx, h->px) || make_plan(nz, h->pz)) {
@@ -1221,15 +1259,11 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
     h->nz = nz;
     h->nh = nx / 2 + 1;
     const int bfly = h->precision == TBFG_PREC_MIXED ? max_bfly<float>() : max_bfly<double>();
-    h->KT = 2 * choose_lines(nx, nz, 1, bfly);
-    h->JT = choose_lines(nz, ny, 0, bfly);
+    h->KT = 2 * choose_lines(nx, nz, 1, bfly, h->px);
+    h->JT = choose_lines(nz, ny, 0, bfly, h->pz);
     if (h->KT < 2 || h->JT < 1) {
-        set_error("no line tiling for %d x %d x %d", nx, ny, nz);
-        return TBFG_EINVAL;
-    }
-    // butterflies per thread: L*N/2 <= max_bfly * threads
-    if ((h->KT / 2) * nx / 2 > bfly * kFftThreads || h->JT * nz / 2 > bfly * kFftThreads) {
-        set_error("line tiling exceeds the per-thread butterfly budget");
+        set_error("no line tiling for %d x %d x %d (a line longer than 1024 points needs every FFT pass "
+                  "within %d butterflies)", nx, ny, nz, bfly * kFftThreads);
         return TBFG_EINVAL;
     }
     const int m = (ny + 63) / 64;
@@ -1286,6 +1320,8 @@ int fps_create(const tbfg_grid_desc* g, const tbfg_fps_opts* o, tbfg_fps** out)
     const bool tl_on = !(tl_env && tl_env[0] == '0');
     h->tlx = tl_on && tl_split(nx).n1 > 0 && nz % (2 * kTlLines) == 0;
     h->tlz = tl_on && tl_split(nz).n1 > 0 && ny % kTlLines == 0;
+    TBFG_TRY(h->precision == TBFG_PREC_MIXED ? set_stockham_lds<float, double>(h.get())
+                                             : set_stockham_lds<double, double>(h.get()));
     // twiddles in FP64 and rounded to FP32.  fft_lines: entry Ns-1 + (r-1)*Ns + k
     // of the pass with sub-transform length Ns and radix R is
     // exp(-2 pi i r k / (Ns R)) (entry n-1 unused); two-level passes (n = N1 N2):
@@ -1358,8 +1394,8 @@ static int spectral_solve(tbfg_fps* h, const TIN* in, double* p, int mode, hipSt
     const bool f32 = sizeof(T) == 4;
     const C2<T>* twx = f32 ? h->twxf.as<C2<T>>() : h->twx.as<C2<T>>();
     const C2<T>* twz = f32 ? h->twzf.as<C2<T>>() : h->twz.as<C2<T>>();
-    const size_t ldx = ((size_t)(h->KT / 2) * (nx + 1) + nx) * sizeof(C2<T>);  // padded lines + twiddles
-    const size_t ldz = ((size_t)h->JT * (nz + 1) + nz) * sizeof(C2<T>);
+    const size_t ldx = stockham_lds_x<T>(h);
+    const size_t ldz = stockham_lds_z<T>(h);
     if (h->tlx)
         tl_dispatch(nx, [&](auto a, auto b) {
             constexpr int N1 = decltype(a)::value, N2 = decltype(b)::value;

@@ -111,7 +111,7 @@ def test_fft_pass_division_by_multiply_high():
     """The FFT passes divide butterfly indices (< 2^15) by the pass's
     sub-lengths with mulhi(x, ceil(2^32 / d)) instead of an integer division:
     exact for every divisor that a length the solver accepts (2^a 3^b, even,
-    <= 1024) produces -- n / radix and the partial radix products."""
+    <= 4096) produces -- n / radix and the partial radix products."""
     def radices(n):
         out, m = [], n
         for r in (4, 2, 3):
@@ -121,7 +121,7 @@ def test_fft_pass_division_by_multiply_high():
         return out if m == 1 else None
 
     divisors = set()
-    for n in range(2, 1025, 2):
+    for n in range(2, 4097, 2):
         rad = radices(n)
         if rad is None:
             continue
