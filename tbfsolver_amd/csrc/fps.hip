@@ -579,17 +579,29 @@ __device__ __forceinline__ C2<T> tw_apply(C2<T> v, C2<T> w)
     return cmul(v, w);
 }
 
-// The pass's twiddle table W_N^(n2 k1) (entry N2 k1 + n2) into LDS
+// The pass's twiddle table W_N^(n2 k1) (entry N2 k1 + n2): fetched into
+// registers before the line loads are issued (clamped, unconditional loads), so
+// that storing it into LDS waits for these loads only and leaves the line loads
+// in flight
 template <class T, int N1, int N2>
-__device__ __forceinline__ void tl_load_twiddles(C2<T>* twl, const C2<T>* __restrict__ tw)
-{
+struct TlTwiddles {
     using S = TlShape<N1, N2>;
+    static constexpr int U = (S::N + S::NT - 1) / S::NT;
+    C2<T> w[U];
+    __device__ __forceinline__ void fetch(const C2<T>* __restrict__ tw)
+    {
 #pragma unroll
-    for (int u = 0; u < (S::N + S::NT - 1) / S::NT; ++u) {
-        const int t = (int)threadIdx.x + u * S::NT;
-        if (t < S::N) twl[t] = tw[t];
+        for (int u = 0; u < U; ++u) w[u] = tw[min((int)threadIdx.x + u * S::NT, S::N - 1)];
     }
-}
+    __device__ __forceinline__ void put(C2<T>* twl) const
+    {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = (int)threadIdx.x + u * S::NT;
+            if (S::N % S::NT == 0 || t < S::N) twl[t] = w[u];
+        }
+    }
+};
 
 // Step A's tail: twiddle W_N^(n2 k1) and store Y[k1][n2] of line l
 template <class T, int SIGN, int N1, int N2>
@@ -615,16 +627,18 @@ __global__ void __launch_bounds__((TlShape<N1, N2>::NT))
     const int m = (int)(id / gj);
     const int j0 = (int)(id % gj) * kTlLines;
     const int tid = threadIdx.x;
+    __builtin_assume(tid < S::NT);  // the launch's block size
+    TlTwiddles<T, N1, N2> tws;
+    tws.fetch(tw);
     {
         const int l = tid / N2, n2 = tid - l * N2;  // lanes along the line
         const bool act = tid < kTlLines * N2;
+        // unconditional loads (idle threads read line 0), so no branch waits for them
+        const C2<T>* src = A + ((size_t)m * ny + j0 + (act ? l : 0)) * N + n2;
         C2<T> v[N1];
-        if (act) {
-            const C2<T>* src = A + ((size_t)m * ny + j0 + l) * N + n2;
 #pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[N2 * n1];
-        }
-        tl_load_twiddles<T, N1, N2>(twl, tw);
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[N2 * n1];
+        tws.put(twl);
         __syncthreads();
         if (act) {
             DftReg<T, -1, N1>::run(v);
@@ -658,16 +672,17 @@ __global__ void __launch_bounds__((TlShape<N1, N2>::NT))
     const int m = (int)(id / gj);
     const int j0 = (int)(id % gj) * kTlLines;
     const int tid = threadIdx.x;
+    __builtin_assume(tid < S::NT);  // the launch's block size
+    TlTwiddles<T, N1, N2> tws;
+    tws.fetch(tw);
     {
         const int l = tid & (kTlLines - 1), n2 = tid / kTlLines;  // lanes across lines
         const bool act = tid < kTlLines * N2;
+        const C2<T>* src = B + ((size_t)m * N + (act ? n2 : 0)) * ny + j0 + l;
         C2<T> v[N1];
-        if (act) {
-            const C2<T>* src = B + ((size_t)m * N + n2) * ny + j0 + l;
 #pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[(size_t)N2 * n1 * ny];
-        }
-        tl_load_twiddles<T, N1, N2>(twl, tw);
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[(size_t)N2 * n1 * ny];
+        tws.put(twl);
         __syncthreads();
         if (act) {
             DftReg<T, +1, N1>::run(v);
@@ -700,17 +715,18 @@ __global__ void __launch_bounds__((TlShape<N1, N2>::NT))
     C2<T>* twl = Y + kTlLines * S::LINE;
     const int j = blockIdx.y, k0 = blockIdx.x * KT;
     const int tid = threadIdx.x;
+    __builtin_assume(tid < S::NT);  // the launch's block size
+    TlTwiddles<T, N1, N2> tws;
+    tws.fetch(tw);
     {
         const int l = tid / N2, n2 = tid - l * N2;  // lanes along the line
         const bool act = tid < kTlLines * N2;
+        const size_t plane = (size_t)N * ny;
+        const TIN* s0 = s + n2 + (size_t)N * (j + (size_t)ny * (k0 + 2 * (act ? l : 0)));
         C2<T> v[N1];
-        if (act) {
-            const size_t plane = (size_t)N * ny;
-            const TIN* s0 = s + n2 + (size_t)N * (j + (size_t)ny * (k0 + 2 * l));
 #pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) v[n1] = C2<T>{(T)s0[N2 * n1], (T)s0[N2 * n1 + plane]};
-        }
-        tl_load_twiddles<T, N1, N2>(twl, tw);
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = C2<T>{(T)s0[N2 * n1], (T)s0[N2 * n1 + plane]};
+        tws.put(twl);
         __syncthreads();
         if (act) {
             DftReg<T, -1, N1>::run(v);
@@ -764,7 +780,10 @@ __global__ void __launch_bounds__((TlShape<N1, N2>::NT))
     C2<T>* twl = Y + kTlLines * S::LINE;
     const int j = blockIdx.y, k0 = blockIdx.x * KT;
     const int tid = threadIdx.x;
-    tl_load_twiddles<T, N1, N2>(twl, tw);
+    __builtin_assume(tid < S::NT);  // the launch's block size
+    TlTwiddles<T, N1, N2> tws;
+    tws.fetch(tw);
+    tws.put(twl);  // here: the compiler would otherwise issue the fetch after the staging loop
     // Z_l[m] = X1[m] + i X2[m] with Hermitian extension, into the lines (as k_xinv)
     for (int t0 = tid; t0 < nh * L; t0 += kB * S::NT) {
         T x1r[kB], x1i[kB], x2r[kB], x2i[kB];
@@ -880,6 +899,74 @@ static void tl_dispatch(int n, F&& f)
 template <class T>
 __device__ __forceinline__ T shfl_t(T v, int src) { return __shfl(v, src, 64); }
 
+// The value of lane - 1 (lane 0 gets 0) or lane + 1 (lane 63 gets 0) of the
+// wavefront by a DPP wave shift, a VALU move; a shuffle goes through the LDS
+// crossbar, which the y solve's exchanges keep busy
+__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true); }
+
+// The values of lanes lane - S and lane + S of a 32-bit register; lanes whose
+// neighbour lies outside the wavefront get an unspecified value.  Stride 1 by
+// DPP wave shifts, the others by shuffles (permlane16/32 swaps at strides 16
+// and 32, and two wave shifts at stride 2, measured slower than the shuffles).
+template <int S>
+__device__ __forceinline__ void nbr32(int u, int lane, int& prev, int& next)
+{
+    if constexpr (S == 1) {
+        prev = dpp_prev(u);
+        next = dpp_next(u);
+    } else {
+        prev = __shfl(u, lane - S, 64);
+        next = __shfl(u, lane + S, 64);
+    }
+}
+template <int S>
+__device__ __forceinline__ void lane_nbrs(float v, int lane, float& prev, float& next)
+{
+    int p, n;
+    nbr32<S>(__builtin_bit_cast(int, v), lane, p, n);
+    prev = __builtin_bit_cast(float, p);
+    next = __builtin_bit_cast(float, n);
+}
+template <int S>
+__device__ __forceinline__ void lane_nbrs(double v, int lane, double& prev, double& next)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    int plo, nlo, phi, nhi;
+    nbr32<S>((int)(unsigned)(b & 0xffffffffull), lane, plo, nlo);
+    nbr32<S>((int)(unsigned)(b >> 32), lane, phi, nhi);
+    prev = __builtin_bit_cast(double, ((unsigned long long)(unsigned)phi << 32) | (unsigned)plo);
+    next = __builtin_bit_cast(double, ((unsigned long long)(unsigned)nhi << 32) | (unsigned)nlo);
+}
+
+// One parallel-cyclic-reduction step at stride S over the 64 lanes' rows
+// A x_{i-S} + x_i + C x_{i+S} = D (unit diagonal kept, so no B is exchanged).
+template <int S, class TC>
+__device__ __forceinline__ void pcr_step(TC& A, TC& C, C2<TC>& D, int lane)
+{
+    TC Am, Cm, Ap, Cp;
+    C2<TC> Dm, Dp;
+    lane_nbrs<S>(A, lane, Am, Ap);
+    lane_nbrs<S>(C, lane, Cm, Cp);
+    lane_nbrs<S>(D.x, lane, Dm.x, Dp.x);
+    lane_nbrs<S>(D.y, lane, Dm.y, Dp.y);
+    if (lane - S < 0) {
+        Am = 0;
+        Cm = 0;
+        Dm = C2<TC>{0, 0};
+    }
+    if (lane + S > 63) {
+        Ap = 0;
+        Cp = 0;
+        Dp = C2<TC>{0, 0};
+    }
+    const TC inv = (TC)1 / ((TC)1 - Cm * A - Ap * C);
+    D = C2<TC>{(D.x - Dm.x * A - Dp.x * C) * inv, (D.y - Dm.y * A - Dp.y * C) * inv};
+    const TC A2 = -Am * A * inv, C2v = -Cp * C * inv;
+    A = A2;
+    C = C2v;
+}
+
 // Solves am_j x_{j-1} + (lam - am_j - ap_j) x_j + ap_j x_{j+1} = d_j for one
 // column with one wavefront.  Lane p owns rows p*m..p*m+m-1 (m <= MR).  Each
 // lane eliminates its block to an affine map of the last unknowns of its own
@@ -975,8 +1062,12 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
             de = d[r];
         }
     // first-row map of the next block
-    C2<TC> na = {shfl_t(fa.x, lane + 1), shfl_t(fa.y, lane + 1)};
-    TC nb_ = shfl_t(fb, lane + 1), ng = shfl_t(fg, lane + 1);
+    C2<TC> na, dummy;
+    TC nb_, ng, dm;
+    lane_nbrs<1>(fa.x, lane, dummy.x, na.x);
+    lane_nbrs<1>(fa.y, lane, dummy.y, na.y);
+    lane_nbrs<1>(fb, lane, dm, nb_);
+    lane_nbrs<1>(fg, lane, dm, ng);
     if (lane == 63) {
         na = C2<TC>{0, 0};
         nb_ = 0;
@@ -987,32 +1078,17 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
     TC A = ae * lb * inv0;
     TC C = ce * ng * inv0;
     C2<TC> D = {(de.x - ae * la.x - ce * na.x) * inv0, (de.y - ae * la.y - ce * na.y) * inv0};
-    // parallel cyclic reduction over the 64 lanes; every row keeps a unit
-    // diagonal, so the eliminations need no division and B is never exchanged
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-        TC Am = shfl_t(A, lane - s), Cm = shfl_t(C, lane - s);
-        C2<TC> Dm = {shfl_t(D.x, lane - s), shfl_t(D.y, lane - s)};
-        TC Ap = shfl_t(A, lane + s), Cp = shfl_t(C, lane + s);
-        C2<TC> Dp = {shfl_t(D.x, lane + s), shfl_t(D.y, lane + s)};
-        if (lane - s < 0) {
-            Am = 0;
-            Cm = 0;
-            Dm = C2<TC>{0, 0};
-        }
-        if (lane + s > 63) {
-            Ap = 0;
-            Cp = 0;
-            Dp = C2<TC>{0, 0};
-        }
-        const TC inv = (TC)1 / ((TC)1 - Cm * A - Ap * C);
-        D = C2<TC>{(D.x - Dm.x * A - Dp.x * C) * inv, (D.y - Dm.y * A - Dp.y * C) * inv};
-        const TC A2 = -Am * A * inv, C2v = -Cp * C * inv;
-        A = A2;
-        C = C2v;
-    }
+    // parallel cyclic reduction over the 64 lanes
+    pcr_step<1>(A, C, D, lane);
+    pcr_step<2>(A, C, D, lane);
+    pcr_step<4>(A, C, D, lane);
+    pcr_step<8>(A, C, D, lane);
+    pcr_step<16>(A, C, D, lane);
+    pcr_step<32>(A, C, D, lane);
     const C2<TC> y = D;
-    C2<TC> ym = {shfl_t(y.x, lane - 1), shfl_t(y.y, lane - 1)};
+    C2<TC> ym;
+    lane_nbrs<1>(y.x, lane, ym.x, dummy.x);
+    lane_nbrs<1>(y.y, lane, ym.y, dummy.y);
     if (lane == 0) ym = C2<TC>{0, 0};
     C2<TC> x[MR];
 #pragma unroll
