@@ -657,7 +657,8 @@ def main():
         str_gbs = out["iterations"] * streamed / (out["p50_ms"] * 1e-3) / 1e9
         out["solve_roofline"] = {"bytes_per_iteration": per_it, "achieved": round(solve_gbs, 1), "unit": "GB/s",
                                  "frac": round(solve_gbs / HBM_PEAK_GBS, 4),
-                                 "model": "the survey's algorithmic model (unfused CG vectors 128 N)",
+                                 "model": "algorithmic bytes of an unfused PCG iteration: CG vectors 128 B per finest cell, "
+                                          "plus the multigrid's arrays on every level",
                                  "streamed_bytes_per_iteration": streamed,
                                  "streamed_achieved": round(str_gbs, 1),
                                  "streamed_frac": round(str_gbs / HBM_PEAK_GBS, 4),
