@@ -939,6 +939,23 @@ __device__ __forceinline__ void lane_nbrs(double v, int lane, double& prev, doub
     next = __builtin_bit_cast(double, ((unsigned long long)(unsigned)nhi << 32) | (unsigned)nlo);
 }
 
+// 1/x by the hardware reciprocal refined with Newton steps (two in FP64, one
+// in FP32: about one ulp), without the scaling and fix-up of a correctly
+// rounded division: the y systems are strictly diagonally dominant (or, for
+// the pinned column, have a unit diagonal after scaling), so x is never zero,
+// denormal or infinite
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ float rcp_nr(float x)
+{
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+
 // One parallel-cyclic-reduction step at stride S over the 64 lanes' rows
 // A x_{i-S} + x_i + C x_{i+S} = D (unit diagonal kept, so no B is exchanged).
 template <int S, class TC>
@@ -960,7 +977,7 @@ __device__ __forceinline__ void pcr_step(TC& A, TC& C, C2<TC>& D, int lane)
         Cp = 0;
         Dp = C2<TC>{0, 0};
     }
-    const TC inv = (TC)1 / ((TC)1 - Cm * A - Ap * C);
+    const TC inv = rcp_nr((TC)1 - Cm * A - Ap * C);
     D = C2<TC>{(D.x - Dm.x * A - Dp.x * C) * inv, (D.y - Dm.y * A - Dp.y * C) * inv};
     const TC A2 = -Am * A * inv, C2v = -Cp * C * inv;
     A = A2;
@@ -1016,13 +1033,13 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
                 const TC rb = (r == 0) ? -a[0] : (TC)0;
                 // one division per row: its reciprocal scales the row's four results
                 if (r == 0) {
-                    const TC inv = (TC)1 / b[0];
+                    const TC inv = rcp_nr(b[0]);
                     al[0] = C2<TC>{d[0].x * inv, d[0].y * inv};
                     be[0] = rb * inv;
                     ga[0] = rg * inv;
                     cp[0] = cint * inv;
                 } else {
-                    const TC inv = (TC)1 / (b[r] - a[r] * cp[r - 1]);
+                    const TC inv = rcp_nr(b[r] - a[r] * cp[r - 1]);
                     al[r] = C2<TC>{(d[r].x - a[r] * al[r - 1].x) * inv, (d[r].y - a[r] * al[r - 1].y) * inv};
                     be[r] = (rb - a[r] * be[r - 1]) * inv;
                     ga[r] = (rg - a[r] * ga[r - 1]) * inv;
@@ -1074,7 +1091,7 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
         ng = 0;
     }
     // the reduced row, scaled to a unit diagonal
-    const TC inv0 = (TC)1 / (ae * lg + bee + ce * nb_);
+    const TC inv0 = rcp_nr(ae * lg + bee + ce * nb_);
     TC A = ae * lb * inv0;
     TC C = ce * ng * inv0;
     C2<TC> D = {(de.x - ae * la.x - ce * na.x) * inv0, (de.y - ae * la.y - ce * na.y) * inv0};
