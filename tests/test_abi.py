@@ -41,6 +41,18 @@ def test_default_params_are_the_reference_channel_values():
     assert p.precision == 0 and p.flexible == -1
 
 
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without the built HIP library every entry point raises: no host fallback
+    stands in for the device path."""
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libtbfgpu.so"))
+    g = T.channel_grid((64, 64, 64))
+    for make in (lambda: T.FastPoissonSolver(g), lambda: T.PCG(g, nLevels=4),
+                 lambda: F.max_divergence(g, None, None, None), lambda: _lib.lib()):
+        with pytest.raises(RuntimeError, match="not built"):
+            make()
+
+
 def test_invalid_grids_and_hierarchies_are_rejected():
     g = T.channel_grid((96, 64, 100))
     with pytest.raises(T.TbfgError) as e:
