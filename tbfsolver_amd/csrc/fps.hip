@@ -967,16 +967,12 @@ __device__ __forceinline__ void pcr_step(TC& A, TC& C, C2<TC>& D, int lane)
     lane_nbrs<S>(C, lane, Cm, Cp);
     lane_nbrs<S>(D.x, lane, Dm.x, Dp.x);
     lane_nbrs<S>(D.y, lane, Dm.y, Dp.y);
-    if (lane - S < 0) {
-        Am = 0;
-        Cm = 0;
-        Dm = C2<TC>{0, 0};
-    }
-    if (lane + S > 63) {
-        Ap = 0;
-        Cp = 0;
-        Dp = C2<TC>{0, 0};
-    }
+    // Rows beyond the wavefront need no zeroing: at the start of the step at
+    // stride S, A = 0 exactly on lanes below S (lane 0's row has no lower
+    // coupling, the wall's a- being 0, and A' = -A(i-S) A inv keeps the band
+    // of zeros doubling) and C = 0 on lanes from 64 - S (likewise from lane
+    // 63), while the values fetched from outside are finite (another lane's
+    // row, or 0): every term they enter is multiplied by that exact zero.
     const TC inv = rcp_nr((TC)1 - Cm * A - Ap * C);
     D = C2<TC>{(D.x - Dm.x * A - Dp.x * C) * inv, (D.y - Dm.y * A - Dp.y * C) * inv};
     const TC A2 = -Am * A * inv, C2v = -Cp * C * inv;
