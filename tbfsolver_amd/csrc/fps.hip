@@ -1074,18 +1074,14 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
             ce = c[r];
             de = d[r];
         }
-    // first-row map of the next block
+    // first-row map of the next block (zeros on lane 63, from the wave shift:
+    // no block above it; likewise ym below on lane 0)
     C2<TC> na, dummy;
     TC nb_, ng, dm;
     lane_nbrs<1>(fa.x, lane, dummy.x, na.x);
     lane_nbrs<1>(fa.y, lane, dummy.y, na.y);
     lane_nbrs<1>(fb, lane, dm, nb_);
     lane_nbrs<1>(fg, lane, dm, ng);
-    if (lane == 63) {
-        na = C2<TC>{0, 0};
-        nb_ = 0;
-        ng = 0;
-    }
     // the reduced row, scaled to a unit diagonal
     const TC inv0 = rcp_nr(ae * lg + bee + ce * nb_);
     TC A = ae * lb * inv0;
@@ -1102,7 +1098,6 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
     C2<TC> ym;
     lane_nbrs<1>(y.x, lane, ym.x, dummy.x);
     lane_nbrs<1>(y.y, lane, ym.y, dummy.y);
-    if (lane == 0) ym = C2<TC>{0, 0};
     C2<TC> x[MR];
 #pragma unroll
     for (int r = 0; r < MR; ++r) {
