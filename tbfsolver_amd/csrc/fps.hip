@@ -990,18 +990,19 @@ __device__ __forceinline__ void pcr_step(TC& A, TC& C, C2<TC>& D, int lane)
 // them).  pinned: the singular (0,0) column, solved with x_{ny-1} = 0 and then
 // made volume-weighted mean free (the reference patches U(n,n)=eps instead,
 // :318-322, and removes the mean afterwards, :388-423).
-template <class TC, int MR, class TS>
+// EXACT: ny = 64 MR, so every lane holds MR rows and the row guards fold away.
+template <class TC, int MR, class TS, bool EXACT>
 __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict__ am, const TC* __restrict__ ap,
                          TC lam, bool pinned, const TC* __restrict__ dyf, TC inv_ly)
 {
     const int lane = threadIdx.x & 63;
-    const int m = (ny + 63) / 64;
+    const int m = EXACT ? MR : (ny + 63) / 64;
     TC a[MR], b[MR], c[MR];
     C2<TC> d[MR];
 #pragma unroll
     for (int r = 0; r < MR; ++r) {
         const int row = lane * m + r;
-        if (r < m && row < ny && !(pinned && row == ny - 1)) {
+        if (r < m && (EXACT || row < ny) && !(pinned && row == ny - 1)) {
             a[r] = am[row];
             c[r] = ap[row];
             b[r] = lam - a[r] - c[r];
@@ -1112,7 +1113,7 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
 #pragma unroll
         for (int r = 0; r < MR; ++r) {
             const int row = lane * m + r;
-            if (r < m && row < ny) {
+            if (r < m && (EXACT || row < ny)) {
                 sx += dyf[row] * x[r].x;
                 sy += dyf[row] * x[r].y;
             }
@@ -1132,7 +1133,7 @@ __device__ void tri_wave(C2<TS>* __restrict__ col, int ny, const TC* __restrict_
 #pragma unroll
     for (int r = 0; r < MR; ++r) {
         const int row = lane * m + r;
-        if (r < m && row < ny) col[row] = C2<TS>{(TS)x[r].x, (TS)x[r].y};
+        if (r < m && (EXACT || row < ny)) col[row] = C2<TS>{(TS)x[r].x, (TS)x[r].y};
     }
 }
 
@@ -1148,7 +1149,7 @@ struct TriTables {
     double inv_ly;
 };
 
-template <class T, int MR>
+template <class T, int MR, bool EXACT>
 __global__ void __launch_bounds__(256) k_ysolve(int nh, int ny, int nz, TriTables tt, C2<T>* __restrict__ B)
 {
     const int sys = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1158,9 +1159,9 @@ __global__ void __launch_bounds__(256) k_ysolve(int nh, int ny, int nz, TriTable
     const double lam = tt.lx[m] + tt.lz[kz];
     const bool pinned = (m == 0 && kz == 0);
     if (sizeof(T) == 8 || pinned) {
-        tri_wave<double, MR, T>(col, ny, tt.am, tt.ap, lam, pinned, tt.dyf, tt.inv_ly);
+        tri_wave<double, MR, T, EXACT>(col, ny, tt.am, tt.ap, lam, pinned, tt.dyf, tt.inv_ly);
     } else {
-        tri_wave<float, MR, T>(col, ny, tt.amf, tt.apf, (float)lam, false, tt.dyff, (float)tt.inv_ly);
+        tri_wave<float, MR, T, EXACT>(col, ny, tt.amf, tt.apf, (float)lam, false, tt.dyff, (float)tt.inv_ly);
     }
 }
 
@@ -1465,7 +1466,12 @@ template <class T, int MR>
 static void launch_ysolve(tbfg_fps* h, C2<T>* B, hipStream_t s)
 {
     const int nsys = h->nh * h->nz;
-    hipLaunchKernelGGL((k_ysolve<T, MR>), dim3(ceil_div(nsys, 4)), dim3(256), 0, s, h->nh, h->ny, h->nz, h->tt, B);
+    if (h->ny == 64 * MR)
+        hipLaunchKernelGGL((k_ysolve<T, MR, true>), dim3(ceil_div(nsys, 4)), dim3(256), 0, s, h->nh, h->ny, h->nz,
+                           h->tt, B);
+    else
+        hipLaunchKernelGGL((k_ysolve<T, MR, false>), dim3(ceil_div(nsys, 4)), dim3(256), 0, s, h->nh, h->ny, h->nz,
+                           h->tt, B);
 }
 
 // one spectral solve: in (double) -> p (double), mode 0 write / 1 accumulate
