@@ -39,10 +39,12 @@ def rel(a, b):
 # column, 16 rows per lane of the tridiagonal solve; the last three: lines of
 # 2048, 3072 and 4096 points, one line per workgroup with more than 64 KB of LDS;
 # (32, 256, 16) and (16, 512, 8): 4 and 8 rows per lane of the y solve's form for
-# ny = 64 x rows per lane)
+# ny = 64 x rows per lane; (32, 96, 16) and (32, 160, 16): 2 and 3 rows per lane
+# with padded rows in the last lanes)
 CASES = [((64, 64, 64), 0), ((32, 24, 16), 1), ((96, 32, 48), 1), ((256, 192, 256), 1), ((192, 128, 96), 0),
          ((256, 16, 32), 1), ((32, 16, 256), 0), ((8, 6, 4), 1), ((1024, 8, 768), 0), ((16, 1024, 8), 1),
-         ((2048, 8, 4), 1), ((3072, 4, 2), 0), ((6, 4, 4096), 1), ((32, 256, 16), 1), ((16, 512, 8), 0)]
+         ((2048, 8, 4), 1), ((3072, 4, 2), 0), ((6, 4, 4096), 1), ((32, 256, 16), 1), ((16, 512, 8), 0),
+         ((32, 96, 16), 1), ((32, 160, 16), 1)]
 
 
 @pytest.mark.parametrize("n,stretch", CASES)
