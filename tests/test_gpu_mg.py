@@ -97,6 +97,33 @@ def test_preconditioner_matches_oracle(torch_gpu, n, stretch, nlev):
     assert rel(mean_free(out), mean_free(zr)) < 1e-8
 
 
+@pytest.mark.parametrize("precision,bound", [("fp64", 1e-13), ("mixed", 5e-6), ("mixed-bf16", 5e-2)])
+@pytest.mark.parametrize("n,stretch,nlev", [((32, 32, 32), 0, 3), ((64, 48, 64), 1, 4)])
+def test_vcycle_continuation_preconditioner_matches_oracle(torch_gpu, monkeypatch, precision, bound, n, stretch,
+                                                           nlev):
+    """One application of the preconditioner the mixed solves use by default
+    (coarse_max_iter=0: the V-cycle continues below the coarsest level to a
+    dense bottom) against the oracle's FP64 cascade mode with one cycle, the
+    plane-march sweeps forced at these sizes.  FP64 agrees to round-off
+    (measured 5-7e-16); FP32 arithmetic to a few units of its round-off
+    (5-6e-7); with the finest q and beta stored in bf16 the V-cycle is that of
+    coefficients perturbed by up to 2^-9 per cell (0.8-2.7e-2 measured; the
+    bound catches a wrong index or a dropped term, which move z by O(1))."""
+    monkeypatch.setenv("TBFG_SWEEP_MIN_CELLS", "0")
+    print("case precondition", precision, n, stretch, nlev, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, 6, stretch, 11)
+    zr = O.mg_apply(go, O.params(n_levels=nlev, coarse_mode=1, coarse_cycles=1), beta, q)
+    s = T.PCG(g, nLevels=nlev, precision=precision, coarse_max_iter=0)
+    z = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+    s.precondition(to_dev(beta, torch_gpu), to_dev(q, torch_gpu), z)
+    torch_gpu.cuda.synchronize()
+    s.close()
+    out = to_host(z)
+    e = rel(mean_free(out), mean_free(zr))
+    print("  rel error vs FP64 V-cycle", e, flush=True)
+    assert e < bound
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
 @pytest.mark.parametrize("n,nlev", [((128, 96, 64), 4), ((96, 32, 40), 3)])
 def test_full_sweep_kernel_matches_half_sweeps(torch_gpu, monkeypatch, precision, tol, n, nlev):
