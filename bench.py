@@ -386,6 +386,8 @@ def main():
     ap.add_argument("--no-fp32-storage", action="store_true", help="skip the solve with FP32 finest-level storage")
     ap.add_argument("--fast-only", action="store_true",
                     help="time only the FAST_MODE FFT solve on the headline grid (for kernel profiles)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="time only the FP64 PCG+MG solve on the headline grid (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the pressure solve of one channel instance runs on a single "
@@ -416,6 +418,16 @@ def main():
     if args.fast_only:
         # (the device copy is the calibration dispatch of the traffic counters)
         print(json.dumps({"fast_solver": fast_leg(torch, T, g, drho, q, reps=args.steps),
+                          "copy_kernel_gbs": round(copy_peak(torch), 1)}), flush=True)
+        return
+    if args.fp64_only:
+        # the FP64 leg's solver as the full run builds it, its timed solves alone
+        fp64 = T.PCG(g, nLevels=4, nPreSweep=2, nPostSweep=2, tolPCG=1e-8, maxIterPCG=200, precision="fp64",
+                     flexible=1, coarse_max_iter=0)
+        total64, ms64, it64, _ = timed_solves(torch, fp64, torch.zeros_like(drho), beta, q, args.steps, args.warmup)
+        print(json.dumps({"fp64": {"solves_per_s": round(args.steps / total64, 3),
+                                   "p50_ms": round(float(np.percentile(ms64, 50)), 4),
+                                   "iterations": int(np.median(it64))},
                           "copy_kernel_gbs": round(copy_peak(torch), 1)}), flush=True)
         return
 
