@@ -44,7 +44,8 @@ def rel(a, b):
 CASES = [((64, 64, 64), 0), ((32, 24, 16), 1), ((96, 32, 48), 1), ((256, 192, 256), 1), ((192, 128, 96), 0),
          ((256, 16, 32), 1), ((32, 16, 256), 0), ((8, 6, 4), 1), ((1024, 8, 768), 0), ((16, 1024, 8), 1),
          ((2048, 8, 4), 1), ((3072, 4, 2), 0), ((6, 4, 4096), 1), ((32, 256, 16), 1), ((16, 512, 8), 0),
-         ((32, 96, 16), 1), ((32, 160, 16), 1)]
+         ((32, 96, 16), 1), ((32, 160, 16), 1), ((384, 4, 144), 0), ((72, 36, 96), 1),
+         ((96, 4, 576), 1), ((12, 12, 36), 1)]
 
 
 @pytest.mark.parametrize("n,stretch", CASES)
