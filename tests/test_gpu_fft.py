@@ -80,7 +80,7 @@ def test_fft_solve_fp64(torch_gpu, n, stretch):
 
 
 @pytest.mark.parametrize("n,stretch", [((64, 64, 64), 0), ((96, 32, 48), 1), ((256, 16, 32), 1), ((256, 192, 256), 1),
-                                       ((2048, 8, 4), 1)])
+                                       ((2048, 8, 4), 1), ((72, 36, 96), 1), ((384, 4, 576), 0)])
 def test_fft_solve_mixed(torch_gpu, n, stretch):
     print("case fft mixed", n, stretch, flush=True)
     g, go = grids(n, stretch)
