@@ -807,6 +807,34 @@ def test_mg_tolerance_loop_matches_oracle(torch_gpu, mg_tol):
         s.close()
 
 
+def test_reference_channel_parameters_match_oracle(torch_gpu):
+    """The reference channel case's solver parameters (specs/pcg_solver:
+    nLevels 5, nPreSweep 2, nPostSweep 4, tolMG 0.4, maxIterMG 100, tolPCG
+    1e-10, Fletcher-Reeves) on a stretched 128x96x128 bubble case, FP64: the
+    same outer iterations (within one), V-cycle counts within one per ten and
+    the same pressure as the oracle's solve with the exact coarsest level
+    (measured: 28 iterations and 39 V-cycles on both, pressures 1e-15 apart)."""
+    n, stretch, nb = (128, 96, 128), 1, 18
+    print("case reference channel parameters", n, flush=True)
+    g, go, rho, _, q, beta = bubble_case(n, nb, stretch, 31)
+    ref, it, res, rc = O.pcg_solve(go, O.params(n_levels=5, n_pre=2, n_post=4, tol=1e-10, max_iter=100,
+                                                coarse_mode=0, flexible=0, mg_tol=0.4, mg_max_iter=100), beta, q)
+    ref_total, ref_max, ref_last = O.last_mg_cycles(with_last=True)
+    ref_total -= ref_last  # the reference's application after the final update is not made here
+    s = T.PCG(g, nLevels=5, nPreSweep=2, nPostSweep=4, tolPCG=1e-10, maxIterPCG=100, flexible=0, tolMG=0.4,
+              maxIterMG=100)
+    psi = torch_gpu.zeros_like(to_dev(q, torch_gpu))
+    s.solvePCG(psi, to_dev(beta, torch_gpu), to_dev(q, torch_gpu))
+    total, most = s.mg_stats()
+    e = rel(mean_free(to_host(psi)), mean_free(ref))
+    print("  gpu", s.iterations, (total, most), "oracle", it, (ref_total, ref_max), "rel L2", e, flush=True)
+    assert rc == 0 and s.converged
+    assert abs(s.iterations - it) <= 1
+    assert abs(total - ref_total) <= max(1, 0.1 * ref_total) and most <= ref_max + 1
+    assert e < 1e-11
+    s.close()
+
+
 def test_mg_tolerance_loop_cap_and_fixed_cycles(torch_gpu, monkeypatch):
     """maxIterMG caps the loop: a target no residual reaches with a cap of two
     equals two fixed cycles per application; the fixed form reports its cycle
