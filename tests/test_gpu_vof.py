@@ -12,7 +12,7 @@ from conftest import to_dev, to_host
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n,stretch", [((48, 40, 32), 1), ((32, 16, 24), 0)])
+@pytest.mark.parametrize("n,stretch", [((48, 40, 32), 1), ((32, 16, 24), 0), ((256, 192, 256), 1)])
 def test_material_props_match_oracle(torch_gpu, n, stretch):
     print("case material props", n, stretch, flush=True)
     g = T.channel_grid(n, stretch_y=stretch, gamma=1.5)
