@@ -81,6 +81,38 @@ def test_fields_match_oracle(torch_gpu):
     assert np.isclose(md, np.abs(O.divergence(go, rx, ry, rz)).max(), rtol=1e-12)
 
 
+def test_fields_match_oracle_full_size(torch_gpu):
+    """The same field kernels at 256x192x256 (stretched y, 72 bubbles): the
+    divergence, beta, the MG projection and the continuity error against the
+    oracle, with the bounds of the small case."""
+    print("case fields full size", flush=True)
+    n = (256, 192, 256)
+    g, go = grids(n, 1)
+    lr = np.random.default_rng(5)
+    rho, _ = F.make_bubble_density(g, 72, 0.3, 1002, nref=10)
+    ux, uy, uz = F.make_velocity(g, 1002)
+    psi = lr.standard_normal(n)
+    adt = 1e-3
+    dux, duy, duz = (to_dev(a, torch_gpu) for a in (ux, uy, uz))
+    drho, dpsi = to_dev(rho, torch_gpu), to_dev(psi, torch_gpu)
+    div = torch_gpu.zeros_like(drho)
+    F.computeSource(g, dux, duy, duz, div)
+    beta = torch_gpu.zeros_like(drho)
+    F.computeBeta(g, adt, drho, beta)
+    torch_gpu.cuda.synchronize()
+    d_ref = O.divergence(go, ux, uy, uz)
+    assert np.abs(to_host(div) - d_ref).max() <= 1e-13 * np.abs(d_ref).max()
+    assert np.array_equal(to_host(beta), O.beta_from_rho(go, adt, rho))
+    del div, beta
+    F.makeVelocityDivFree(g, adt, drho, dpsi, dux, duy, duz)
+    torch_gpu.cuda.synchronize()
+    rx, ry, rz = O.project_mg(go, adt, rho, psi, ux, uy, uz)
+    for a, b in ((dux, rx), (duy, ry), (duz, rz)):
+        assert np.abs(to_host(a) - b).max() <= 1e-14 * np.abs(b).max()
+    md = F.max_divergence(g, dux, duy, duz)
+    assert np.isclose(md, np.abs(O.divergence(go, rx, ry, rz)).max(), rtol=1e-12)
+
+
 @pytest.mark.parametrize("n,stretch,nlev", [((32, 32, 32), 0, 3), ((64, 48, 64), 1, 4)])
 def test_preconditioner_matches_oracle(torch_gpu, n, stretch, nlev):
     print("case precondition", n, stretch, nlev, flush=True)
